@@ -1,0 +1,148 @@
+/*
+ * janus_amd -- C ABI of the MI355X (gfx950) Lippmann-Schwinger hot path.
+ *
+ * Plain C: opaque handles, device pointers, sizes and int status codes.  No
+ * torch types cross this boundary.  Every compute call is stream ordered on
+ * the hipStream_t passed as `void *stream` (NULL = the null stream).  All
+ * device arrays are FP64, C order, in janus's layouts:
+ *
+ *   real field      (n0, n1[, n2], ncomp)             ncomp = sym or 1
+ *   spectrum        (n0, n1[, n2'], ncomp)            n_last' = 2 (n_last / 2 + 1)
+ *                   i.e. per frequency point: Re of the ncomp components,
+ *                   then Im of the ncomp components (janus/green.pyx:119-120,
+ *                   280-282; for ncomp = 1 the plain interleaved layout of
+ *                   janus/fft/serial/_serial_fft.pyx:57-74, 181-200).
+ *
+ * sym = 3 (2D, Mandel order xx, yy, sqrt2 xy) or 6 (3D, xx, yy, zz, sqrt2 yz,
+ * sqrt2 zx, sqrt2 xy) -- janus/mandelvoigt.py:26-40.
+ *
+ * Status codes: JX_OK (0) or a negative JX_E* value; jx_last_error() returns
+ * a thread-local message for the last failure of the calling thread.
+ * Handles are not thread safe (they own scratch memory), as in janus.
+ */
+#ifndef JANUS_AMD_H
+#define JANUS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JX_OK 0
+#define JX_EINVAL (-1)       /* invalid argument (maps to ValueError) */
+#define JX_EHIP (-2)         /* HIP runtime error */
+#define JX_ENOMEM (-3)       /* device allocation failed */
+#define JX_EUNSUPPORTED (-4) /* size or mode this build does not handle */
+
+#define JX_GREEN_TRUNCATED 0
+#define JX_GREEN_FILTERED 1
+
+#define JX_SOLVE_CONVERGED 1 /* residual < tol */
+#define JX_SOLVE_MAXITER 2   /* max_iter iterations without reaching tol */
+#define JX_SOLVE_NONFINITE 3 /* residual became NaN or Inf */
+
+typedef struct jx_fft jx_fft;
+typedef struct jx_green jx_green;
+typedef struct jx_local jx_local;
+typedef struct jx_solver jx_solver;
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *jx_last_error(void);
+/* ABI version of this library. */
+int jx_version(void);
+
+/* ---------------------------------------------------------------------
+ * Real FFT.  Replaces janus.fft.serial.create_real and _RealFFT2D/3D
+ * (janus/fft/serial/_serial_fft.pyx:20-53 create_real/create_real_2D/3D,
+ * :57-74 and :181-200 shapes, :152-163 r2c, :165-177 c2r).
+ * r2c is unnormalized, c2r scales by 1/(n0 n1 [n2]).  Inputs are preserved.
+ * ------------------------------------------------------------------- */
+int jx_fft_create(int dim, const int64_t *shape, jx_fft **out);
+int jx_fft_destroy(jx_fft *fft);
+/* r: (n0, n1[, n2]) real; c: (n0, n1[, ...], 2 (n_last/2 + 1)). */
+int jx_fft_r2c(jx_fft *fft, const double *r, double *c, void *stream);
+int jx_fft_c2r(jx_fft *fft, const double *c, double *r, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Discrete Green operator for strains of an isotropic reference material
+ * (g0, nu0).  Replaces janus.green.truncated / filtered
+ * (janus/green.pyx:22-36) with DiscreteGreenOperator2D/3D.apply
+ * (:182-216, :353-390, truncated :453-495, :531-580, filtered
+ * :607-694, :730-973) and the continuous operator of
+ * janus/material/elastic/linear/isotropic.pyx:133-156, 173-200, 242-308.
+ * kind = JX_GREEN_TRUNCATED or JX_GREEN_FILTERED; h > 0 is the cell size.
+ * ------------------------------------------------------------------- */
+int jx_green_create(int dim, const int64_t *shape, double h, int kind,
+                    double g0, double nu0, jx_green **out);
+int jx_green_destroy(jx_green *green);
+/* y = Gamma0[x]; x, y: (n0, n1[, n2], sym) device arrays; x == y allowed. */
+int jx_green_apply(jx_green *green, const double *x, double *y, void *stream);
+/* Host-side per-frequency matrix Gamma-hat(b), b in [0, n_i), written to
+ * out[sym * sym] (row major).  Replaces set_frequency + to_memoryview
+ * (janus/green.pyx:146-171, 312-346). */
+int jx_green_matrix_at(jx_green *green, const int32_t *b, double *out);
+/* Continuous Gamma0(k) of the material (g, nu) in dimension dim, written to
+ * out[sym * sym]; replaces _GreenOperatorForStrains*.set_frequency +
+ * to_memoryview (isotropic.pyx:173-200, 242-308, 213-224, 333-371). */
+int jx_gamma0_at(int dim, double g, double nu, const double *k, double *out);
+
+/* ---------------------------------------------------------------------
+ * Block-diagonal isotropic local operator x -> (sph J + dev K) : x, voxel
+ * by voxel.  Replaces janus.operators.BlockDiagonalOperator2D/3D whose loc
+ * entries are isotropic_4(sph, dev, dim) (janus/operators.pyx:53-67,
+ * 292-299, 315-324, 352-365, 617-716).  The handle keeps the device
+ * pointers it is given (a shallow reference, as operators.pyx:76-77 keeps
+ * loc); the caller keeps them alive.
+ * ------------------------------------------------------------------- */
+/* phase: (n0, n1[, n2]) uint8 device array; table: host array of nphase
+ * (sph, dev) pairs, nphase in [1, 256]. */
+int jx_local_create_phases(int dim, const int64_t *shape, const uint8_t *phase,
+                           const double *table, int nphase, jx_local **out);
+/* sph, dev: (n0, n1[, n2]) device arrays of per-voxel coefficients. */
+int jx_local_create_fields(int dim, const int64_t *shape, const double *sph,
+                           const double *dev, jx_local **out);
+int jx_local_destroy(jx_local *local);
+/* y = local : x; x, y: (n0, n1[, n2], sym); x == y allowed. */
+int jx_local_apply(jx_local *local, const double *x, double *y, void *stream);
+/* mean over the grid of local : x (out_tau[sym]) and of x (out_eps[sym]),
+ * written to host memory (synchronizes the stream).  Either may be NULL. */
+int jx_local_means(jx_local *local, const double *x, double *out_tau,
+                   double *out_eps, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Basic (Moulinec-Suquet) fixed-point solver, the loop of
+ * sphinx/tutorials/square_basic/square_basic.py:83-92:
+ *     eps <- E + rhs - Gamma0[local : eps]
+ * with residual ||eps_{k+1} - eps_k|| * res_scale; the default res_scale
+ * (pass 0) is 1 / (sqrt(N) ||E||) as in square_basic.py:86, 91.  Stops
+ * after the first iteration whose residual is < tol, or after max_iter.
+ * The fused passes keep Gamma0 and the local operator out of memory.
+ * ------------------------------------------------------------------- */
+int jx_solver_create(jx_green *green, jx_local *local, jx_solver **out);
+int jx_solver_destroy(jx_solver *solver);
+/* eps: device field (n0, n1[, n2], sym), holds the initial iterate on entry
+ * (e.g. E everywhere) and the last iterate on return.  E: host sym vector.
+ * rhs: optional device field added to E (NULL for none).  res_hist: host
+ * array of max_iter doubles (may be NULL).  iters, status: host outputs.
+ * Synchronizes the stream. */
+int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
+                    const double *rhs, double tol, int max_iter,
+                    double res_scale, double *res_hist, int *iters,
+                    int *status, void *stream);
+
+/* Measurement helper (no janus counterpart): runs `iters` fixed-point
+ * iterations exactly as jx_solver_solve does (tol ignored) and records a
+ * hipEvent before and after every pass on `stream`.  ms_out[6] receives the
+ * average duration in milliseconds of the passes: 0 last-axis r2c with the
+ * local operator, 1 axis-1 forward (3D), 2 axis-0 forward + Gamma-hat +
+ * inverse, 3 axis-1 inverse (3D), 4 last-axis c2r with the fixed-point
+ * update, 5 residual reduction.  Synchronizes the stream. */
+int jx_solver_profile(jx_solver *solver, double *eps, const double *E,
+                      int iters, double *ms_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JANUS_AMD_H */

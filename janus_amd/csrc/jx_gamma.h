@@ -1,0 +1,144 @@
+// Green operator for strains of an isotropic reference material, Mandel
+// notation, computed from the wave vector (never stored).  Shared by the
+// device kernels and the host-side per-frequency API.
+//
+// Gamma0(k) follows janus/material/elastic/linear/isotropic.pyx:133-156
+// (daux1 = 1/g, daux2 = 1/(2 g (1 - nu)), daux3 = 1/(4 g), daux4 = 1/(2 g)),
+// :173-200 (2D) and :242-308 (3D), keeping janus's operation order; it is
+// exactly zero at k = 0.  The coefficients are stored as the upper triangle,
+// row major: 2D g00 g01 g02 g11 g12 g22, 3D g00 .. g05 g11 .. g15 g22 .. g55.
+#pragma once
+
+#include "jx_internal.h"
+
+namespace jx {
+
+#define JX_HD __host__ __device__ __forceinline__
+
+constexpr double kSqrt2 = 1.41421356237309504880;
+
+JX_HD void gamma0_2d(double k0, double k1, double daux1, double daux2,
+                     double daux3, double daux4, double g[6]) {
+  double k0k0 = k0 * k0;
+  double k1k1 = k1 * k1;
+  if (k0k0 + k1k1 == 0.) {
+    for (int i = 0; i < 6; ++i) g[i] = 0.;
+    return;
+  }
+  double s = 1.0 / (k0k0 + k1k1);
+  k0k0 *= s;
+  k1k1 *= s;
+  double k0k1 = s * k0 * k1;
+  g[0] = k0k0 * (daux1 - daux2 * k0k0);                      // g00
+  g[3] = k1k1 * (daux1 - daux2 * k1k1);                      // g11
+  double dummy = -daux2 * k0k0 * k1k1;
+  g[1] = dummy;                                              // g01
+  g[5] = 2 * (daux3 + dummy);                                // g22
+  g[2] = kSqrt2 * k0k1 * (daux4 - daux2 * k0k0);             // g02
+  g[4] = kSqrt2 * k0k1 * (daux4 - daux2 * k1k1);             // g12
+}
+
+JX_HD void gamma0_3d(double k0, double k1, double k2, double daux1,
+                     double daux2, double daux3, double daux4, double g[21]) {
+  double k0k0 = k0 * k0;
+  double k1k1 = k1 * k1;
+  double k2k2 = k2 * k2;
+  if (k0k0 + k1k1 + k2k2 == 0.) {
+    for (int i = 0; i < 21; ++i) g[i] = 0.;
+    return;
+  }
+  double s = 1.0 / (k0k0 + k1k1 + k2k2);
+  k0k0 *= s;
+  k1k1 *= s;
+  k2k2 *= s;
+  double k1k2 = s * k1 * k2;
+  double k2k0 = s * k2 * k0;
+  double k0k1 = s * k0 * k1;
+  // index map of the upper triangle
+  // 0:g00 1:g01 2:g02 3:g03 4:g04 5:g05 6:g11 7:g12 8:g13 9:g14 10:g15
+  // 11:g22 12:g23 13:g24 14:g25 15:g33 16:g34 17:g35 18:g44 19:g45 20:g55
+  g[0] = k0k0 * (daux1 - daux2 * k0k0);
+  g[6] = k1k1 * (daux1 - daux2 * k1k1);
+  g[11] = k2k2 * (daux1 - daux2 * k2k2);
+  g[15] = 2. * (daux3 * (k1k1 + k2k2) - daux2 * k1k1 * k2k2);
+  g[18] = 2. * (daux3 * (k2k2 + k0k0) - daux2 * k2k2 * k0k0);
+  g[20] = 2. * (daux3 * (k0k0 + k1k1) - daux2 * k0k0 * k1k1);
+  g[1] = -daux2 * k0k0 * k1k1;
+  g[2] = -daux2 * k0k0 * k2k2;
+  g[3] = -kSqrt2 * daux2 * k0k0 * k1k2;
+  g[4] = kSqrt2 * k2k0 * (daux4 - daux2 * k0k0);
+  g[5] = kSqrt2 * k0k1 * (daux4 - daux2 * k0k0);
+  g[7] = -daux2 * k1k1 * k2k2;
+  g[8] = kSqrt2 * k1k2 * (daux4 - daux2 * k1k1);
+  g[9] = -kSqrt2 * daux2 * k1k1 * k2k0;
+  g[10] = kSqrt2 * k0k1 * (daux4 - daux2 * k1k1);
+  g[12] = kSqrt2 * k1k2 * (daux4 - daux2 * k2k2);
+  g[13] = kSqrt2 * k2k0 * (daux4 - daux2 * k2k2);
+  g[14] = -kSqrt2 * daux2 * k2k2 * k0k1;
+  g[16] = 2 * k0k1 * (daux3 - daux2 * k2k2);
+  g[17] = 2 * k2k0 * (daux3 - daux2 * k1k1);
+  g[19] = 2 * k1k2 * (daux3 - daux2 * k0k0);
+}
+
+// Upper-triangle position of (i, j), i <= j, for sym = 3 or 6.
+JX_HD int tri_index(int sym, int i, int j) {
+  return i * sym - (i * (i - 1)) / 2 + (j - i);
+}
+
+// Discrete Gamma-hat at integer frequency b of the grid, from the per-axis
+// tables.  Truncated: Gamma0 of the aliased wave vector (green.pyx:432-451,
+// 505-529).  Filtered: sum over the 2^dim alias combinations of
+// prod_i cos^2(h k_i / 4) Gamma0(k), accumulated in janus's order -- axis 0
+// slowest, the "minus" alias first (green.pyx:607-694, 730-910).
+template <int DIM>
+JX_HD void gamma_hat(const GreenDev &G, const int *b, double *g) {
+  constexpr int NC = DIM == 2 ? 6 : 21;
+  if (G.kind == JX_GREEN_TRUNCATED) {
+    if (DIM == 2)
+      gamma0_2d(G.ax[0].kt[b[0]], G.ax[1].kt[b[1]], G.daux1, G.daux2, G.daux3,
+                G.daux4, g);
+    else
+      gamma0_3d(G.ax[0].kt[b[0]], G.ax[1].kt[b[1]], G.ax[2].kt[b[2]], G.daux1,
+                G.daux2, G.daux3, G.daux4, g);
+    return;
+  }
+  double km[DIM], kp[DIM], wm[DIM], wp[DIM];
+  for (int i = 0; i < DIM; ++i) {
+    km[i] = G.ax[i].km[b[i]];
+    kp[i] = G.ax[i].kp[b[i]];
+    wm[i] = G.ax[i].wm[b[i]];
+    wp[i] = G.ax[i].wp[b[i]];
+  }
+  for (int i = 0; i < NC; ++i) g[i] = 0.;
+  double t[NC];
+  for (int c = 0; c < (1 << DIM); ++c) {
+    double k[DIM];
+    double w = 1.0;
+    for (int i = 0; i < DIM; ++i) {
+      int plus = (c >> (DIM - 1 - i)) & 1;
+      k[i] = plus ? kp[i] : km[i];
+      double wi = plus ? wp[i] : wm[i];
+      w = (i == 0) ? wi : w * wi;
+    }
+    if (DIM == 2)
+      gamma0_2d(k[0], k[1], G.daux1, G.daux2, G.daux3, G.daux4, t);
+    else
+      gamma0_3d(k[0], k[1], k[DIM - 1], G.daux1, G.daux2, G.daux3, G.daux4, t);
+    for (int i = 0; i < NC; ++i) g[i] += w * t[i];
+  }
+}
+
+// y = G x for the symmetric matrix stored as an upper triangle.
+template <int SYM>
+JX_HD void sym_matvec(const double *g, const double *x, double *y) {
+  for (int i = 0; i < SYM; ++i) {
+    double acc = 0.;
+    for (int j = 0; j < SYM; ++j) {
+      int a = i < j ? i : j, bb = i < j ? j : i;
+      acc += g[tri_index(SYM, a, bb)] * x[j];
+    }
+    y[i] = acc;
+  }
+}
+
+}  // namespace jx

@@ -1,0 +1,128 @@
+// Internal declarations shared by the janus_amd HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/janus_amd.h"
+
+namespace jx {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define JX_HIP(call)                                  \
+  do {                                                \
+    hipError_t jx_e_ = (call);                        \
+    if (jx_e_ != hipSuccess) return jx::hip_fail(jx_e_, #call); \
+  } while (0)
+
+// ---------------------------------------------------------------- limits
+constexpr int kMaxStages = 24;   // radix stages per FFT length
+constexpr int kMaxDim = 3;
+constexpr int kThreads = 256;    // threads per workgroup of the FFT kernels
+
+// One FFT length: radix sequence and twiddle table W[m] = exp(-2 pi i m / L).
+struct FftLen {
+  int L = 0;
+  int nstages = 0;
+  int radix[kMaxStages] = {};
+  const double2 *W = nullptr;  // device, L entries
+};
+
+// Per-axis tables of the discrete Green operators (device pointers).
+//   kt: truncated wave number  s (b - n) if 2b > n else s b   (green.pyx:432-445)
+//   km, kp: filtered aliases    s (b - n), s b                 (green.pyx:740-800)
+//   wm, wp: filtered weights    cos^2(h k / 4) of km, kp
+struct AxisTab {
+  const double *kt = nullptr, *km = nullptr, *kp = nullptr, *wm = nullptr,
+               *wp = nullptr;
+};
+
+struct GreenDev {
+  int dim = 0;
+  int kind = 0;  // JX_GREEN_TRUNCATED / JX_GREEN_FILTERED
+  double daux1 = 0, daux2 = 0, daux3 = 0, daux4 = 0;
+  AxisTab ax[kMaxDim];
+};
+
+// Local operator modes.
+enum LocalMode { LOCAL_NONE = 0, LOCAL_PHASES = 1, LOCAL_FIELDS = 2 };
+
+struct LocalDev {
+  int mode = LOCAL_NONE;
+  const uint8_t *phase = nullptr;
+  const double *table = nullptr;  // device, 2 * nphase (sph, dev) pairs
+  const double *sph = nullptr;
+  const double *dev = nullptr;
+};
+
+// Device-side state of a running solve.
+struct SolveState {
+  int iter;      // completed iterations
+  int done;      // 0 running, else JX_SOLVE_* status
+  double tol;
+  double res_scale;
+  int max_iter;
+  int pad;
+};
+
+// Geometry of a grid with ncomp components per point.
+struct Grid {
+  int dim = 0;
+  int64_t n[kMaxDim] = {1, 1, 1};
+  int64_t N = 0;      // voxels
+  int64_t rows = 0;   // N / n_last
+  int64_t L = 0;      // n_last
+  int64_t nh = 0;     // n_last / 2 + 1
+  int64_t Nh = 0;     // rows * nh spectral points
+};
+
+Grid make_grid(int dim, const int64_t *shape);
+
+// Twiddle / plan cache (per device, shared by all handles).
+int get_fft_len(int L, FftLen *out);
+
+// LDS limit per workgroup on the current device.
+int lds_limit_bytes();
+
+// ------------------------------------------------------------- launchers
+// Forward transform of the last axis of a real field with optional local
+// operator, writing the half spectrum.
+int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
+                        const LocalDev &loc, const SolveState *st,
+                        hipStream_t s);
+// Complex transform along axis `axis` (< dim - 1) of a spectrum; dir = -1
+// forward, +1 inverse, 0 = forward + Gamma-hat + inverse (axis must be 0).
+int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
+                double *out, const GreenDev *green, const SolveState *st,
+                hipStream_t s);
+// Inverse transform of the last axis (scaled by 1/N).  When E != nullptr
+// the fixed-point update eps <- E + rhs - y is applied in place on y and
+// per-block partial sums of (eps_new - eps_old)^2 go to partials.
+struct RowsInverseLs {
+  const double *rhs = nullptr;  // optional field
+  double E[6] = {};
+  double *partials = nullptr;
+  bool enabled = false;
+};
+int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
+                        const RowsInverseLs &ls, const SolveState *st,
+                        int *nblocks_out, hipStream_t s);
+int rows_inverse_nblocks(const Grid &g, int ncomp);
+int launch_finalize(const double *partials, int nblocks, SolveState *st,
+                    double *res_hist, hipStream_t s);
+int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
+                       const double *x, double *y, hipStream_t s);
+int launch_local_sums(const Grid &g, int ncomp, const LocalDev &loc,
+                      const double *x, double *partials, int *nblocks,
+                      hipStream_t s);
+int launch_reduce_columns(const double *partials, int nblocks, int ncol,
+                          double *out, hipStream_t s);
+
+}  // namespace jx

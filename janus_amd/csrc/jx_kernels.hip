@@ -1,0 +1,651 @@
+// Device kernels of the Lippmann-Schwinger hot path and their launchers.
+//
+// Data layouts (FP64, C order):
+//   real field  (rows, L, C)        rows = n0 [* n1], L = n_last, C comps
+//   spectrum    (rows, nh, 2, C)    nh = L / 2 + 1; Re of the C comps, then
+//                                   Im -- janus's dft_x layout
+//                                   (janus/green.pyx:119-120, 280-282)
+//
+// Pass structure of one application of Gamma0 (3D; 2D drops pass B/D):
+//   A  rows_forward   last-axis r2c (+ optional local operator)
+//   B  axis(1, fwd)
+//   C  axis(0, fwd -> Gamma-hat -> inv)
+//   D  axis(1, inv)
+//   E  rows_inverse   last-axis c2r, 1/N (+ optional fixed-point update)
+// Real sequences are paired two by two as the real and imaginary parts of
+// one complex sequence, which is then transformed once.
+
+#include <cmath>
+
+#include "jx_fft_lds.h"
+#include "jx_gamma.h"
+#include "jx_internal.h"
+
+namespace jx {
+
+// LDS row stride of a sequence: one complex pad for even lengths breaks the
+// power-of-two strides between sequences.
+static inline int lds_stride(int L) { return (L % 2 == 0) ? L + 1 : L; }
+
+// ------------------------------------------------------------ local operator
+// (sph J + dev K) : e in Mandel notation, janus/operators.pyx:315-324 (2D)
+// and :352-365 (3D), tr = (sph - dev) / dim (:299).  Contraction into FMA is
+// disabled so that the result is the one of the separate products and sums.
+template <int C>
+__device__ __forceinline__ void iso4(double tr, double dev, const double *e,
+                                     double *t) {
+#pragma clang fp contract(off)
+  if (C == 3) {
+    double aux = tr * (e[0] + e[1]);
+    t[0] = aux + dev * e[0];
+    t[1] = aux + dev * e[1];
+    t[2] = dev * e[2];
+  } else {
+    double aux = tr * (e[0] + e[1] + e[2]);
+    t[0] = aux + dev * e[0];
+    t[1] = aux + dev * e[1];
+    t[2] = aux + dev * e[2];
+    t[3] = dev * e[3];
+    t[4] = dev * e[4];
+    t[5] = dev * e[5];
+  }
+}
+
+// Phase tables hold (tr, dev) per phase; field mode computes tr per voxel.
+template <int C>
+__device__ __forceinline__ void local_voxel(const LocalDev &loc, int64_t vox,
+                                            const double *e, double *t) {
+  double tr, dev;
+  if (loc.mode == LOCAL_PHASES) {
+    int p = loc.phase[vox];
+    tr = loc.table[2 * p];
+    dev = loc.table[2 * p + 1];
+  } else {
+    double sph = loc.sph[vox];
+    dev = loc.dev[vox];
+    tr = (sph - dev) / (C == 3 ? 2.0 : 3.0);
+  }
+  iso4<C>(tr, dev, e, t);
+}
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum (fixed order); result valid in thread 0.
+__device__ double block_sum(double v, double *red) {
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += red[w];
+  }
+  return s;
+}
+
+struct RowsArgs {
+  int64_t nrows;
+  int L, nh, Ls, RB;
+  double scale;
+};
+
+__device__ __forceinline__ bool solve_done(const SolveState *st) {
+  return st != nullptr && *((volatile const int *)&st->done) != 0;
+}
+
+// ------------------------------------------------------------ pass A
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_rows_forward(const double *__restrict__ x, double *__restrict__ spec,
+                   RowsArgs a, FftLen P, LocalDev loc, const SolveState *st) {
+  if (solve_done(st)) return;
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t row0 = (int64_t)blockIdx.x * a.RB;
+  const int rb = (int)min((int64_t)a.RB, a.nrows - row0);
+  const int Q = rb * C;
+  const int nseq = (Q + 1) / 2;
+  const int nseq_max = (a.RB * C + 1) / 2;
+  double2 *bufA = smem;
+  double2 *bufB = smem + (size_t)nseq_max * a.Ls;
+  const int L = a.L;
+
+  for (int v = tid; v < rb * L; v += nth) {
+    int r = v / L, j = v - r * L;
+    int64_t vox = (row0 + r) * L + j;
+    double e[C], t[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
+    if (loc.mode != LOCAL_NONE) {
+      local_voxel<C>(loc, vox, e, t);
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = e[c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      double *slot = (double *)&bufA[(q >> 1) * a.Ls + j];
+      slot[q & 1] = t[c];
+    }
+    if ((Q & 1) && r == rb - 1) bufA[((Q - 1) >> 1) * a.Ls + j].y = 0.;
+  }
+  __syncthreads();
+  double2 *res = lds_fft<-1>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+
+  const int nh = a.nh;
+  for (int v = tid; v < rb * nh * C; v += nth) {
+    int c = v % C;
+    int rest = v / C;
+    int k = rest % nh;
+    int r = rest / nh;
+    int q = r * C + c;
+    const double2 *Z = res + (q >> 1) * a.Ls;
+    double2 zk = Z[k];
+    double2 zm = Z[k == 0 ? 0 : L - k];
+    double re, im;
+    if (!(q & 1)) {
+      re = 0.5 * (zk.x + zm.x);
+      im = 0.5 * (zk.y - zm.y);
+    } else {
+      re = 0.5 * (zk.y + zm.y);
+      im = -0.5 * (zk.x - zm.x);
+    }
+    int64_t pt = (row0 + r) * nh + k;
+    spec[pt * 2 * C + c] = re;
+    spec[pt * 2 * C + C + c] = im;
+  }
+}
+
+// ------------------------------------------------------------ passes B, C, D
+struct AxisArgs {
+  int L, Ls, T, ntiles;
+  int64_t stride_ax, n_outer, stride_outer, n_inner;
+  int nh;  // last-axis half length, decodes the inner index in 3D pass C
+};
+
+template <int C, int MODE>
+__global__ void __launch_bounds__(kThreads)
+    k_axis(const double *in, double *out, AxisArgs a, FftLen P, GreenDev G,
+           const SolveState *st) {
+  if (solve_done(st)) return;
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t o = blockIdx.x / a.ntiles;
+  const int64_t m0 = (int64_t)(blockIdx.x % a.ntiles) * a.T;
+  const int tc = (int)min((int64_t)a.T, a.n_inner - m0);
+  const int W2 = tc * 2 * C;
+  const int nseq = tc * C;
+  const int64_t base = o * a.stride_outer + m0 * 2 * C;
+  double2 *bufA = smem;
+  double2 *bufB = smem + (size_t)a.T * C * a.Ls;
+  const int L = a.L;
+
+  for (int v = tid; v < L * W2; v += nth) {
+    int e = v / W2;
+    int w = v - e * W2;
+    int t = w / (2 * C);
+    int rem = w - t * 2 * C;
+    int part = rem >= C;
+    int c = rem - part * C;
+    double val = in[base + (int64_t)e * a.stride_ax + w];
+    ((double *)&bufA[(t * C + c) * a.Ls + e])[part] = val;
+  }
+  __syncthreads();
+
+  double2 *res;
+  if (MODE == 0) {
+    res = lds_fft<-1>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+    if constexpr (C == 3 || C == 6) {
+      constexpr int DIM = C == 3 ? 2 : 3;
+      constexpr int NC = C == 3 ? 6 : 21;
+      for (int v = tid; v < tc * L; v += nth) {
+        int t = v / L, e = v - t * L;
+        int64_t m = m0 + t;
+        int b[3];
+        b[0] = e;
+        if (DIM == 2) {
+          b[1] = (int)m;
+        } else {
+          b[1] = (int)(m / a.nh);
+          b[2] = (int)(m - (int64_t)b[1] * a.nh);
+        }
+        double g[NC];
+        gamma_hat<DIM>(G, b, g);
+        double xr[C], xi[C], yr[C], yi[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          double2 z = res[(t * C + c) * a.Ls + e];
+          xr[c] = z.x;
+          xi[c] = z.y;
+        }
+        sym_matvec<C>(g, xr, yr);
+        sym_matvec<C>(g, xi, yi);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          res[(t * C + c) * a.Ls + e] = make_double2(yr[c], yi[c]);
+      }
+    }
+    __syncthreads();
+    double2 *other = (res == bufA) ? bufB : bufA;
+    res = lds_fft<+1>(res, other, nseq, P, a.Ls, tid, nth);
+  } else {
+    res = lds_fft<MODE>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+  }
+
+  for (int v = tid; v < L * W2; v += nth) {
+    int e = v / W2;
+    int w = v - e * W2;
+    int t = w / (2 * C);
+    int rem = w - t * 2 * C;
+    int part = rem >= C;
+    int c = rem - part * C;
+    out[base + (int64_t)e * a.stride_ax + w] =
+        ((const double *)&res[(t * C + c) * a.Ls + e])[part];
+  }
+}
+
+// ------------------------------------------------------------ pass E
+struct LsArgs {
+  const double *rhs;
+  double *partials;
+  double E[6];
+};
+
+template <int C, bool LS>
+__global__ void __launch_bounds__(kThreads)
+    k_rows_inverse(const double *__restrict__ spec, double *y, RowsArgs a,
+                   FftLen P, LsArgs ls, const SolveState *st) {
+  if (solve_done(st)) return;
+  extern __shared__ double2 smem[];
+  __shared__ double red[kThreads / 64];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t row0 = (int64_t)blockIdx.x * a.RB;
+  const int rb = (int)min((int64_t)a.RB, a.nrows - row0);
+  const int Q = rb * C;
+  const int nseq = (Q + 1) / 2;
+  const int nseq_max = (a.RB * C + 1) / 2;
+  double2 *bufA = smem;
+  double2 *bufB = smem + (size_t)nseq_max * a.Ls;
+  const int L = a.L, nh = a.nh;
+
+  // Load the half spectra: pair p = (Xa, Xb).  Slot k holds Xa[k] and slot
+  // L - k holds Xb[k] (0 < k < L/2); the DC and Nyquist slots hold
+  // (Re Xa, Re Xb) -- their imaginary parts are dropped, as in FFTW's and
+  // pocketfft's c2r.
+  for (int v = tid; v < rb * nh * C; v += nth) {
+    int c = v % C;
+    int rest = v / C;
+    int k = rest % nh;
+    int r = rest / nh;
+    int q = r * C + c;
+    int part = q & 1;
+    int64_t pt = (row0 + r) * nh + k;
+    double re = spec[pt * 2 * C + c];
+    double im = spec[pt * 2 * C + C + c];
+    double2 *Z = bufA + (q >> 1) * a.Ls;
+    bool lonely = (!part) && (q + 1 == Q);
+    if (k == 0 || 2 * k == L) {
+      ((double *)&Z[k])[part] = re;
+      if (lonely) Z[k].y = 0.;
+    } else if (!part) {
+      Z[k] = make_double2(re, im);
+      if (lonely) Z[L - k] = make_double2(0., 0.);
+    } else {
+      Z[L - k] = make_double2(re, im);
+    }
+  }
+  __syncthreads();
+  // Z[k] = Xa[k] + i Xb[k], Z[L-k] = conj(Xa[k]) + i conj(Xb[k])
+  const int half = (L - 1) / 2;
+  for (int v = tid; v < nseq * half; v += nth) {
+    int p = v / half;
+    int k = 1 + (v - p * half);
+    double2 *Z = bufA + p * a.Ls;
+    double2 A = Z[k], B = Z[L - k];
+    Z[k] = make_double2(A.x - B.y, A.y + B.x);
+    Z[L - k] = make_double2(A.x + B.y, B.x - A.y);
+  }
+  __syncthreads();
+  double2 *res = lds_fft<+1>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+
+  double acc = 0.;
+  for (int v = tid; v < rb * L; v += nth) {
+    int r = v / L, j = v - r * L;
+    int64_t vox = (row0 + r) * L + j;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      double2 z = res[(q >> 1) * a.Ls + j];
+      double val = a.scale * ((q & 1) ? z.y : z.x);
+      if (LS) {
+        double nv = ls.E[c] - val;
+        if (ls.rhs) nv = (ls.E[c] + ls.rhs[vox * C + c]) - val;
+        double d = nv - y[vox * C + c];
+        acc += d * d;
+        y[vox * C + c] = nv;
+      } else {
+        y[vox * C + c] = val;
+      }
+    }
+  }
+  if (LS) {
+    double s = block_sum(acc, red);
+    if (tid == 0) ls.partials[blockIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------ residual
+__global__ void __launch_bounds__(kThreads)
+    k_finalize(const double *partials, int nb, SolveState *st,
+               double *res_hist) {
+  if (solve_done(st)) return;
+  __shared__ double red[kThreads / 64];
+  double acc = 0.;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[i];
+  double total = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    double res = sqrt(total) * st->res_scale;
+    int it = st->iter;
+    res_hist[it] = res;
+    st->iter = it + 1;
+    if (!isfinite(res))
+      st->done = JX_SOLVE_NONFINITE;
+    else if (res < st->tol)
+      st->done = JX_SOLVE_CONVERGED;
+    else if (it + 1 >= st->max_iter)
+      st->done = JX_SOLVE_MAXITER;
+  }
+}
+
+// ------------------------------------------------------------ local apply
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_local_apply(const double *x, double *y, int64_t N, LocalDev loc) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double e[C], t[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) e[c] = x[v * C + c];
+    local_voxel<C>(loc, v, e, t);
+#pragma unroll
+    for (int c = 0; c < C; ++c) y[v * C + c] = t[c];
+  }
+}
+
+// per-block sums of local : x (columns 0..C-1) and x (columns C..2C-1)
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_local_sums(const double *x, int64_t N, LocalDev loc, double *partials) {
+  __shared__ double red[kThreads / 64];
+  double acc[2 * C];
+#pragma unroll
+  for (int c = 0; c < 2 * C; ++c) acc[c] = 0.;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double e[C], t[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) e[c] = x[v * C + c];
+    local_voxel<C>(loc, v, e, t);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      acc[c] += t[c];
+      acc[C + c] += e[c];
+    }
+  }
+  for (int c = 0; c < 2 * C; ++c) {
+    double s = block_sum(acc[c], red);
+    if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * 2 * C + c] = s;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_reduce_columns(const double *partials, int nb, int ncol, double *out) {
+  __shared__ double red[kThreads / 64];
+  for (int c = 0; c < ncol; ++c) {
+    double acc = 0.;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+      acc += partials[(int64_t)i * ncol + c];
+    double s = block_sum(acc, red);
+    if (threadIdx.x == 0) out[c] = s;
+    __syncthreads();
+  }
+}
+
+// ============================================================ launchers
+
+static int rows_per_block(int L, int C) {
+  int Ls = lds_stride(L);
+  // aim at about 1536 complex entries per ping-pong buffer
+  int nseq = std::max(1, 1536 / Ls);
+  int rb = std::max(1, (2 * nseq) / C);
+  return rb;
+}
+
+static size_t rows_lds_bytes(int L, int C, int RB) {
+  int nseq_max = (RB * C + 1) / 2;
+  return 2 * (size_t)nseq_max * lds_stride(L) * sizeof(double2);
+}
+
+static int check_lds(size_t bytes, const char *what) {
+  if ((long long)bytes > lds_limit_bytes())
+    return fail(JX_EUNSUPPORTED,
+                std::string(what) + ": transform length too large for LDS (" +
+                    std::to_string(bytes) + " bytes needed)");
+  return JX_OK;
+}
+
+template <typename K>
+static int set_lds_attr(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute((const void *)kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+  return JX_OK;
+}
+
+int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
+                        const LocalDev &loc, const SolveState *st,
+                        hipStream_t s) {
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  RowsArgs a;
+  a.nrows = g.rows;
+  a.L = (int)g.L;
+  a.nh = (int)g.nh;
+  a.Ls = lds_stride(a.L);
+  a.RB = rows_per_block(a.L, ncomp);
+  a.scale = 1.0;
+  size_t lds = rows_lds_bytes(a.L, ncomp, a.RB);
+  if ((rc = check_lds(lds, "rows_forward"))) return rc;
+  int64_t nb = (g.rows + a.RB - 1) / a.RB;
+  dim3 grid((unsigned)nb), block(kThreads);
+  switch (ncomp) {
+    case 1:
+      set_lds_attr(k_rows_forward<1>, lds);
+      hipLaunchKernelGGL(k_rows_forward<1>, grid, block, lds, s, x, spec, a, P,
+                         loc, st);
+      break;
+    case 3:
+      set_lds_attr(k_rows_forward<3>, lds);
+      hipLaunchKernelGGL(k_rows_forward<3>, grid, block, lds, s, x, spec, a, P,
+                         loc, st);
+      break;
+    case 6:
+      set_lds_attr(k_rows_forward<6>, lds);
+      hipLaunchKernelGGL(k_rows_forward<6>, grid, block, lds, s, x, spec, a, P,
+                         loc, st);
+      break;
+    default:
+      return fail(JX_EINVAL, "rows_forward: ncomp must be 1, 3 or 6");
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+template <int C>
+static void launch_axis_c(int dir, dim3 grid, size_t lds, hipStream_t s,
+                          const double *in, double *out, const AxisArgs &a,
+                          const FftLen &P, const GreenDev &G,
+                          const SolveState *st) {
+  if (dir < 0) {
+    set_lds_attr(k_axis<C, -1>, lds);
+    hipLaunchKernelGGL((k_axis<C, -1>), grid, dim3(kThreads), lds, s, in, out,
+                       a, P, G, st);
+  } else if (dir > 0) {
+    set_lds_attr(k_axis<C, 1>, lds);
+    hipLaunchKernelGGL((k_axis<C, 1>), grid, dim3(kThreads), lds, s, in, out,
+                       a, P, G, st);
+  } else {
+    set_lds_attr(k_axis<C, 0>, lds);
+    hipLaunchKernelGGL((k_axis<C, 0>), grid, dim3(kThreads), lds, s, in, out,
+                       a, P, G, st);
+  }
+}
+
+int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
+                double *out, const GreenDev *green, const SolveState *st,
+                hipStream_t s) {
+  if (axis < 0 || axis >= g.dim - 1)
+    return fail(JX_EINVAL, "launch_axis: axis out of range");
+  if (dir == 0 && (axis != 0 || green == nullptr || ncomp == 1))
+    return fail(JX_EINVAL, "launch_axis: Gamma-hat pass needs axis 0");
+  FftLen P;
+  int rc = get_fft_len((int)g.n[axis], &P);
+  if (rc) return rc;
+  AxisArgs a;
+  a.L = (int)g.n[axis];
+  a.Ls = lds_stride(a.L);
+  a.nh = (int)g.nh;
+  // points after `axis` (all inner axes, last one halved) are contiguous
+  int64_t inner = g.nh;
+  for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
+  int64_t outer = 1;
+  for (int d = 0; d < axis; ++d) outer *= g.n[d];
+  a.n_inner = inner;
+  a.n_outer = outer;
+  a.stride_ax = inner * 2 * ncomp;
+  a.stride_outer = (int64_t)a.L * inner * 2 * ncomp;
+  int T = std::max(1, 1536 / (a.Ls * ncomp));
+  if (T > inner) T = (int)inner;
+  a.T = T;
+  a.ntiles = (int)((inner + T - 1) / T);
+  size_t lds = 2 * (size_t)T * ncomp * a.Ls * sizeof(double2);
+  if ((rc = check_lds(lds, "axis"))) return rc;
+  int64_t nb = outer * a.ntiles;
+  dim3 grid((unsigned)nb);
+  GreenDev G = green ? *green : GreenDev();
+  switch (ncomp) {
+    case 1: launch_axis_c<1>(dir, grid, lds, s, in, out, a, P, G, st); break;
+    case 3: launch_axis_c<3>(dir, grid, lds, s, in, out, a, P, G, st); break;
+    case 6: launch_axis_c<6>(dir, grid, lds, s, in, out, a, P, G, st); break;
+    default: return fail(JX_EINVAL, "launch_axis: ncomp must be 1, 3 or 6");
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int rows_inverse_nblocks(const Grid &g, int ncomp) {
+  int RB = rows_per_block((int)g.L, ncomp);
+  return (int)((g.rows + RB - 1) / RB);
+}
+
+int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
+                        const RowsInverseLs &ls, const SolveState *st,
+                        int *nblocks_out, hipStream_t s) {
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  RowsArgs a;
+  a.nrows = g.rows;
+  a.L = (int)g.L;
+  a.nh = (int)g.nh;
+  a.Ls = lds_stride(a.L);
+  a.RB = rows_per_block(a.L, ncomp);
+  a.scale = 1.0 / (double)g.N;
+  size_t lds = rows_lds_bytes(a.L, ncomp, a.RB);
+  if ((rc = check_lds(lds, "rows_inverse"))) return rc;
+  int64_t nb = (g.rows + a.RB - 1) / a.RB;
+  if (nblocks_out) *nblocks_out = (int)nb;
+  LsArgs la;
+  la.rhs = ls.rhs;
+  la.partials = ls.partials;
+  for (int c = 0; c < 6; ++c) la.E[c] = ls.E[c];
+  dim3 grid((unsigned)nb), block(kThreads);
+#define JX_RI(CC, LSV)                                                        \
+  do {                                                                        \
+    set_lds_attr(k_rows_inverse<CC, LSV>, lds);                               \
+    hipLaunchKernelGGL((k_rows_inverse<CC, LSV>), grid, block, lds, s, spec, \
+                       y, a, P, la, st);                                      \
+  } while (0)
+  if (ls.enabled) {
+    if (ncomp == 3) JX_RI(3, true);
+    else if (ncomp == 6) JX_RI(6, true);
+    else return fail(JX_EINVAL, "rows_inverse: fixed point needs sym comps");
+  } else {
+    if (ncomp == 1) JX_RI(1, false);
+    else if (ncomp == 3) JX_RI(3, false);
+    else if (ncomp == 6) JX_RI(6, false);
+    else return fail(JX_EINVAL, "rows_inverse: ncomp must be 1, 3 or 6");
+  }
+#undef JX_RI
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_finalize(const double *partials, int nblocks, SolveState *st,
+                    double *res_hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kThreads), 0, s, partials,
+                     nblocks, st, res_hist);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
+                       const double *x, double *y, hipStream_t s) {
+  int64_t nb = std::min<int64_t>((g.N + kThreads - 1) / kThreads, 65536);
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_local_apply<3>, dim3((unsigned)nb), dim3(kThreads), 0,
+                       s, x, y, g.N, loc);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_local_apply<6>, dim3((unsigned)nb), dim3(kThreads), 0,
+                       s, x, y, g.N, loc);
+  else
+    return fail(JX_EINVAL, "local_apply: ncomp must be 3 or 6");
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_local_sums(const Grid &g, int ncomp, const LocalDev &loc,
+                      const double *x, double *partials, int *nblocks,
+                      hipStream_t s) {
+  int nb = (int)std::min<int64_t>((g.N + kThreads - 1) / kThreads, 1024);
+  *nblocks = nb;
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_local_sums<3>, dim3(nb), dim3(kThreads), 0, s, x, g.N,
+                       loc, partials);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_local_sums<6>, dim3(nb), dim3(kThreads), 0, s, x, g.N,
+                       loc, partials);
+  else
+    return fail(JX_EINVAL, "local_sums: ncomp must be 3 or 6");
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_reduce_columns(const double *partials, int nblocks, int ncol,
+                          double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_columns, dim3(1), dim3(kThreads), 0, s, partials,
+                     nblocks, ncol, out);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+}  // namespace jx

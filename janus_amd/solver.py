@@ -1,0 +1,146 @@
+"""On-device basic (Moulinec-Suquet) fixed-point solver.
+
+janus leaves the solver loop to user scripts; the canonical one is
+sphinx/tutorials/square_basic/square_basic.py:83-97:
+
+    eps = E;  repeat: new = E - Gamma0[(C - C0) : eps]
+                      res = ||new - eps|| / (sqrt(N) ||E||);  eps = new
+
+:class:`LSSolver` runs exactly that loop on the GPU through
+``jx_solver_solve``: every iteration is five fused passes (local operator
+fused into the first FFT pass, Gamma-hat between the forward and inverse
+transforms of axis 0, ``E - .`` and the residual partial sums fused into the
+last pass), the residual is reduced on the device and the host reads the
+solver state only once per chunk of iterations.  The iteration count is the
+one the host loop would give: the loop stops after the first iteration whose
+residual is below ``tol``.
+"""
+
+import ctypes
+from collections import namedtuple
+
+import numpy as np
+
+from . import _device, _lib
+from .material.elastic.linear.isotropic import IsotropicLinearElasticMaterial
+
+SolveResult = namedtuple('SolveResult', 'eps residuals iterations status')
+
+STATUS = {_lib.JX_SOLVE_CONVERGED: 'converged',
+          _lib.JX_SOLVE_MAXITER: 'max_iter',
+          _lib.JX_SOLVE_NONFINITE: 'nonfinite'}
+
+
+class LSSolver:
+    """Fixed point eps <- E + rhs - Gamma0[local : eps] on the GPU.
+
+    Args:
+        green: a janus_amd.green operator built with a transform.
+        local: a janus_amd.operators.BlockDiagonalOperator2D/3D holding
+            C - C0.
+    """
+
+    def __init__(self, green, local):
+        if green.transform is None:
+            raise ValueError('the Green operator needs a transform')
+        if tuple(green.ishape) != tuple(local.ishape):
+            raise ValueError('Green operator and local operator shapes '
+                             'differ: {0} vs {1}'.format(green.ishape,
+                                                         local.ishape))
+        _lib.require_gpu()
+        self.green = green
+        self.local = local
+        self.shape = tuple(green.ishape)
+        self.sym = self.shape[-1]
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().jx_solver_create(green.handle, local.handle,
+                                               ctypes.byref(h)))
+        self._handle = h
+
+    def __del__(self):
+        h = getattr(self, '_handle', None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.jx_solver_destroy(h)
+            self._handle = None
+
+    def solve(self, E, tol=1e-10, max_iter=200, eps=None, rhs=None,
+              res_scale=None):
+        """Run the fixed point.
+
+        Args:
+            E: macroscopic strain (sym,), Mandel notation.
+            tol: stop after the first iteration whose residual is < tol.
+            max_iter: iteration cap.
+            eps: optional float64 GPU tensor of shape ``grid + (sym,)``: the
+                initial iterate, overwritten by the result.  Default: E.
+            rhs: optional GPU field added to E.
+            res_scale: residual normalization; default 1 / (sqrt(N) ||E||).
+        Returns:
+            SolveResult(eps, residuals, iterations, status).
+        """
+        t = _device.torch()
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        if E.shape != (self.sym,):
+            raise ValueError('E must have shape ({0},)'.format(self.sym))
+        if eps is None:
+            eps = t.empty(self.shape, dtype=t.float64, device='cuda')
+            eps[...] = t.from_numpy(E).to('cuda')
+        elif not (_device.is_device_tensor(eps) and eps.is_contiguous() and
+                  tuple(eps.shape) == self.shape and eps.dtype == t.float64):
+            raise ValueError('eps must be a contiguous float64 GPU tensor of '
+                             'shape {0}'.format(self.shape))
+        rhs_ptr = None
+        if rhs is not None:
+            if not (_device.is_device_tensor(rhs) and rhs.is_contiguous() and
+                    tuple(rhs.shape) == self.shape):
+                raise ValueError('rhs must be a contiguous GPU field of '
+                                 'shape {0}'.format(self.shape))
+            rhs_ptr = rhs.data_ptr()
+        if res_scale is None:
+            res_scale = 0.0
+        hist = np.zeros(max_iter, dtype=np.float64)
+        iters = ctypes.c_int(0)
+        status = ctypes.c_int(0)
+        _lib.check(_lib.lib().jx_solver_solve(
+            self._handle, eps.data_ptr(), E.ctypes.data, rhs_ptr, float(tol),
+            int(max_iter), float(res_scale), hist.ctypes.data,
+            ctypes.byref(iters), ctypes.byref(status),
+            _lib.current_stream()))
+        n = iters.value
+        return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
+
+    def mean_stress(self, eps, E, mat0):
+        """<sigma> = C0 : E + <(C - C0) : eps> (square_basic.rst:212-221)."""
+        tau, _ = self.local.means(eps)
+        return reference_stiffness(mat0).dot(np.asarray(E)) + tau
+
+
+def reference_stiffness(mat0):
+    """Mandel matrix of the isotropic reference stiffness C0 = dim k0 J +
+    2 g0 K."""
+    if not isinstance(mat0, IsotropicLinearElasticMaterial):
+        raise ValueError('isotropic reference material expected')
+    from .operators import isotropic_4
+    return isotropic_4(mat0.dim * mat0.k, 2. * mat0.g, mat0.dim).to_memoryview()
+
+
+def delta_stiffness(mat, mat0):
+    """(sph, dev) of C - C0 as built in square_basic.py:26-29."""
+    return mat.dim * (mat.k - mat0.k), 2. * (mat.g - mat0.g)
+
+
+def effective_stiffness(solver, mat0, tol=1e-10, max_iter=200):
+    """Columns C_eff[:, j] = <sigma> under the Mandel unit loads E = e_j.
+
+    Returns (C_eff, iterations per load, status per load)."""
+    sym = solver.sym
+    C = np.zeros((sym, sym))
+    its, sts = [], []
+    for j in range(sym):
+        E = np.zeros(sym)
+        E[j] = 1.0
+        r = solver.solve(E, tol=tol, max_iter=max_iter)
+        C[:, j] = solver.mean_stress(r.eps, E, mat0)
+        its.append(r.iterations)
+        sts.append(r.status)
+    return C, its, sts

@@ -1,0 +1,422 @@
+"""numpy restatement of janus's serial Lippmann-Schwinger path (FP64).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``; never by ``janus_amd``.
+
+What each function restates, and what pins it (the tests in
+``tests/test_oracle.py`` carry out every pin listed here):
+
+===========================  ==============================================  ==========================================
+function                     janus source                                    pinned by
+===========================  ==============================================  ==========================================
+bulk_modulus                 material/elastic/linear/isotropic.pyx:79-97     test_isotropic_linear_elastic_material.py:14-20 values
+gamma0                       isotropic.pyx:133-156, 173-200, 242-308         tensor closed form (test_greenop.py:21-35) within
+                                                                             325 nulp; printed matrix sphinx/materials.rst:53-68
+truncated_wavevector         green.pyx:432-445, 505-523                      brute-force alias rule (test_discrete_green_operator.py:214-222)
+gamma_hat_truncated          green.pyx:432-451, 505-529                      same; 18 fixtures tests/data/*.npz
+gamma_hat_filtered           green.pyx:607-694, 730-910                      brute-force alias sum
+                                                                             (test_discrete_green_operator.py:249-260); fixtures
+green_apply                  green.pyx:182-216, 353-390, 453-495, 531-580    18 fixtures (rtol=ULP, atol=10 ULP)
+iso4_apply                   operators.pyx:292-299, 315-324, 352-365         J/K projector matrices (test_operators.py:180-227)
+block_diag_apply             operators.pyx:655-664, 706-716                  per-voxel loop (test_operators.py:409-450)
+basic_scheme                 sphinx/tutorials/square_basic/square_basic.py   published C_1212 = 1.41903971282
+                             :17-97                                          (square_basic.rst:227)
+rfft / irfft                 fft/serial/_serial_fft.pyx:152-177, 296-325     numpy.fft (test_fft.py:37-70) -- the
+                                                                             oracle's FFT *is* numpy's pocketfft
+adjoint_gradient             (no janus counterpart) implicit                 central finite differences of basic_scheme
+                             differentiation of the fixed point
+===========================  ==============================================  ==========================================
+"""
+
+import itertools
+
+import numpy as np
+
+SQRT2 = np.sqrt(2.0)
+
+
+def sym_of(dim):
+    return (dim * (dim + 1)) // 2
+
+
+# --------------------------------------------------------------------------
+# Materials
+# --------------------------------------------------------------------------
+
+def bulk_modulus(g, nu, dim):
+    """Bulk modulus, plane strain in 2D (isotropic.pyx:79-97)."""
+    if dim == 2:
+        return g / (1. - 2. * nu)
+    if dim == 3:
+        return 2. / 3. * (1. + nu) / (1. - 2. * nu) * g
+    raise ValueError('dim must be 2 or 3 (was {0})'.format(dim))
+
+
+def delta_c_coefficients(g, nu, g0, nu0, dim):
+    """(sph, dev) of C - C0 as built by square_basic.py:26-29."""
+    k = bulk_modulus(g, nu, dim)
+    k0 = bulk_modulus(g0, nu0, dim)
+    return dim * (k - k0), 2. * (g - g0)
+
+
+# --------------------------------------------------------------------------
+# Continuous Green operator for strains
+# --------------------------------------------------------------------------
+
+def gamma0(k, g, nu):
+    """Mandel matrix of Gamma0(k), shape (..., sym, sym); 0 at k = 0.
+
+    Follows isotropic.pyx:133-156 (auxiliary constants), :173-200 (2D) and
+    :242-308 (3D), with the same order of floating-point operations.
+    """
+    k = np.asarray(k, dtype=np.float64)
+    dim = k.shape[-1]
+    daux1 = 1.0 / g
+    daux2 = 0.5 / (g * (1.0 - nu))
+    daux3 = 0.25 / g
+    daux4 = 0.5 / g
+    sym = sym_of(dim)
+    out = np.zeros(k.shape[:-1] + (sym, sym), dtype=np.float64)
+    if dim == 2:
+        k0 = k[..., 0]
+        k1 = k[..., 1]
+        k0k0 = k0 * k0
+        k1k1 = k1 * k1
+        kk = k0k0 + k1k1
+        zero = kk == 0.
+        with np.errstate(divide='ignore', invalid='ignore'):
+            s = 1.0 / np.where(zero, 1.0, kk)
+        k0k0 = k0k0 * s
+        k1k1 = k1k1 * s
+        k0k1 = s * k0 * k1
+        g00 = k0k0 * (daux1 - daux2 * k0k0)
+        g11 = k1k1 * (daux1 - daux2 * k1k1)
+        dummy = -daux2 * k0k0 * k1k1
+        g01 = dummy
+        g22 = 2 * (daux3 + dummy)
+        g02 = SQRT2 * k0k1 * (daux4 - daux2 * k0k0)
+        g12 = SQRT2 * k0k1 * (daux4 - daux2 * k1k1)
+        coeffs = {(0, 0): g00, (0, 1): g01, (0, 2): g02,
+                  (1, 1): g11, (1, 2): g12, (2, 2): g22}
+    elif dim == 3:
+        k0 = k[..., 0]
+        k1 = k[..., 1]
+        k2 = k[..., 2]
+        k0k0 = k0 * k0
+        k1k1 = k1 * k1
+        k2k2 = k2 * k2
+        kk = k0k0 + k1k1 + k2k2
+        zero = kk == 0.
+        with np.errstate(divide='ignore', invalid='ignore'):
+            s = 1.0 / np.where(zero, 1.0, kk)
+        k0k0 = k0k0 * s
+        k1k1 = k1k1 * s
+        k2k2 = k2k2 * s
+        k1k2 = s * k1 * k2
+        k2k0 = s * k2 * k0
+        k0k1 = s * k0 * k1
+        coeffs = {
+            (0, 0): k0k0 * (daux1 - daux2 * k0k0),
+            (1, 1): k1k1 * (daux1 - daux2 * k1k1),
+            (2, 2): k2k2 * (daux1 - daux2 * k2k2),
+            (3, 3): 2. * (daux3 * (k1k1 + k2k2) - daux2 * k1k1 * k2k2),
+            (4, 4): 2. * (daux3 * (k2k2 + k0k0) - daux2 * k2k2 * k0k0),
+            (5, 5): 2. * (daux3 * (k0k0 + k1k1) - daux2 * k0k0 * k1k1),
+            (0, 1): -daux2 * k0k0 * k1k1,
+            (0, 2): -daux2 * k0k0 * k2k2,
+            (0, 3): -SQRT2 * daux2 * k0k0 * k1k2,
+            (0, 4): SQRT2 * k2k0 * (daux4 - daux2 * k0k0),
+            (0, 5): SQRT2 * k0k1 * (daux4 - daux2 * k0k0),
+            (1, 2): -daux2 * k1k1 * k2k2,
+            (1, 3): SQRT2 * k1k2 * (daux4 - daux2 * k1k1),
+            (1, 4): -SQRT2 * daux2 * k1k1 * k2k0,
+            (1, 5): SQRT2 * k0k1 * (daux4 - daux2 * k1k1),
+            (2, 3): SQRT2 * k1k2 * (daux4 - daux2 * k2k2),
+            (2, 4): SQRT2 * k2k0 * (daux4 - daux2 * k2k2),
+            (2, 5): -SQRT2 * daux2 * k2k2 * k0k1,
+            (3, 4): 2 * k0k1 * (daux3 - daux2 * k2k2),
+            (3, 5): 2 * k2k0 * (daux3 - daux2 * k1k1),
+            (4, 5): 2 * k1k2 * (daux3 - daux2 * k0k0),
+        }
+    else:
+        raise ValueError('dim must be 2 or 3 (was {0})'.format(dim))
+    for (i, j), c in coeffs.items():
+        c = np.where(zero, 0., c)
+        out[..., i, j] = c
+        out[..., j, i] = c
+    return out
+
+
+# --------------------------------------------------------------------------
+# Discrete Green operators (per frequency)
+# --------------------------------------------------------------------------
+
+def _steps(n, h):
+    """s_i = 2 pi / (h n_i) (green.pyx:140-141, 305-307)."""
+    return [2. * np.pi / (h * ni) for ni in n]
+
+
+def truncated_wavevector(b, n, h):
+    """Aliased wave vector of the truncated operator (green.pyx:432-445).
+
+    k_i = s_i (b_i - n_i) if 2 b_i > n_i else s_i b_i.  ``b`` may be an
+    integer array of shape (..., dim).
+    """
+    b = np.asarray(b)
+    s = _steps(n, h)
+    k = np.empty(b.shape, dtype=np.float64)
+    for i, ni in enumerate(n):
+        bi = b[..., i]
+        k[..., i] = np.where(2 * bi > ni, s[i] * (bi - ni), s[i] * bi)
+    return k
+
+
+def gamma_hat_truncated(b, n, h, g, nu):
+    """Gamma-hat(b) of the truncated operator (green.pyx:432-451)."""
+    return gamma0(truncated_wavevector(b, n, h), g, nu)
+
+
+def _filtered_axis_tables(n, h):
+    """Per-axis aliases and weights of the filtered operator.
+
+    For axis i and index b: the two aliases k- = s_i (b - n_i), k+ = s_i b
+    and their weights cos^2(h k / 4) (green.pyx:740-800).
+    """
+    s = _steps(n, h)
+    tabs = []
+    for i, ni in enumerate(n):
+        b = np.arange(ni)
+        km = s[i] * (b - ni)
+        kp = s[i] * b
+        wm = np.cos(0.25 * h * km)
+        wm = wm * wm
+        wp = np.cos(0.25 * h * kp)
+        wp = wp * wp
+        tabs.append((km, kp, wm, wp))
+    return tabs
+
+
+def gamma_hat_filtered(b, n, h, g, nu):
+    """Gamma-hat(b) of the filtered operator (green.pyx:607-694, 730-910).
+
+    Weighted sum over the 2^dim alias combinations, accumulated in janus's
+    order: the first axis varies slowest and the "minus" alias comes first
+    (k1 = (-,-,-), k2 = (-,-,+), ..., k8 = (+,+,+)).
+    """
+    b = np.asarray(b)
+    dim = len(n)
+    tabs = _filtered_axis_tables(n, h)
+    acc = None
+    for combo in itertools.product((0, 1), repeat=dim):
+        k = np.empty(b.shape, dtype=np.float64)
+        w = None
+        for i in range(dim):
+            km, kp, wm, wp = tabs[i]
+            bi = b[..., i]
+            if combo[i] == 0:
+                k[..., i] = km[bi]
+                wi = wm[bi]
+            else:
+                k[..., i] = kp[bi]
+                wi = wp[bi]
+            w = wi if w is None else w * wi
+        term = w[..., None, None] * gamma0(k, g, nu)
+        acc = term if acc is None else acc + term
+    return acc
+
+
+def gamma_hat(kind, b, n, h, g, nu):
+    if kind == 'truncated':
+        return gamma_hat_truncated(b, n, h, g, nu)
+    if kind == 'filtered':
+        return gamma_hat_filtered(b, n, h, g, nu)
+    raise ValueError(kind)
+
+
+# --------------------------------------------------------------------------
+# FFT (janus layout) and Green apply
+# --------------------------------------------------------------------------
+
+def rfft_janus(r):
+    """r2c in janus's interleaved layout (_serial_fft.pyx:152-163).
+
+    Output shape (..., 2 * (n_last // 2 + 1)); even index = Re, odd = Im
+    (sphinx/fft_tutorial.rst:43).
+    """
+    c = np.fft.rfftn(r)
+    out = np.empty(c.shape[:-1] + (2 * c.shape[-1],), dtype=np.float64)
+    out[..., 0::2] = c.real
+    out[..., 1::2] = c.imag
+    return out
+
+
+def irfft_janus(c, shape):
+    """c2r from janus's interleaved layout, scaled by 1/N (:165-177)."""
+    z = c[..., 0::2] + 1j * c[..., 1::2]
+    return np.fft.irfftn(z, s=shape, axes=tuple(range(len(shape))))
+
+
+def _half_indices(n, i0):
+    """Integer frequency indices of the half spectrum for axis-0 slab i0."""
+    dim = len(n)
+    axes = [np.array([i0])] + [np.arange(ni) for ni in n[1:-1]] \
+        + [np.arange(n[-1] // 2 + 1)]
+    grids = np.meshgrid(*axes, indexing='ij')
+    b = np.stack(grids, axis=-1)
+    return b.reshape(b.shape[1:]) if dim > 1 else b
+
+
+def green_apply(x, kind, g, nu, h=1.0):
+    """y = Gamma0[x] on the grid (green.pyx:182-216, 353-390, 453-580).
+
+    r2c of each component, per-frequency symmetric matvec applied to Re
+    and Im, c2r of each component (c2r drops the imaginary parts of the
+    DC and Nyquist planes of the last axis, as FFTW and pocketfft do).
+    """
+    x = np.asarray(x, dtype=np.float64)
+    n = x.shape[:-1]
+    dim = len(n)
+    axes = tuple(range(dim))
+    xh = np.fft.rfftn(x, axes=axes)
+    yh = np.empty_like(xh)
+    for i0 in range(n[0]):
+        b = _half_indices(n, i0)
+        G = gamma_hat(kind, b, n, h, g, nu)
+        yh[i0] = np.einsum('...ij,...j->...i', G, xh[i0].real) \
+            + 1j * np.einsum('...ij,...j->...i', G, xh[i0].imag)
+    return np.fft.irfftn(yh, s=n, axes=axes)
+
+
+# --------------------------------------------------------------------------
+# Local operators
+# --------------------------------------------------------------------------
+
+def iso4_apply(sph, dev, x):
+    """(sph J + dev K) : x in Mandel notation (operators.pyx:292-299,
+    315-324, 352-365).  ``sph``/``dev`` broadcast against x[..., 0]."""
+    x = np.asarray(x, dtype=np.float64)
+    sym = x.shape[-1]
+    dim = 2 if sym == 3 else 3
+    sph = np.asarray(sph, dtype=np.float64)
+    dev = np.asarray(dev, dtype=np.float64)
+    tr = (sph - dev) / float(dim)
+    if dim == 2:
+        aux = tr * (x[..., 0] + x[..., 1])
+    else:
+        aux = tr * (x[..., 0] + x[..., 1] + x[..., 2])
+    y = np.empty_like(x)
+    for i in range(dim):
+        y[..., i] = aux + dev * x[..., i]
+    for i in range(dim, sym):
+        y[..., i] = dev * x[..., i]
+    return y
+
+
+def iso4_matrix(sph, dev, dim):
+    """Matrix of sph J + dev K (operators.pyx:326-340, 367-408)."""
+    sym = sym_of(dim)
+    a = np.zeros((sym, sym))
+    a[:dim, :dim] = (sph - dev) / dim
+    for i in range(dim):
+        a[i, i] = (sph + (dim - 1) * dev) / dim
+    for i in range(dim, sym):
+        a[i, i] = dev
+    return a
+
+
+def block_diag_apply(phase, table, x):
+    """Block-diagonal isotropic operator from a phase map (operators.pyx
+    :655-664, 706-716): y[v] = iso4(table[phase[v]]) : x[v]."""
+    table = np.asarray(table, dtype=np.float64)
+    sph = table[phase, 0]
+    dev = table[phase, 1]
+    return iso4_apply(sph, dev, x)
+
+
+# --------------------------------------------------------------------------
+# Basic fixed-point scheme (square_basic.py:83-97)
+# --------------------------------------------------------------------------
+
+def basic_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
+                 rhs=None, eps0=None):
+    """Moulinec-Suquet iteration eps <- R - Gamma0[dC : eps].
+
+    ``sph``/``dev`` are per-voxel fields (shape = grid) of dC = C - C0.
+    R = E (macroscopic strain, constant) plus the optional field ``rhs``.
+    The residual is ||eps_{k+1} - eps_k|| / (sqrt(N) ||E||)
+    (square_basic.py:86, 91); when ||E|| = 0 the plain norm / sqrt(N) is used.
+    Iteration stops after the first iteration whose residual is < tol, or
+    after ``max_iter`` iterations.  Returns (eps, residual history).
+    """
+    E = np.asarray(E, dtype=np.float64)
+    shape = np.shape(sph)
+    N = int(np.prod(shape))
+    nE = np.linalg.norm(E)
+    normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
+    eps = np.empty(shape + (E.size,), dtype=np.float64)
+    eps[...] = E if eps0 is None else eps0
+    res = []
+    for _ in range(max_iter):
+        tau = iso4_apply(sph, dev, eps)
+        new = green_apply(tau, kind, g0, nu0, h)
+        new = E - new
+        if rhs is not None:
+            new += rhs
+        r = normalization * np.linalg.norm(new - eps)
+        res.append(r)
+        eps = new
+        if r < tol:
+            break
+    return eps, np.array(res)
+
+
+def mean_stress(sph, dev, eps, g0, nu0, E):
+    """<sigma> = C0 : E + <dC : eps> (square_basic.rst:212-221)."""
+    dim = 2 if eps.shape[-1] == 3 else 3
+    k0 = bulk_modulus(g0, nu0, dim)
+    c0 = iso4_matrix(dim * k0, 2 * g0, dim)
+    tau = iso4_apply(sph, dev, eps)
+    return c0 @ np.asarray(E) + tau.reshape(-1, eps.shape[-1]).mean(axis=0)
+
+
+# --------------------------------------------------------------------------
+# Adjoint (no janus counterpart): implicit differentiation of the fixed point
+# --------------------------------------------------------------------------
+
+def adjoint_gradient(sph, dev, eps, grad_eps, kind, g0, nu0, tol, max_iter,
+                     h=1.0):
+    """Gradient of L(eps(dC)) w.r.t. the per-voxel (sph, dev) of dC.
+
+    eps solves (I + G dC) eps = E with G = Gamma0 (symmetric, as is dC).
+    With g = dL/deps, mu solves (I + G dC) mu = G g, i.e. the fixed point
+    mu <- G g - G[dC : mu]; then dL/dsph_x = -(J eps_x) . mu_x and
+    dL/ddev_x = -(K eps_x) . mu_x.  Returns (dsph, ddev, mu, residuals).
+    """
+    sym = eps.shape[-1]
+    dim = 2 if sym == 3 else 3
+    Gg = green_apply(grad_eps, kind, g0, nu0, h)
+    mu, res = _field_rhs_scheme(sph, dev, Gg, kind, g0, nu0, tol, max_iter, h)
+    tr_e = eps[..., :dim].sum(axis=-1)
+    tr_m = mu[..., :dim].sum(axis=-1)
+    j_term = tr_e * tr_m / dim
+    dsph = -j_term
+    ddev = -((eps * mu).sum(axis=-1) - j_term)
+    return dsph, ddev, mu, res
+
+
+def _field_rhs_scheme(sph, dev, rhs, kind, g0, nu0, tol, max_iter, h):
+    """mu <- rhs - Gamma0[dC : mu], residual ||mu_{k+1}-mu_k||/sqrt(N)/||rhs||_rms."""
+    shape = np.shape(sph)
+    N = int(np.prod(shape))
+    nr = np.linalg.norm(rhs) / np.sqrt(N)
+    normalization = 1. / np.sqrt(N) / (nr if nr > 0 else 1.0)
+    mu = rhs.copy()
+    res = []
+    for _ in range(max_iter):
+        new = rhs - green_apply(iso4_apply(sph, dev, mu), kind, g0, nu0, h)
+        r = normalization * np.linalg.norm(new - mu)
+        res.append(r)
+        mu = new
+        if r < tol:
+            break
+    return mu, np.array(res)

@@ -1,0 +1,85 @@
+"""GPU block-diagonal isotropic operator vs the oracle's per-voxel loop
+(janus's tests/test_operators.py:409-450 asks 0 ULP: the kernel keeps the
+products and sums separate, so results are bit-identical)."""
+
+import itertools
+
+import numpy as np
+import pytest
+
+from janus_amd.operators import BlockDiagonalOperator2D, \
+    BlockDiagonalOperator3D, block_diagonal_operator, isotropic_4
+from oracle import janus_oracle as jo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_object_array_constructor(dim):
+    rng = np.random.default_rng(dim)
+    sym = (dim * (dim + 1)) // 2
+    shape = tuple(range(sym + dim, sym, -1))
+    loc = np.empty(shape, dtype=object)
+    for index in itertools.product(*map(range, shape)):
+        loc[index] = isotropic_4(2. * rng.random() - 1, 2. * rng.random() - 1,
+                                 dim)
+    op = block_diagonal_operator(loc)
+    for _ in range(3):
+        x = rng.random(op.ishape)
+        expected = np.empty(op.oshape)
+        for index in itertools.product(*map(range, shape)):
+            loc[index].apply(x[index], expected[index])
+        y = np.empty(op.oshape)
+        ret = op.apply(x, y)
+        assert ret.base is y
+        np.testing.assert_array_equal(y, expected)
+        np.testing.assert_array_equal(op.apply(x), expected)
+        op.apply(x, x)   # in place
+        np.testing.assert_array_equal(x, expected)
+
+
+@pytest.mark.parametrize('dim, n', [(2, (33, 17)), (3, (9, 10, 11))])
+def test_from_phases_and_fields(dim, n):
+    rng = np.random.default_rng(3)
+    sym = (dim * (dim + 1)) // 2
+    table = rng.uniform(-2, 5, size=(7, 2))
+    phase = rng.integers(0, 7, size=n).astype(np.uint8)
+    x = rng.standard_normal(n + (sym,))
+    cls = BlockDiagonalOperator2D if dim == 2 else BlockDiagonalOperator3D
+    expected = jo.block_diag_apply(phase, table, x)
+    np.testing.assert_array_equal(
+        cls.from_phases(phase, table).apply(x), expected)
+    sph = table[phase, 0]
+    dev = table[phase, 1]
+    np.testing.assert_array_equal(
+        cls.from_fields(sph, dev).apply(x), jo.iso4_apply(sph, dev, x))
+
+
+def test_means():
+    rng = np.random.default_rng(9)
+    n = (20, 21, 22)
+    table = rng.uniform(-2, 5, size=(3, 2))
+    phase = rng.integers(0, 3, size=n).astype(np.uint8)
+    x = rng.standard_normal(n + (6,))
+    op = BlockDiagonalOperator3D.from_phases(phase, table)
+    tau, eps = op.means(x)
+    np.testing.assert_allclose(
+        tau, jo.block_diag_apply(phase, table, x).reshape(-1, 6).mean(0),
+        rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(eps, x.reshape(-1, 6).mean(0), rtol=1e-13,
+                               atol=1e-15)
+
+
+def test_invalid():
+    with pytest.raises(ValueError):
+        BlockDiagonalOperator3D.from_phases(np.zeros((4, 4, 4), np.uint8),
+                                            np.zeros((300, 2)))
+    with pytest.raises(ValueError):
+        BlockDiagonalOperator2D.from_phases(np.full((4, 4), 3, np.uint8),
+                                            np.zeros((2, 2)))
+    op = BlockDiagonalOperator2D.from_phases(np.zeros((4, 5), np.uint8),
+                                             np.ones((1, 2)))
+    with pytest.raises(ValueError):
+        op.apply(np.zeros((4, 5, 2)))
+    with pytest.raises(ValueError):
+        op.apply(np.zeros((4, 5, 3)), np.zeros((5, 4, 3)))

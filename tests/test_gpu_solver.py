@@ -1,0 +1,123 @@
+"""Fused GPU fixed-point solver vs the oracle's basic scheme
+(square_basic.py:83-97), iterate for iterate: same iteration count, strain
+fields within 1e-12 relative, residual histories within 1e-10 relative;
+plus the published tutorial result C_1212 = 1.41903971282
+(square_basic.rst:227) computed entirely on the GPU."""
+
+import numpy as np
+import pytest
+
+import janus_amd.fft.serial as fft
+import janus_amd.material.elastic.linear.isotropic as material
+from janus_amd import microstructure as ms
+from janus_amd.green import filtered, truncated
+from janus_amd.operators import BlockDiagonalOperator2D, \
+    BlockDiagonalOperator3D
+from janus_amd.solver import LSSolver, delta_stiffness
+from oracle import janus_oracle as jo
+
+pytestmark = pytest.mark.gpu
+
+FACTORIES = {'truncated': truncated, 'filtered': filtered}
+
+
+def build(phase, mats, mat0, kind):
+    dim = phase.ndim
+    n = phase.shape
+    table = np.array([delta_stiffness(m, mat0) for m in mats])
+    cls = BlockDiagonalOperator2D if dim == 2 else BlockDiagonalOperator3D
+    local = cls.from_phases(phase, table)
+    green = FACTORIES[kind](mat0.green_operator(), n, 1.0, fft.create_real(n))
+    return LSSolver(green, local), table
+
+
+CASES = [
+    # dim, n, kind, (mu_m, nu_m), (mu_i, nu_i)
+    (2, 32, 'truncated', (1.0, 0.3), (10.0, 0.3)),
+    (2, 33, 'filtered', (1.0, 0.3), (20.0, 0.2)),
+    (3, 16, 'filtered', (1.0, 0.3), (50.0, 0.25)),
+    (3, 15, 'truncated', (1.0, 0.3), (5.0, 0.3)),
+]
+
+
+@pytest.mark.parametrize('dim, n, kind, m, i', CASES)
+def test_iterates_match_oracle(dim, n, kind, m, i):
+    if dim == 2:
+        phase = ms.square_inclusion(n, 0.5)
+    else:
+        phase = ms.spheres(n, 3.0, 0.2, seed=1)
+    mat_m = material.create(*m, dim)
+    mat_i = material.create(*i, dim)
+    # Moulinec-Suquet midpoint reference medium
+    mat0 = material.create(0.5 * (m[0] + i[0]), 0.3, dim)
+    solver, table = build(phase, [mat_m, mat_i], mat0, kind)
+    sym = solver.sym
+    rng = np.random.default_rng(n)
+    E = rng.standard_normal(sym)
+    r = solver.solve(E, tol=1e-10, max_iter=300)
+    assert r.status == 'converged'
+    eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1], E,
+                                       kind, mat0.g, mat0.nu, tol=1e-10,
+                                       max_iter=300)
+    assert r.iterations == len(res_ref)
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-9, atol=1e-15)
+    eps = r.eps.cpu().numpy()
+    err = np.abs(eps - eps_ref).max() / np.abs(eps_ref).max()
+    assert err < 1e-12, err
+
+
+def test_tutorial_c1212_on_gpu():
+    """square_basic.py:60-97: 2D 256^2, truncated, 400 iterations."""
+    n, dim = 256, 2
+    mat_i = material.create(100, 0.2, dim)
+    mat_m = material.create(1, 0.3, dim)
+    mat0 = material.create(50, 0.3, dim)
+    phase = ms.square_inclusion(n, 0.5, centered=False)
+    solver, _ = build(phase, [mat_m, mat_i], mat0, 'truncated')
+    E = np.array([0., 0., 1.])
+    r = solver.solve(E, tol=0.0, max_iter=400)
+    assert r.iterations == 400 and r.status == 'max_iter'
+    sigma = solver.mean_stress(r.eps, E, mat0)
+    c1212 = 0.5 * sigma[-1] / E[-1]
+    assert abs(c1212 - 1.41903971282) < 6e-12
+
+
+def test_divergent_reference_medium_reports_status():
+    """C0 = matrix with contrast 10 diverges under the basic scheme (as it
+    does in the oracle); the solver stops at max_iter with growing
+    residuals, and the first iterates still match the oracle."""
+    n, dim = 32, 2
+    mat_m = material.create(1.0, 0.3, dim)
+    mat_i = material.create(10.0, 0.3, dim)
+    phase = ms.square_inclusion(n, 0.5)
+    solver, table = build(phase, [mat_m, mat_i], mat_m, 'truncated')
+    E = np.array([0., 0., 1.])
+    r = solver.solve(E, tol=1e-10, max_iter=8)
+    _, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1], E,
+                                 'truncated', 1.0, 0.3, tol=1e-10, max_iter=8)
+    assert r.status == 'max_iter' and r.iterations == 8
+    assert r.residuals[-1] > r.residuals[0]
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-9)
+
+
+def test_field_rhs_matches_oracle():
+    n, dim = 20, 3
+    phase = ms.spheres(n, 4.0, 0.25, seed=3)
+    mat_m = material.create(1.0, 0.3, dim)
+    mat_i = material.create(8.0, 0.3, dim)
+    mat0 = material.create(4.5, 0.3, dim)
+    solver, table = build(phase, [mat_m, mat_i], mat0, 'filtered')
+    import torch
+    rng = np.random.default_rng(4)
+    rhs = rng.standard_normal((n, n, n, 6))
+    rhs_d = torch.from_numpy(rhs).cuda()
+    eps0 = rhs_d.clone()
+    scale = 1.0 / np.linalg.norm(rhs)
+    r = solver.solve(np.zeros(6), tol=1e-11, max_iter=200, eps=eps0,
+                     rhs=rhs_d, res_scale=scale)
+    mu, res = jo._field_rhs_scheme(table[phase, 0], table[phase, 1], rhs,
+                                   'filtered', mat0.g, mat0.nu, 1e-11, 200,
+                                   1.0)
+    assert r.iterations == len(res)
+    np.testing.assert_allclose(r.eps.cpu().numpy(), mu, rtol=0,
+                               atol=1e-12 * np.abs(mu).max())

@@ -1,0 +1,292 @@
+"""Pins of the CPU oracle (oracle/janus_oracle.py) against janus's own
+known answers and against independent closed forms.
+
+* 18 Green-operator fixtures of tests/data/*.npz (sampled into
+  tests/golden/green_fixture_samples.npz), tolerance of
+  tests/test_discrete_green_operator.py:199 (rtol = ULP, atol = 10 ULP);
+* the published tutorial result C_1212 = 1.41903971282
+  (sphinx/tutorials/square_basic/square_basic.rst:227);
+* Gamma0(k) against the tensor closed form of tests/test_greenop.py:21-35
+  (325 nulp, wave vectors of :127-158) and the matrix printed in
+  sphinx/materials.rst:53-68;
+* per-frequency Gamma-hat against brute-force alias sums built from that
+  tensor closed form (tests/test_discrete_green_operator.py:214-222, 249-260);
+* isotropic tensor against the J/K projector matrices
+  (tests/test_operators.py:180-227), block-diagonal operator against a
+  per-voxel loop (:409-450);
+* material constants (tests/test_isotropic_linear_elastic_material.py:14-20);
+* the adjoint gradient against central finite differences of the solve.
+"""
+
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from oracle import janus_oracle as jo
+
+ULP = np.finfo(np.float64).eps
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, 'golden', 'green_fixture_samples.npz')
+
+
+def fixture_cases():
+    with np.load(GOLDEN, allow_pickle=False) as d:
+        return [str(n) for n in d['names']]
+
+
+def load_fixture(name):
+    with np.load(GOLDEN, allow_pickle=False) as d:
+        shape = tuple(int(v) for v in d[name + '/shape'])
+        kind = 'truncated' if int(d[name + '/kind']) == 0 else 'filtered'
+        comp = int(d[name + '/comp'])
+        lo, hi = d[name + '/lo'], d[name + '/hi']
+        rec = {k: d[name + '/' + k] for k in
+               ('idx', 'y', 'sum', 'sum_abs', 'sum_sq', 'max_abs')}
+    sym = 3 if len(shape) == 2 else 6
+    x = np.zeros(shape + (sym,))
+    x[tuple(slice(a, b) for a, b in zip(lo, hi)) + (comp,)] = 1.0
+    return kind, x, rec
+
+
+def check_against_fixture(y, rec):
+    """Fixture comparison used for both the oracle and the GPU path."""
+    idx = rec['idx']
+    yv = y[tuple(idx.T)]
+    ye = rec['y']
+    np.testing.assert_allclose(yv, ye, rtol=ULP, atol=10 * ULP)
+    sym = y.shape[-1]
+    np.testing.assert_allclose(y.reshape(-1, sym).sum(axis=0), rec['sum'],
+                               rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.square(y).sum(), rec['sum_sq'], rtol=1e-14)
+    np.testing.assert_allclose(np.abs(y).sum(), rec['sum_abs'], rtol=1e-13)
+
+
+@pytest.mark.parametrize('name', fixture_cases())
+def test_green_apply_fixtures(name):
+    kind, x, rec = load_fixture(name)
+    y = jo.green_apply(x, kind, 0.75, 0.3, 1.0)
+    check_against_fixture(y, rec)
+
+
+def test_tutorial_c1212():
+    """square_basic.py:60-97 with the published result."""
+    n, dim = 256, 2
+    sph_i, dev_i = jo.delta_c_coefficients(100., 0.2, 50., 0.3, dim)
+    sph_m, dev_m = jo.delta_c_coefficients(1., 0.3, 50., 0.3, dim)
+    phase = np.zeros((n, n), dtype=np.uint8)
+    imax = int(np.ceil(n * 0.5 - 0.5))
+    phase[:imax, :imax] = 1
+    tab = np.array([[sph_m, dev_m], [sph_i, dev_i]])
+    E = np.array([0., 0., 1.])
+    eps, res = jo.basic_scheme(tab[phase, 0], tab[phase, 1], E, 'truncated',
+                               50., 0.3, tol=0.0, max_iter=400)
+    assert len(res) == 400
+    tau = jo.block_diag_apply(phase, tab, eps)
+    c1212 = 50. + 0.5 * tau.reshape(-1, 3).mean(axis=0)[-1] / E[-1]
+    assert abs(c1212 - 1.41903971282) < 6e-12
+
+
+# ---------------------------------------------------------------- Gamma0
+
+def _delta(i, j):
+    return 1. if i == j else 0.
+
+
+def tensor_green(k, g, nu):
+    """Closed form of tests/test_greenop.py:21-35 with the Mandel map of
+    janus/mandelvoigt.py:51-63 (independent of the oracle's formulas)."""
+    dim = len(k)
+    sym = (dim * (dim + 1)) // 2
+    k2 = float(np.sum(np.square(k)))
+    if k2 == 0.:
+        return np.zeros((sym, sym))
+    n = np.asarray(k) / np.sqrt(k2)
+    pairs = [(0, 0), (1, 1), (0, 1)] if dim == 2 else \
+        [(0, 0), (1, 1), (2, 2), (1, 2), (2, 0), (0, 1)]
+    a = np.empty((sym, sym))
+    for ij, (i, j) in enumerate(pairs):
+        for kl, (kk, ll) in enumerate(pairs):
+            v = ((_delta(i, kk) * n[j] * n[ll] + _delta(i, ll) * n[j] * n[kk]
+                  + _delta(j, kk) * n[i] * n[ll] + _delta(j, ll) * n[i] * n[kk])
+                 - 2 * n[i] * n[j] * n[kk] * n[ll] / (1. - nu)) / (4 * g)
+            if ij >= dim and kl >= dim:
+                v *= 2.
+            elif ij >= dim or kl >= dim:
+                v *= np.sqrt(2.)
+            a[ij, kl] = v
+    return a
+
+
+def wave_vectors(dim):
+    """tests/test_greenop.py:127-158."""
+    out = []
+    if dim == 2:
+        for theta in np.linspace(0., 2. * np.pi, num=20, endpoint=False):
+            for r in (2.5, 3.5):
+                out.append(np.array([r * np.cos(theta), r * np.sin(theta)]))
+    else:
+        thetas = [(np.pi * i) / 10 for i in range(10)]
+        phis = [(2. * np.pi * i) / 20 for i in range(20)]
+        for phi in phis:
+            for theta in thetas:
+                for r in (2.5, 3.5):
+                    out.append(np.array([r * np.cos(theta) * np.cos(phi),
+                                         r * np.cos(theta) * np.sin(phi),
+                                         r * np.sin(theta)]))
+    out.append(np.zeros(dim))
+    return out
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_gamma0_closed_form(dim):
+    g, nu = 0.7, 0.3
+    for k in wave_vectors(dim):
+        np.testing.assert_array_almost_equal_nulp(
+            tensor_green(k, g, nu), jo.gamma0(k, g, nu), 325)
+
+
+def test_gamma0_published_matrix():
+    """sphinx/materials.rst:53-68, printed to 8 decimals."""
+    expected = np.array([
+        [0.06778426, -0.01457726, -0.03279883, -0.03092304, 0.13606136, 0.09070758],
+        [-0.01457726, 0.22740525, -0.13119534, 0.17935362, -0.06184607, 0.05978454],
+        [-0.03279883, -0.13119534, 0.34766764, 0.02473843, 0.01236921, -0.09276911],
+        [-0.03092304, 0.17935362, 0.02473843, 0.20189504, -0.05976676, 0.0196793],
+        [0.13606136, -0.06184607, 0.01236921, -0.05976676, 0.29154519, 0.17055394],
+        [0.09070758, 0.05978454, -0.09276911, 0.0196793, 0.17055394, 0.14941691]])
+    actual = jo.gamma0(np.array([1., 2., 3.]), 1.0, 0.3)
+    np.testing.assert_allclose(actual, expected, rtol=0, atol=5.1e-9)
+
+
+GRID_SIZES = ([(8, 8), (8, 16), (16, 8), (4, 4, 4)]
+              + list(itertools.permutations((4, 8, 16))))
+
+
+def brute_force_gamma_hat(kind, b, n, g, nu):
+    """tests/test_discrete_green_operator.py:214-222 (truncated) and
+    :249-260 (filtered), built on the tensor closed form."""
+    n = np.asarray(n)
+    b = np.asarray(b)
+    dim = len(n)
+    if kind == 'truncated':
+        i = 2 * b > n
+        k = np.where(i, 2. * np.pi * (b - n) / n, 2. * np.pi * b / n)
+        return tensor_green(k, g, nu)
+    acc = 0.
+    for off in itertools.product(*itertools.repeat(range(-1, 1), dim)):
+        bb = b + np.asarray(off) * n
+        k = 2 * np.pi * bb / n
+        w = np.square(np.prod(np.cos(0.25 * k)))
+        acc = acc + w * tensor_green(k, g, nu)
+    return acc
+
+
+@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+@pytest.mark.parametrize('n', GRID_SIZES)
+def test_gamma_hat_brute_force(kind, n):
+    g, nu = 0.75, 0.3
+    bs = np.array(list(itertools.product(*[range(ni) for ni in n])))
+    actual = jo.gamma_hat(kind, bs, n, 1.0, g, nu)
+    for b, a in zip(bs, actual):
+        expected = brute_force_gamma_hat(kind, b, n, g, nu)
+        # the closed form and janus's expression differ by rounding only
+        np.testing.assert_allclose(a, expected, rtol=1e-13, atol=1e-15)
+
+
+def test_h_cancels():
+    """k = 2 pi b / (h n): the truncated and filtered operators do not
+    depend on h (green.pyx:140-141, 740-800)."""
+    n = (8, 16)
+    bs = np.array(list(itertools.product(*[range(ni) for ni in n])))
+    for kind in ('truncated', 'filtered'):
+        a = jo.gamma_hat(kind, bs, n, 1.0, 0.75, 0.3)
+        b = jo.gamma_hat(kind, bs, n, 1. / 256, 0.75, 0.3)
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-16)
+
+
+# ---------------------------------------------------------------- local ops
+
+@pytest.mark.parametrize('dim', [2, 3])
+@pytest.mark.parametrize('sph, dev', [(1., 0.), (0., 1.), (2.5, -3.5)])
+def test_iso4_projectors(dim, sph, dev):
+    sym = (dim * (dim + 1)) // 2
+    # J_ijkl = d_ij d_kl / dim, K = I - J, Mandel matrices
+    J = np.zeros((sym, sym))
+    J[:dim, :dim] = 1. / dim
+    K = np.eye(sym) - J
+    expected = sph * J + dev * K
+    actual = np.stack([jo.iso4_apply(sph, dev, e) for e in np.eye(sym)], axis=1)
+    np.testing.assert_allclose(actual, expected, rtol=ULP, atol=ULP)
+    np.testing.assert_allclose(jo.iso4_matrix(sph, dev, dim), expected,
+                               rtol=ULP, atol=ULP)
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_block_diag_per_voxel_loop(dim):
+    rng = np.random.default_rng(1)
+    sym = (dim * (dim + 1)) // 2
+    shape = (5, 4, 3)[:dim]
+    table = rng.uniform(-1, 1, size=(4, 2))
+    phase = rng.integers(0, 4, size=shape)
+    x = rng.random(shape + (sym,))
+    y = jo.block_diag_apply(phase, table, x)
+    for index in itertools.product(*map(range, shape)):
+        sph, dev = table[phase[index]]
+        m = jo.iso4_matrix(sph, dev, dim)
+        np.testing.assert_allclose(y[index], m @ x[index], rtol=4 * ULP,
+                                   atol=4 * ULP)
+
+
+@pytest.mark.parametrize('g, k, nu, dim', [(1.5, 3.75, 0.3, 2),
+                                           (1.5, 3.25, 0.3, 3)])
+def test_bulk_modulus(g, k, nu, dim):
+    assert jo.bulk_modulus(g, nu, dim) == k
+
+
+def test_fft_layout():
+    """Interleaved Re/Im along the last axis (sphinx/fft_tutorial.rst:43)."""
+    rng = np.random.default_rng(20150312)
+    a = 2. * rng.random((6, 10)) - 1.
+    c = jo.rfft_janus(a)
+    assert c.shape == (6, 12)
+    ref = np.fft.rfft2(a)
+    np.testing.assert_array_equal(c[:, 0::2], ref.real)
+    np.testing.assert_array_equal(c[:, 1::2], ref.imag)
+    np.testing.assert_allclose(jo.irfft_janus(c, a.shape), a, atol=1e-15)
+
+
+# ---------------------------------------------------------------- adjoint
+
+def test_adjoint_gradient_finite_differences():
+    """dL/dsph, dL/ddev from the adjoint solve vs central differences of
+    the converged forward solve, L = <(dC : eps)_xy>^2-like functional."""
+    rng = np.random.default_rng(7)
+    n, dim = 12, 2
+    g0, nu0 = 5.0, 0.3
+    sph = rng.uniform(-3., 3., (n, n))
+    dev = rng.uniform(-3., 3., (n, n))
+    E = np.array([0.1, -0.2, 1.0])
+    w = rng.standard_normal((n, n, 3))
+
+    def loss(s, d):
+        eps, _ = jo.basic_scheme(s, d, E, 'filtered', g0, nu0, tol=1e-14,
+                                 max_iter=2000)
+        return float((w * eps).sum()), eps
+
+    L0, eps = loss(sph, dev)
+    dsph, ddev, _, res = jo.adjoint_gradient(sph, dev, eps, w, 'filtered', g0,
+                                             nu0, tol=1e-14, max_iter=2000)
+    h = 1e-6
+    for (i, j) in [(0, 0), (3, 7), (11, 5), (6, 6)]:
+        sp, sm = sph.copy(), sph.copy()
+        sp[i, j] += h
+        sm[i, j] -= h
+        fd = (loss(sp, dev)[0] - loss(sm, dev)[0]) / (2 * h)
+        assert abs(fd - dsph[i, j]) < 1e-7 * max(1., abs(fd))
+        dp, dm = dev.copy(), dev.copy()
+        dp[i, j] += h
+        dm[i, j] -= h
+        fd = (loss(sph, dp)[0] - loss(sph, dm)[0]) / (2 * h)
+        assert abs(fd - ddev[i, j]) < 1e-7 * max(1., abs(fd))
