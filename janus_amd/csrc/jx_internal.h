@@ -102,6 +102,10 @@ int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 double *out, const GreenDev *green, const SolveState *st,
                 hipStream_t s);
+// Register-FFT variant of launch_axis for power-of-two lengths 16 ... 512.
+int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
+                    const double *in, double *out, const GreenDev *green,
+                    const SolveState *st, const FftLen &P, hipStream_t s);
 // Inverse transform of the last axis (scaled by 1/N).  When E != nullptr
 // the fixed-point update eps <- E + rhs - y is applied in place on y and
 // per-block partial sums of (eps_new - eps_old)^2 go to partials.

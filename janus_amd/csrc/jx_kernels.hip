@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "jx_fft_lds.h"
+#include "jx_fft_reg.h"
 #include "jx_gamma.h"
 #include "jx_internal.h"
 
@@ -250,6 +251,107 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Gamma-hat at frequency (b0 = e, inner index m) applied to the C complex
+// values z[c * zs] (real and imaginary parts separately).
+template <int C>
+__device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
+                                            int nh, double2 *z, int zs) {
+  constexpr int DIM = C == 3 ? 2 : 3;
+  constexpr int NC = C == 3 ? 6 : 21;
+  int b[3];
+  b[0] = e;
+  if (DIM == 2) {
+    b[1] = (int)m;
+  } else {
+    b[1] = (int)(m / nh);
+    b[2] = (int)(m - (int64_t)b[1] * nh);
+  }
+  double g[NC];
+  gamma_hat<DIM>(G, b, g);
+  double xr[C], xi[C], yr[C], yi[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    double2 v = z[c * zs];
+    xr[c] = v.x;
+    xi[c] = v.y;
+  }
+  sym_matvec<C>(g, xr, yr);
+  sym_matvec<C>(g, xi, yi);
+#pragma unroll
+  for (int c = 0; c < C; ++c) z[c * zs] = make_double2(yr[c], yi[c]);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that tiles sharing cache lines meet in the
+// same L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t nx = 8;
+  int64_t q = nb / nx, r = nb - q * nx;
+  int64_t x = b - (b / nx) * nx, slot = b / nx;
+  return x * q + (x < r ? x : r) + slot;
+}
+
+// Register-resident transform along a strided axis of the spectrum for
+// power-of-two L.  NSEQ = T * C complex sequences per block (T points of the
+// contiguous inner index, all C components), TPP = L / 16 threads each.
+// Thread tid = t * NSEQ + s: consecutive lanes read consecutive doubles of
+// the T * 2C doubles that one axis element of the tile spans.
+template <int L, int C, int NSEQ, int MODE>
+__global__ void __launch_bounds__(NSEQ *(L / 16))
+    k_axis_reg(const double *in, double *out, AxisArgs a, const double2 *W,
+               GreenDev G, const SolveState *st) {
+  if (solve_done(st)) return;
+  constexpr int TPP = L / 16;
+  constexpr int T = NSEQ / C;
+  constexpr int LS = reg_lds_stride(L);
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x;
+  const int s = tid % NSEQ, t = tid / NSEQ;
+  const int tp = s / C, c = s - tp * C;
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t o = blk / a.ntiles;
+  const int64_t m0 = (blk - o * a.ntiles) * T;
+  const int tc = (int)min((int64_t)T, a.n_inner - m0);
+  const bool valid = tp < tc;
+  const double *src = in + o * a.stride_outer + (m0 + tp) * 2 * C + c;
+  double *dst = out + o * a.stride_outer + (m0 + tp) * 2 * C + c;
+  double2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (valid) {
+      const double *p = src + (int64_t)reg_in_pos<L>(t, i) * a.stride_ax;
+      v[i] = make_double2(p[0], p[C]);
+    } else {
+      v[i] = make_double2(0., 0.);
+    }
+  }
+  double2 *seq = smem + s * LS;
+  reg_fft<L, (MODE > 0 ? 1 : -1)>(v, t, seq, W);
+  if constexpr (MODE == 0 && (C == 3 || C == 6)) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+    __syncthreads();
+    for (int w = tid; w < tc * L; w += NSEQ * TPP) {
+      int p = w / L, e = w - p * L;
+      gamma_point<C>(G, e, m0 + p, a.nh, smem + (p * C) * LS + lp(e), LS);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+    __syncthreads();
+    reg_fft<L, 1>(v, t, seq, W);
+  }
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      double *p = dst + (int64_t)reg_out_pos<L>(t, i) * a.stride_ax;
+      p[0] = v[i].x;
+      p[C] = v[i].y;
+    }
+  }
+}
+
 // ------------------------------------------------------------ pass E
 struct LsArgs {
   const double *rhs;
@@ -340,15 +442,198 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// ------------------------------------------------------------ passes A, E
+// Register-resident last-axis transforms for power-of-two L.  A block
+// handles RB rows, i.e. 2 NP real sequences paired into NP complex ones
+// (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / 16 threads
+// per pair.
+
+template <int L, int C, int NP, bool LOCAL>
+__global__ void __launch_bounds__(NP *(L / 16))
+    k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
+                       int64_t nrows, const double2 *W, LocalDev loc,
+                       const SolveState *st) {
+  if (solve_done(st)) return;
+  constexpr int TPP = L / 16, LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int NT = NP * TPP;
+  constexpr int RB = 2 * NP / C;
+  static_assert((2 * NP) % C == 0, "whole rows per block");
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x;
+  const int64_t row0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
+  const int rb = (int)min((int64_t)RB, nrows - row0);
+  // stage the rows (local operator applied) into the pair sequences
+  for (int w = tid; w < RB * L; w += NT) {
+    int r = w / L, j = w - r * L;
+    double t[C];
+    if (r < rb) {
+      int64_t vox = (row0 + r) * L + j;
+      double e[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
+      if (LOCAL) {
+        local_voxel<C>(loc, vox, e, t);
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) t[c] = e[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = 0.;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      ((double *)&smem[(q >> 1) * LS + lp(j)])[q & 1] = t[c];
+    }
+  }
+  __syncthreads();
+  const int p = tid % NP, t = tid / NP;
+  double2 *seq = smem + p * LS;
+  double2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  __syncthreads();
+  reg_fft<L, -1>(v, t, seq, W);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  __syncthreads();
+  // separate the pairs: Xa = (Z[k] + conj Z[-k]) / 2, Xb = (Z[k] - conj Z[-k]) / 2i
+  for (int w = tid; w < rb * NH * C; w += NT) {
+    int c = w % C;
+    int rest = w / C;
+    int k = rest % NH;
+    int r = rest / NH;
+    int q = r * C + c;
+    const double2 *Z = smem + (q >> 1) * LS;
+    double2 zk = Z[lp(k)];
+    double2 zm = Z[lp(k == 0 ? 0 : L - k)];
+    double re, im;
+    if (!(q & 1)) {
+      re = 0.5 * (zk.x + zm.x);
+      im = 0.5 * (zk.y - zm.y);
+    } else {
+      re = 0.5 * (zk.y + zm.y);
+      im = -0.5 * (zk.x - zm.x);
+    }
+    int64_t pt = (row0 + r) * NH + k;
+    spec[pt * 2 * C + c] = re;
+    spec[pt * 2 * C + C + c] = im;
+  }
+}
+
+template <int L, int C, int NP, bool LS_UPDATE>
+__global__ void __launch_bounds__(NP *(L / 16))
+    k_rows_inverse_reg(const double *__restrict__ spec, double *y,
+                       int64_t nrows, double scale, const double2 *W,
+                       LsArgs ls, const SolveState *st) {
+  if (solve_done(st)) return;
+  constexpr int TPP = L / 16, LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int NT = NP * TPP;
+  constexpr int RB = 2 * NP / C;
+  extern __shared__ double2 smem[];
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t row0 = blk * RB;
+  const int rb = (int)min((int64_t)RB, nrows - row0);
+  // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
+  // (0 < k < L/2); DC and Nyquist slots <- (Re Xa, Re Xb), their imaginary
+  // parts dropped as FFTW's and pocketfft's c2r do
+  for (int w = tid; w < RB * NH * C; w += NT) {
+    int c = w % C;
+    int rest = w / C;
+    int k = rest % NH;
+    int r = rest / NH;
+    int q = r * C + c;
+    int part = q & 1;
+    double re = 0., im = 0.;
+    if (r < rb) {
+      int64_t pt = (row0 + r) * NH + k;
+      re = spec[pt * 2 * C + c];
+      im = spec[pt * 2 * C + C + c];
+    }
+    double2 *Z = smem + (q >> 1) * LS;
+    if (k == 0 || 2 * k == L)
+      ((double *)&Z[lp(k)])[part] = re;
+    else if (!part)
+      Z[lp(k)] = make_double2(re, im);
+    else
+      Z[lp(L - k)] = make_double2(re, im);
+  }
+  __syncthreads();
+  constexpr int HALF = L / 2 - 1;
+  if (HALF > 0) {
+    for (int w = tid; w < NP * HALF; w += NT) {
+      int pp = w / HALF;
+      int k = 1 + (w - pp * HALF);
+      double2 *Z = smem + pp * LS;
+      double2 A = Z[lp(k)], B = Z[lp(L - k)];
+      Z[lp(k)] = make_double2(A.x - B.y, A.y + B.x);
+      Z[lp(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();
+  }
+  const int p = tid % NP, t = tid / NP;
+  double2 *seq = smem + p * LS;
+  double2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  __syncthreads();
+  reg_fft<L, 1>(v, t, seq, W);
+  double acc = 0.;
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    const int q = 2 * p + part;
+    const int r = q / C, c = q - (q / C) * C;
+    if (r < rb) {
+      double *yr = y + (row0 + r) * L * C + c;
+      const double *hr = ls.rhs ? ls.rhs + (row0 + r) * L * C + c : nullptr;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
+        double val = scale * (part ? v[i].y : v[i].x);
+        if (LS_UPDATE) {
+          double nv = ls.E[c] - val;
+          if (hr) nv = (ls.E[c] + hr[off]) - val;
+          double d = nv - yr[off];
+          acc += d * d;
+          yr[off] = nv;
+        } else {
+          yr[off] = val;
+        }
+      }
+    }
+  }
+  if (LS_UPDATE) {
+    double sum = block_sum(acc, red);
+    if (tid == 0) ls.partials[blk] = sum;
+  }
+}
+
 // ------------------------------------------------------------ residual
-__global__ void __launch_bounds__(kThreads)
+constexpr int kFinalizeThreads = 1024;
+
+__global__ void __launch_bounds__(kFinalizeThreads)
     k_finalize(const double *partials, int nb, SolveState *st,
                double *res_hist) {
   if (solve_done(st)) return;
-  __shared__ double red[kThreads / 64];
-  double acc = 0.;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[i];
-  double total = block_sum(acc, red);
+  __shared__ double red[kFinalizeThreads / 64];
+  // eight independent accumulators keep eight loads in flight per thread;
+  // the summation order is fixed, so the result is deterministic
+  double acc[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  const int stride = kFinalizeThreads * 8;
+  for (int i0 = 0; i0 < nb; i0 += stride) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int i = i0 + u * kFinalizeThreads + threadIdx.x;
+      if (i < nb) acc[u] += partials[i];
+    }
+  }
+  double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+               ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  double total = block_sum(sum, red);
   if (threadIdx.x == 0) {
     double res = sqrt(total) * st->res_scale;
     int it = st->iter;
@@ -420,19 +705,6 @@ __global__ void __launch_bounds__(kThreads)
 
 // ============================================================ launchers
 
-static int rows_per_block(int L, int C) {
-  int Ls = lds_stride(L);
-  // aim at about 1536 complex entries per ping-pong buffer
-  int nseq = std::max(1, 1536 / Ls);
-  int rb = std::max(1, (2 * nseq) / C);
-  return rb;
-}
-
-static size_t rows_lds_bytes(int L, int C, int RB) {
-  int nseq_max = (RB * C + 1) / 2;
-  return 2 * (size_t)nseq_max * lds_stride(L) * sizeof(double2);
-}
-
 static int check_lds(size_t bytes, const char *what) {
   if ((long long)bytes > lds_limit_bytes())
     return fail(JX_EUNSUPPORTED,
@@ -450,12 +722,129 @@ static int set_lds_attr(K kernel, size_t bytes) {
   return JX_OK;
 }
 
+
+// Rows per block of the register kernels (0 = not available).
+template <int L>
+constexpr int reg_np() {
+  return L <= 256 ? 192 / (L / 16) : 6;
+}
+
+static int reg_rows_per_block(int L, int C) {
+  if (!reg_fft_length(L) || (C != 1 && C != 3 && C != 6)) return 0;
+  int np = L <= 256 ? 192 / (L / 16) : 6;
+  return 2 * np / C;
+}
+
+template <int L, int C>
+static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
+                                      double *spec, const LocalDev &loc,
+                                      const SolveState *st, const double2 *W,
+                                      hipStream_t s) {
+  constexpr int NP = reg_np<L>();
+  constexpr int RB = 2 * NP / C;
+  size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
+  int rc = check_lds(lds, "rows_forward");
+  if (rc) return rc;
+  dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * (L / 16));
+  if (loc.mode != LOCAL_NONE) {
+    if constexpr (C == 3 || C == 6) {
+      set_lds_attr(k_rows_forward_reg<L, C, NP, true>, lds);
+      hipLaunchKernelGGL((k_rows_forward_reg<L, C, NP, true>), grid, block, lds,
+                         s, x, spec, g.rows, W, loc, st);
+    } else {
+      return fail(JX_EINVAL, "local operator needs sym components");
+    }
+  } else {
+    set_lds_attr(k_rows_forward_reg<L, C, NP, false>, lds);
+    hipLaunchKernelGGL((k_rows_forward_reg<L, C, NP, false>), grid, block, lds,
+                       s, x, spec, g.rows, W, loc, st);
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+template <int L, int C>
+static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
+                                      double *y, const LsArgs &la, bool upd,
+                                      const SolveState *st, const double2 *W,
+                                      hipStream_t s) {
+  constexpr int NP = reg_np<L>();
+  constexpr int RB = 2 * NP / C;
+  size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
+  int rc = check_lds(lds, "rows_inverse");
+  if (rc) return rc;
+  dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * (L / 16));
+  double scale = 1.0 / (double)g.N;
+  if (upd) {
+    if constexpr (C == 3 || C == 6) {
+      set_lds_attr(k_rows_inverse_reg<L, C, NP, true>, lds);
+      hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, true>), grid, block,
+                         lds, s, spec, y, g.rows, scale, W, la, st);
+    } else {
+      return fail(JX_EINVAL, "fixed point needs sym components");
+    }
+  } else {
+    set_lds_attr(k_rows_inverse_reg<L, C, NP, false>, lds);
+    hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, false>), grid, block, lds,
+                       s, spec, y, g.rows, scale, W, la, st);
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+#define JX_DISPATCH_LC(FN, ...)                                            \
+  switch (L) {                                                             \
+    case 16: JX_DISPATCH_C(16, FN, __VA_ARGS__);                           \
+    case 32: JX_DISPATCH_C(32, FN, __VA_ARGS__);                           \
+    case 64: JX_DISPATCH_C(64, FN, __VA_ARGS__);                           \
+    case 128: JX_DISPATCH_C(128, FN, __VA_ARGS__);                         \
+    case 256: JX_DISPATCH_C(256, FN, __VA_ARGS__);                         \
+    case 512: JX_DISPATCH_C(512, FN, __VA_ARGS__);                         \
+    default: return fail(JX_EUNSUPPORTED, "no register FFT for length");   \
+  }
+#define JX_DISPATCH_C(LL, FN, ...)                                         \
+  switch (C) {                                                             \
+    case 1: return FN<LL, 1>(__VA_ARGS__);                                 \
+    case 3: return FN<LL, 3>(__VA_ARGS__);                                 \
+    case 6: return FN<LL, 6>(__VA_ARGS__);                                 \
+    default: return fail(JX_EINVAL, "ncomp must be 1, 3 or 6");            \
+  }
+
+static int rows_forward_reg(const Grid &g, int C, const double *x, double *spec,
+                            const LocalDev &loc, const SolveState *st,
+                            const double2 *W, hipStream_t s) {
+  const int L = (int)g.L;
+  JX_DISPATCH_LC(launch_rows_forward_reg_lc, g, x, spec, loc, st, W, s)
+}
+
+static int rows_inverse_reg(const Grid &g, int C, const double *spec, double *y,
+                            const LsArgs &la, bool upd, const SolveState *st,
+                            const double2 *W, hipStream_t s) {
+  const int L = (int)g.L;
+  JX_DISPATCH_LC(launch_rows_inverse_reg_lc, g, spec, y, la, upd, st, W, s)
+}
+
+static int rows_per_block(int L, int C) {
+  int Ls = lds_stride(L);
+  // aim at about 1536 complex entries per ping-pong buffer
+  int nseq = std::max(1, 1536 / Ls);
+  int rb = std::max(1, (2 * nseq) / C);
+  return rb;
+}
+
+static size_t rows_lds_bytes(int L, int C, int RB) {
+  int nseq_max = (RB * C + 1) / 2;
+  return 2 * (size_t)nseq_max * lds_stride(L) * sizeof(double2);
+}
+
 int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
                         const LocalDev &loc, const SolveState *st,
                         hipStream_t s) {
   FftLen P;
   int rc = get_fft_len((int)g.L, &P);
   if (rc) return rc;
+  if (reg_rows_per_block((int)g.L, ncomp))
+    return rows_forward_reg(g, ncomp, x, spec, loc, st, P.W, s);
   RowsArgs a;
   a.nrows = g.rows;
   a.L = (int)g.L;
@@ -510,6 +899,84 @@ static void launch_axis_c(int dir, dim3 grid, size_t lds, hipStream_t s,
   }
 }
 
+template <int L, int C, int NSEQ>
+static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
+                               const double *in, double *out,
+                               const AxisArgs &a, const double2 *W,
+                               const GreenDev &G, const SolveState *st) {
+  dim3 block(NSEQ * (L / 16));
+  if (dir < 0) {
+    set_lds_attr(k_axis_reg<L, C, NSEQ, -1>, lds);
+    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, -1>), grid, block, lds, s, in,
+                       out, a, W, G, st);
+  } else if (dir > 0) {
+    set_lds_attr(k_axis_reg<L, C, NSEQ, 1>, lds);
+    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, 1>), grid, block, lds, s, in,
+                       out, a, W, G, st);
+  } else {
+    set_lds_attr(k_axis_reg<L, C, NSEQ, 0>, lds);
+    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, 0>), grid, block, lds, s, in,
+                       out, a, W, G, st);
+  }
+}
+
+// sequences per block: 192 threads for every length up to 512
+template <int L>
+constexpr int reg_nseq() {
+  return L <= 256 ? 192 / (L / 16) : 6;
+}
+
+template <int L>
+static int launch_axis_reg_l(const Grid &g, int ncomp, int axis, int dir,
+                             const double *in, double *out,
+                             const GreenDev *green, const SolveState *st,
+                             const FftLen &P, hipStream_t s) {
+  constexpr int NSEQ = reg_nseq<L>();
+  static_assert(NSEQ % 6 == 0, "sequences per block must hold whole points");
+  AxisArgs a;
+  a.L = L;
+  a.Ls = reg_lds_stride(L);
+  a.nh = (int)g.nh;
+  int64_t inner = g.nh;
+  for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
+  int64_t outer = 1;
+  for (int d = 0; d < axis; ++d) outer *= g.n[d];
+  a.n_inner = inner;
+  a.n_outer = outer;
+  a.stride_ax = inner * 2 * ncomp;
+  a.stride_outer = (int64_t)L * inner * 2 * ncomp;
+  const int T = NSEQ / ncomp;
+  a.T = T;
+  a.ntiles = (int)((inner + T - 1) / T);
+  size_t lds = (size_t)NSEQ * reg_lds_stride(L) * sizeof(double2);
+  int rc = check_lds(lds, "axis");
+  if (rc) return rc;
+  dim3 grid((unsigned)(outer * a.ntiles));
+  GreenDev G = green ? *green : GreenDev();
+  switch (ncomp) {
+    case 1: launch_axis_reg_lc<L, 1, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st); break;
+    case 3: launch_axis_reg_lc<L, 3, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st); break;
+    case 6: launch_axis_reg_lc<L, 6, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st); break;
+    default: return fail(JX_EINVAL, "launch_axis: ncomp must be 1, 3 or 6");
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
+                    const double *in, double *out, const GreenDev *green,
+                    const SolveState *st, const FftLen &P, hipStream_t s) {
+  switch (P.L) {
+    case 16: return launch_axis_reg_l<16>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 32: return launch_axis_reg_l<32>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 64: return launch_axis_reg_l<64>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 128: return launch_axis_reg_l<128>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 256: return launch_axis_reg_l<256>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 512: return launch_axis_reg_l<512>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    default: return fail(JX_EUNSUPPORTED, "no register FFT for this length");
+  }
+}
+
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 double *out, const GreenDev *green, const SolveState *st,
                 hipStream_t s) {
@@ -520,6 +987,8 @@ int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
   FftLen P;
   int rc = get_fft_len((int)g.n[axis], &P);
   if (rc) return rc;
+  if (reg_fft_length((int)g.n[axis]))
+    return launch_axis_reg(g, ncomp, axis, dir, in, out, green, st, P, s);
   AxisArgs a;
   a.L = (int)g.n[axis];
   a.Ls = lds_stride(a.L);
@@ -553,7 +1022,8 @@ int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
 }
 
 int rows_inverse_nblocks(const Grid &g, int ncomp) {
-  int RB = rows_per_block((int)g.L, ncomp);
+  int RB = reg_rows_per_block((int)g.L, ncomp);
+  if (!RB) RB = rows_per_block((int)g.L, ncomp);
   return (int)((g.rows + RB - 1) / RB);
 }
 
@@ -563,6 +1033,14 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
   FftLen P;
   int rc = get_fft_len((int)g.L, &P);
   if (rc) return rc;
+  if (int RBr = reg_rows_per_block((int)g.L, ncomp)) {
+    LsArgs la;
+    la.rhs = ls.rhs;
+    la.partials = ls.partials;
+    for (int c = 0; c < 6; ++c) la.E[c] = ls.E[c];
+    if (nblocks_out) *nblocks_out = (int)((g.rows + RBr - 1) / RBr);
+    return rows_inverse_reg(g, ncomp, spec, y, la, ls.enabled, st, P.W, s);
+  }
   RowsArgs a;
   a.nrows = g.rows;
   a.L = (int)g.L;
@@ -602,8 +1080,8 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
 
 int launch_finalize(const double *partials, int nblocks, SolveState *st,
                     double *res_hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kThreads), 0, s, partials,
-                     nblocks, st, res_hist);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s,
+                     partials, nblocks, st, res_hist);
   JX_HIP(hipGetLastError());
   return JX_OK;
 }

@@ -54,11 +54,11 @@ def test_iterates_match_oracle(dim, n, kind, m, i):
     sym = solver.sym
     rng = np.random.default_rng(n)
     E = rng.standard_normal(sym)
-    r = solver.solve(E, tol=1e-10, max_iter=300)
+    r = solver.solve(E, tol=1e-10, max_iter=1000)
     assert r.status == 'converged'
     eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1], E,
                                        kind, mat0.g, mat0.nu, tol=1e-10,
-                                       max_iter=300)
+                                       max_iter=1000)
     assert r.iterations == len(res_ref)
     np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-9, atol=1e-15)
     eps = r.eps.cpu().numpy()
