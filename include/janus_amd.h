@@ -131,6 +131,18 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
                     double res_scale, double *res_hist, int *iters,
                     int *status, void *stream);
 
+/* Adjoint contraction (no janus counterpart; implicit differentiation of
+ * the fixed point of jx_solver_solve).  With eps the converged strain and
+ * mu the solution of the adjoint fixed point mu <- G g - G[local : mu]
+ * (g = dL/deps, G = Gamma0), the gradient with respect to the per-voxel
+ * coefficients of local = sph J + dev K is
+ *     dsph = -(J eps) . mu = -tr(eps) tr(mu) / dim,
+ *     ddev = -(K eps) . mu = -(eps . mu) + tr(eps) tr(mu) / dim.
+ * eps, mu: (n0, n1[, n2], sym) device arrays; dsph, ddev: (n0, n1[, n2]). */
+int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
+                        const double *mu, double *dsph, double *ddev,
+                        void *stream);
+
 /* Measurement helper (no janus counterpart): runs `iters` fixed-point
  * iterations exactly as jx_solver_solve does (tol ignored) and records a
  * hipEvent before and after every pass on `stream`.  ms_out[6] receives the

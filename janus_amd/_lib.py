@@ -58,6 +58,8 @@ SIGNATURES = [
     ('jx_solver_solve', ctypes.c_int,
      [_VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
       _VP, _VP, _VP, _VP]),
+    ('jx_adjoint_contract', ctypes.c_int,
+     [ctypes.c_int, _c_int64_p, _VP, _VP, _VP, _VP, _VP]),
     ('jx_solver_profile', ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]),
 ]
 

@@ -1,0 +1,148 @@
+"""Implicit-differentiation adjoint of the fixed-point solve (PyTorch-ROCm).
+
+janus has no gradients; this module adds them on top of the same fused GPU
+iteration.  With G = Gamma0 and dC = sph J + dev K (both symmetric), the
+converged strain solves F(eps, dC) = eps + G[dC : eps] - E = 0.  For a loss
+L(eps) with g = dL/deps:
+
+* the adjoint state lambda solves (I + dC G) lambda = g; mu := G lambda then
+  solves (I + G dC) mu = G g, i.e. the *same* fixed point as the forward
+  solve, mu <- G g - G[dC : mu], with the field G g as right-hand side;
+* dL/dsph_x = -(J eps_x) . mu_x and dL/ddev_x = -(K eps_x) . mu_x (the
+  contraction kernel jx_adjoint_contract);
+* dL/dE = sum_x lambda_x = sum_x (g_x - dC_x : mu_x).
+
+Only the converged eps is kept from the forward pass.  The backward costs
+one Green-operator application, one fixed-point solve and one contraction.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _device, _lib
+from .operators import BlockDiagonalOperator2D, BlockDiagonalOperator3D
+from .solver import LSSolver
+
+
+def _torch():
+    return _device.torch()
+
+
+def _local_from_fields(sph, dev):
+    cls = BlockDiagonalOperator2D if sph.dim() == 2 else BlockDiagonalOperator3D
+    return cls.from_fields(sph, dev)
+
+
+def contract(eps, mu):
+    """(dsph, ddev) = (-(J eps).mu, -(K eps).mu) voxel by voxel (GPU)."""
+    t = _torch()
+    grid = tuple(eps.shape[:-1])
+    dsph = t.empty(grid, dtype=t.float64, device=eps.device)
+    ddev = t.empty(grid, dtype=t.float64, device=eps.device)
+    _lib.check(_lib.lib().jx_adjoint_contract(
+        len(grid), _lib.shape_array(grid), eps.data_ptr(), mu.data_ptr(),
+        dsph.data_ptr(), ddev.data_ptr(), _lib.current_stream()))
+    return dsph, ddev
+
+
+def adjoint_solve(green, local, g, tol, max_iter):
+    """mu solving mu = G g - G[local : mu]; returns (mu, SolveResult)."""
+    t = _torch()
+    gg = green.apply(g.contiguous())
+    norm = float(t.linalg.vector_norm(gg))
+    if norm == 0.0:
+        return t.zeros_like(gg), None
+    mu = gg.clone()
+    solver = LSSolver(green, local)
+    r = solver.solve(np.zeros(gg.shape[-1]), tol=tol, max_iter=max_iter,
+                     eps=mu, rhs=gg, res_scale=1.0 / norm)
+    return mu, r
+
+
+class _LSSolveFunction:
+    """Namespace holding the torch.autograd.Function (built lazily so that
+    importing this module does not import torch)."""
+    fn = None
+
+
+def _make_function():
+    t = _torch()
+
+    class LSSolveFunction(t.autograd.Function):
+        @staticmethod
+        def forward(ctx, sph, dev, E, green, tol, max_iter, info):
+            sph = sph.detach().contiguous()
+            dev = dev.detach().contiguous()
+            local = _local_from_fields(sph, dev)
+            solver = LSSolver(green, local)
+            r = solver.solve(E.detach().cpu().numpy(), tol=tol,
+                             max_iter=max_iter)
+            if info is not None:
+                info['forward'] = r
+            ctx.save_for_backward(sph, dev, r.eps)
+            ctx.green = green
+            ctx.tol = tol
+            ctx.max_iter = max_iter
+            ctx.info = info
+            return r.eps
+
+        @staticmethod
+        def backward(ctx, g):
+            sph, dev, eps = ctx.saved_tensors
+            local = _local_from_fields(sph, dev)
+            mu, r = adjoint_solve(ctx.green, local, g, ctx.tol, ctx.max_iter)
+            if ctx.info is not None:
+                ctx.info['adjoint'] = r
+            dsph, ddev = contract(eps, mu)
+            lam = g.contiguous() - local.apply(mu)
+            dims = tuple(range(lam.dim() - 1))
+            dE = lam.sum(dim=dims)
+            return dsph, ddev, dE, None, None, None, None
+
+    return LSSolveFunction
+
+
+def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None):
+    """Differentiable basic-scheme solve.
+
+    Args:
+        sph, dev: float64 GPU tensors of the grid shape, the per-voxel
+            coefficients of C - C0 = sph J + dev K (may require grad).
+        E: float64 tensor (sym,), macroscopic strain (may require grad).
+        green: a janus_amd.green operator with a transform (reference C0).
+        info: optional dict receiving the SolveResult of the forward
+            ('forward') and adjoint ('adjoint') solves.
+    Returns:
+        eps, the converged strain field (grid + (sym,)).
+    """
+    if _LSSolveFunction.fn is None:
+        _LSSolveFunction.fn = _make_function()
+    return _LSSolveFunction.fn.apply(sph, dev, E, green, tol, max_iter, info)
+
+
+def iso_delta(g, nu, mat0, dim):
+    """(sph, dev) of C(g, nu) - C0 for tensors g, nu (differentiable):
+    sph = dim (k - k0), dev = 2 (g - g0) as square_basic.py:26-29 builds it,
+    with k from isotropic.pyx:79-97."""
+    if dim == 2:
+        k = g / (1. - 2. * nu)
+    else:
+        k = 2. / 3. * (1. + nu) / (1. - 2. * nu) * g
+    return dim * (k - mat0.k), 2. * (g - mat0.g)
+
+
+def mean_stress(sph, dev, eps, E, mat0):
+    """<sigma> = C0 : E + <dC : eps> as a differentiable torch expression."""
+    t = _torch()
+    dim = eps.dim() - 1
+    sym = eps.shape[-1]
+    tr = (sph - dev) / dim
+    trace = eps[..., :dim].sum(dim=-1)
+    tau_n = (tr * trace).unsqueeze(-1) + dev.unsqueeze(-1) * eps[..., :dim]
+    tau_s = dev.unsqueeze(-1) * eps[..., dim:]
+    tau = t.cat([tau_n, tau_s], dim=-1)
+    mean_tau = tau.reshape(-1, sym).mean(dim=0)
+    from .solver import reference_stiffness
+    c0 = t.from_numpy(reference_stiffness(mat0)).to(eps.device)
+    return c0 @ E + mean_tau

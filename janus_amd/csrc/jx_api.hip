@@ -608,6 +608,18 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
   return JX_OK;
 }
 
+int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
+                        const double *mu, double *dsph, double *ddev,
+                        void *stream) {
+  int rc = check_shape(dim, shape);
+  if (rc) return rc;
+  if (!eps || !mu || !dsph || !ddev)
+    return fail(JX_EINVAL, "jx_adjoint_contract: NULL argument");
+  Grid g = make_grid(dim, shape);
+  return launch_adjoint_contract(g, dim == 2 ? 3 : 6, eps, mu, dsph, ddev,
+                                 (hipStream_t)stream);
+}
+
 int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
                       double *ms_out, void *stream) {
   if (!S || !eps || !E || !ms_out || iters < 1)

@@ -128,6 +128,151 @@ JX_HD void gamma_hat(const GreenDev &G, const int *b, double *g) {
   }
 }
 
+// Device evaluation of Gamma-hat through the two moment tensors of the
+// aliases, A_ij = sum_c w_c n_i n_j and B_ijkl = sum_c w_c n_i n_j n_k n_l
+// (n = k / |k| of alias c, weight w_c; one alias of weight 1 for the
+// truncated operator).  Gamma0_ijkl(n) = (d_ik n_j n_l + d_il n_j n_k +
+// d_jk n_i n_l + d_jl n_i n_k) / (4 g) - n_i n_j n_k n_l / (2 g (1 - nu))
+// (the closed form of janus's tests/test_greenop.py:21-35) is linear in
+// n (x) n and n (x) n (x) n (x) n, so the weighted sum of janus's
+// green.pyx:730-910 equals the Mandel assembly below, at about 40 percent
+// of the operations of summing eight full Gamma0 matrices.  Aliases with
+// k = 0 contribute 0 (isotropic.pyx:253-275).
+__device__ __forceinline__ void moments_add2(const double *k, double w,
+                                             double *A, double *B) {
+  double kk = k[0] * k[0] + k[1] * k[1];
+  if (kk == 0.) return;
+  double s = 1.0 / kk;
+  double u[3], t[3];
+  u[0] = s * k[0] * k[0];
+  u[1] = s * k[0] * k[1];
+  u[2] = s * k[1] * k[1];
+  A[0] += w * u[0];
+  A[1] += w * u[1];
+  A[2] += w * u[2];
+  t[0] = w * u[0];
+  t[1] = w * u[1];
+  t[2] = w * u[2];
+  B[0] += t[0] * u[0];
+  B[1] += t[0] * u[1];
+  B[2] += t[0] * u[2];
+  B[3] += t[1] * u[2];
+  B[4] += t[2] * u[2];
+}
+
+__device__ __forceinline__ void moments_add3(const double *k, double w,
+                                             double *A, double *B) {
+  double kk = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+  if (kk == 0.) return;
+  double s = 1.0 / kk;
+  double u[6], t[6];
+  u[0] = s * k[0] * k[0];
+  u[1] = s * k[0] * k[1];
+  u[2] = s * k[0] * k[2];
+  u[3] = s * k[1] * k[1];
+  u[4] = s * k[1] * k[2];
+  u[5] = s * k[2] * k[2];
+  A[0] += w * u[0];
+  A[1] += w * u[1];
+  A[2] += w * u[2];
+  A[3] += w * u[3];
+  A[4] += w * u[4];
+  A[5] += w * u[5];
+  t[0] = w * u[0];
+  t[1] = w * u[1];
+  t[2] = w * u[2];
+  t[3] = w * u[3];
+  t[4] = w * u[4];
+  t[5] = w * u[5];
+  B[0] += t[0] * u[0];
+  B[1] += t[0] * u[1];
+  B[2] += t[0] * u[2];
+  B[3] += t[0] * u[3];
+  B[4] += t[0] * u[4];
+  B[5] += t[0] * u[5];
+  B[6] += t[1] * u[3];
+  B[7] += t[1] * u[4];
+  B[8] += t[1] * u[5];
+  B[9] += t[2] * u[5];
+  B[10] += t[3] * u[3];
+  B[11] += t[3] * u[4];
+  B[12] += t[3] * u[5];
+  B[13] += t[4] * u[5];
+  B[14] += t[5] * u[5];
+}
+
+template <int DIM>
+__device__ __forceinline__ void gamma_hat_moments(const GreenDev &G,
+                                                  const int *b, double *g) {
+  constexpr int NA = DIM == 2 ? 3 : 6;
+  constexpr int NB = DIM == 2 ? 5 : 15;
+  double A[NA], B[NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) A[i] = 0.;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) B[i] = 0.;
+  if (G.kind == JX_GREEN_TRUNCATED) {
+    double k[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) k[i] = G.ax[i].kt[b[i]];
+    if constexpr (DIM == 2) moments_add2(k, 1.0, A, B);
+    else moments_add3(k, 1.0, A, B);
+  } else {
+    double km[DIM], kp[DIM], wm[DIM], wp[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+      km[i] = G.ax[i].km[b[i]];
+      kp[i] = G.ax[i].kp[b[i]];
+      wm[i] = G.ax[i].wm[b[i]];
+      wp[i] = G.ax[i].wp[b[i]];
+    }
+#pragma unroll
+    for (int c = 0; c < (1 << DIM); ++c) {
+      double k[DIM];
+      double w = 1.0;
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        int plus = (c >> (DIM - 1 - i)) & 1;
+        k[i] = plus ? kp[i] : km[i];
+        w = (i == 0) ? (plus ? wp[i] : wm[i]) : w * (plus ? wp[i] : wm[i]);
+      }
+      if constexpr (DIM == 2) moments_add2(k, w, A, B);
+      else moments_add3(k, w, A, B);
+    }
+  }
+  const double d2 = G.daux2, d3 = G.daux3;
+  if constexpr (DIM == 2) {
+    g[0] = (d3 * (A[0] + A[0] + A[0] + A[0]) - d2 * B[0]);
+    g[1] = (-d2 * B[2]);
+    g[2] = kSqrt2 * (d3 * (A[1] + A[1]) - d2 * B[1]);
+    g[3] = (d3 * (A[2] + A[2] + A[2] + A[2]) - d2 * B[4]);
+    g[4] = kSqrt2 * (d3 * (A[1] + A[1]) - d2 * B[3]);
+    g[5] = 2. * (d3 * (A[2] + A[0]) - d2 * B[2]);
+  } else {
+    g[0] = (d3 * (A[0] + A[0] + A[0] + A[0]) - d2 * B[0]);
+    g[1] = (-d2 * B[3]);
+    g[2] = (-d2 * B[5]);
+    g[3] = kSqrt2 * (-d2 * B[4]);
+    g[4] = kSqrt2 * (d3 * (A[2] + A[2]) - d2 * B[2]);
+    g[5] = kSqrt2 * (d3 * (A[1] + A[1]) - d2 * B[1]);
+    g[6] = (d3 * (A[3] + A[3] + A[3] + A[3]) - d2 * B[10]);
+    g[7] = (-d2 * B[12]);
+    g[8] = kSqrt2 * (d3 * (A[4] + A[4]) - d2 * B[11]);
+    g[9] = kSqrt2 * (-d2 * B[7]);
+    g[10] = kSqrt2 * (d3 * (A[1] + A[1]) - d2 * B[6]);
+    g[11] = (d3 * (A[5] + A[5] + A[5] + A[5]) - d2 * B[14]);
+    g[12] = kSqrt2 * (d3 * (A[4] + A[4]) - d2 * B[13]);
+    g[13] = kSqrt2 * (d3 * (A[2] + A[2]) - d2 * B[9]);
+    g[14] = kSqrt2 * (-d2 * B[8]);
+    g[15] = 2. * (d3 * (A[5] + A[3]) - d2 * B[12]);
+    g[16] = 2. * (d3 * (A[1]) - d2 * B[8]);
+    g[17] = 2. * (d3 * (A[2]) - d2 * B[7]);
+    g[18] = 2. * (d3 * (A[0] + A[5]) - d2 * B[5]);
+    g[19] = 2. * (d3 * (A[4]) - d2 * B[4]);
+    g[20] = 2. * (d3 * (A[3] + A[0]) - d2 * B[3]);
+  }
+}
+
 // y = G x for the symmetric matrix stored as an upper triangle.
 template <int SYM>
 JX_HD void sym_matvec(const double *g, const double *x, double *y) {

@@ -123,6 +123,9 @@ int launch_finalize(const double *partials, int nblocks, SolveState *st,
                     double *res_hist, hipStream_t s);
 int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
                        const double *x, double *y, hipStream_t s);
+int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,
+                            const double *mu, double *dsph, double *ddev,
+                            hipStream_t s);
 int launch_local_sums(const Grid &g, int ncomp, const LocalDev &loc,
                       const double *x, double *partials, int *nblocks,
                       hipStream_t s);

@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kThreads)
           b[2] = (int)(m - (int64_t)b[1] * a.nh);
         }
         double g[NC];
-        gamma_hat<DIM>(G, b, g);
+        gamma_hat_moments<DIM>(G, b, g);
         double xr[C], xi[C], yr[C], yi[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -267,7 +267,7 @@ __device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
     b[2] = (int)(m - (int64_t)b[1] * nh);
   }
   double g[NC];
-  gamma_hat<DIM>(G, b, g);
+  gamma_hat_moments<DIM>(G, b, g);
   double xr[C], xi[C], yr[C], yi[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -523,7 +523,9 @@ __global__ void __launch_bounds__(NP *(L / 16))
   }
 }
 
-template <int L, int C, int NP, bool LS_UPDATE>
+// MODE 0: y = c2r(spec); 1: y <- E - c2r(spec) with residual partials;
+// 2: y <- (E + rhs) - c2r(spec) with residual partials.
+template <int L, int C, int NP, int MODE>
 __global__ void __launch_bounds__(NP *(L / 16))
     k_rows_inverse_reg(const double *__restrict__ spec, double *y,
                        int64_t nrows, double scale, const double2 *W,
@@ -532,12 +534,31 @@ __global__ void __launch_bounds__(NP *(L / 16))
   constexpr int TPP = L / 16, LS = reg_lds_stride(L), NH = L / 2 + 1;
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
+  constexpr bool LS_UPDATE = MODE > 0;
   extern __shared__ double2 smem[];
   __shared__ double red[4];
   const int tid = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
+  const int p = tid % NP, t = tid / NP;
+  // the previous iterate (and right-hand side) is loaded first so that its
+  // latency overlaps the transform
+  double old[2][16], rhs[MODE == 2 ? 2 : 1][16];
+  if (LS_UPDATE) {
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const int q = 2 * p + part;
+      const int r = q / C, c = q - (q / C) * C;
+      const int64_t base = (row0 + r) * L * C + c;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        int64_t off = base + (int64_t)reg_out_pos<L>(t, i) * C;
+        old[part][i] = r < rb ? y[off] : 0.;
+        if constexpr (MODE == 2) rhs[part][i] = r < rb ? ls.rhs[off] : 0.;
+      }
+    }
+  }
   // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
   // (0 < k < L/2); DC and Nyquist slots <- (Re Xa, Re Xb), their imaginary
   // parts dropped as FFTW's and pocketfft's c2r do
@@ -575,7 +596,6 @@ __global__ void __launch_bounds__(NP *(L / 16))
     }
     __syncthreads();
   }
-  const int p = tid % NP, t = tid / NP;
   double2 *seq = smem + p * LS;
   double2 v[16];
 #pragma unroll
@@ -589,15 +609,14 @@ __global__ void __launch_bounds__(NP *(L / 16))
     const int r = q / C, c = q - (q / C) * C;
     if (r < rb) {
       double *yr = y + (row0 + r) * L * C + c;
-      const double *hr = ls.rhs ? ls.rhs + (row0 + r) * L * C + c : nullptr;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
         double val = scale * (part ? v[i].y : v[i].x);
-        if (LS_UPDATE) {
+        if constexpr (LS_UPDATE) {
           double nv = ls.E[c] - val;
-          if (hr) nv = (ls.E[c] + hr[off]) - val;
-          double d = nv - yr[off];
+          if constexpr (MODE == 2) nv = (ls.E[c] + rhs[part][i]) - val;
+          double d = nv - old[part][i];
           acc += d * d;
           yr[off] = nv;
         } else {
@@ -660,6 +679,37 @@ __global__ void __launch_bounds__(kThreads)
     local_voxel<C>(loc, v, e, t);
 #pragma unroll
     for (int c = 0; c < C; ++c) y[v * C + c] = t[c];
+  }
+}
+
+// Adjoint contraction of the implicit gradient (no janus counterpart): for
+// dC = sph J + dev K at each voxel, dL/dsph = -(J eps) . mu and
+// dL/ddev = -(K eps) . mu = -(eps . mu - (J eps) . mu), with
+// (J eps) . mu = tr(eps) tr(mu) / dim in Mandel notation.
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_adjoint_contract(const double *eps, const double *mu, int64_t N,
+                       double *dsph, double *ddev) {
+  constexpr int D = C == 3 ? 2 : 3;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double e[C], m[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      e[c] = eps[v * C + c];
+      m[c] = mu[v * C + c];
+    }
+    double tre = 0., trm = 0., dot = 0.;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      tre += e[c];
+      trm += m[c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) dot += e[c] * m[c];
+    double j = tre * trm / (double)D;
+    dsph[v] = -j;
+    ddev[v] = -(dot - j);
   }
 }
 
@@ -777,15 +827,21 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
   double scale = 1.0 / (double)g.N;
   if (upd) {
     if constexpr (C == 3 || C == 6) {
-      set_lds_attr(k_rows_inverse_reg<L, C, NP, true>, lds);
-      hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, true>), grid, block,
-                         lds, s, spec, y, g.rows, scale, W, la, st);
+      if (la.rhs) {
+        set_lds_attr(k_rows_inverse_reg<L, C, NP, 2>, lds);
+        hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 2>), grid, block,
+                           lds, s, spec, y, g.rows, scale, W, la, st);
+      } else {
+        set_lds_attr(k_rows_inverse_reg<L, C, NP, 1>, lds);
+        hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 1>), grid, block,
+                           lds, s, spec, y, g.rows, scale, W, la, st);
+      }
     } else {
       return fail(JX_EINVAL, "fixed point needs sym components");
     }
   } else {
-    set_lds_attr(k_rows_inverse_reg<L, C, NP, false>, lds);
-    hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, false>), grid, block, lds,
+    set_lds_attr(k_rows_inverse_reg<L, C, NP, 0>, lds);
+    hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 0>), grid, block, lds,
                        s, spec, y, g.rows, scale, W, la, st);
   }
   JX_HIP(hipGetLastError());
@@ -1097,6 +1153,22 @@ int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
                        s, x, y, g.N, loc);
   else
     return fail(JX_EINVAL, "local_apply: ncomp must be 3 or 6");
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,
+                            const double *mu, double *dsph, double *ddev,
+                            hipStream_t s) {
+  int64_t nb = std::min<int64_t>((g.N + kThreads - 1) / kThreads, 65536);
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_adjoint_contract<3>, dim3((unsigned)nb),
+                       dim3(kThreads), 0, s, eps, mu, g.N, dsph, ddev);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_adjoint_contract<6>, dim3((unsigned)nb),
+                       dim3(kThreads), 0, s, eps, mu, g.N, dsph, ddev);
+  else
+    return fail(JX_EINVAL, "adjoint_contract: ncomp must be 3 or 6");
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
