@@ -6,12 +6,13 @@
  * the hipStream_t passed as `void *stream` (NULL = the null stream).  All
  * device arrays are FP64, C order, in janus's layouts:
  *
- *   real field      (n0, n1[, n2], ncomp)             ncomp = sym or 1
- *   spectrum        (n0, n1[, n2'], ncomp)            n_last' = 2 (n_last / 2 + 1)
- *                   i.e. per frequency point: Re of the ncomp components,
- *                   then Im of the ncomp components (janus/green.pyx:119-120,
- *                   280-282; for ncomp = 1 the plain interleaved layout of
- *                   janus/fft/serial/_serial_fft.pyx:57-74, 181-200).
+ *   real field      (n0, n1[, n2], sym)               sym strain components
+ *   scalar field    (n0, n1[, n2])                    FFT input / output
+ *   spectrum        (n0, n1[, n2'])                   n_last' = 2 (n_last/2 + 1),
+ *                   Re and Im interleaved along the last axis
+ *                   (janus/fft/serial/_serial_fft.pyx:57-74, 181-200).
+ * The Green operator's spectrum scratch is internal (per frequency, the sym
+ * complex components side by side) and never crosses this boundary.
  *
  * sym = 3 (2D, Mandel order xx, yy, sqrt2 xy) or 6 (3D, xx, yy, zz, sqrt2 yz,
  * sqrt2 zx, sqrt2 xy) -- janus/mandelvoigt.py:26-40.

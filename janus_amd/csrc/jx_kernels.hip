@@ -2,9 +2,10 @@
 //
 // Data layouts (FP64, C order):
 //   real field  (rows, L, C)        rows = n0 [* n1], L = n_last, C comps
-//   spectrum    (rows, nh, 2, C)    nh = L / 2 + 1; Re of the C comps, then
-//                                   Im -- janus's dft_x layout
-//                                   (janus/green.pyx:119-120, 280-282)
+//   spectrum    (rows, nh, C, 2)    nh = L / 2 + 1; complex values, the C
+//                                   components of a frequency side by side
+//                                   (for C = 1: janus's r2c layout,
+//                                   _serial_fft.pyx:57-74, 181-200)
 //
 // Pass structure of one application of Gamma0 (3D; 2D drops pass B/D):
 //   A  rows_forward   last-axis r2c (+ optional local operator)
@@ -158,8 +159,8 @@ __global__ void __launch_bounds__(kThreads)
       im = -0.5 * (zk.x - zm.x);
     }
     int64_t pt = (row0 + r) * nh + k;
-    spec[pt * 2 * C + c] = re;
-    spec[pt * 2 * C + C + c] = im;
+    spec[pt * 2 * C + 2 * c] = re;
+    spec[pt * 2 * C + 2 * c + 1] = im;
   }
 }
 
@@ -192,8 +193,8 @@ __global__ void __launch_bounds__(kThreads)
     int w = v - e * W2;
     int t = w / (2 * C);
     int rem = w - t * 2 * C;
-    int part = rem >= C;
-    int c = rem - part * C;
+    int part = rem & 1;
+    int c = rem >> 1;
     double val = in[base + (int64_t)e * a.stride_ax + w];
     ((double *)&bufA[(t * C + c) * a.Ls + e])[part] = val;
   }
@@ -244,8 +245,8 @@ __global__ void __launch_bounds__(kThreads)
     int w = v - e * W2;
     int t = w / (2 * C);
     int rem = w - t * 2 * C;
-    int part = rem >= C;
-    int c = rem - part * C;
+    int part = rem & 1;
+    int c = rem >> 1;
     out[base + (int64_t)e * a.stride_ax + w] =
         ((const double *)&res[(t * C + c) * a.Ls + e])[part];
   }
@@ -313,14 +314,13 @@ __global__ void __launch_bounds__(NSEQ *(L / 16))
   const int64_t m0 = (blk - o * a.ntiles) * T;
   const int tc = (int)min((int64_t)T, a.n_inner - m0);
   const bool valid = tp < tc;
-  const double *src = in + o * a.stride_outer + (m0 + tp) * 2 * C + c;
-  double *dst = out + o * a.stride_outer + (m0 + tp) * 2 * C + c;
+  const double *src = in + o * a.stride_outer + (m0 + tp) * 2 * C + 2 * c;
+  double *dst = out + o * a.stride_outer + (m0 + tp) * 2 * C + 2 * c;
   double2 v[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (valid) {
-      const double *p = src + (int64_t)reg_in_pos<L>(t, i) * a.stride_ax;
-      v[i] = make_double2(p[0], p[C]);
+      v[i] = *(const double2 *)(src + (int64_t)reg_in_pos<L>(t, i) * a.stride_ax);
     } else {
       v[i] = make_double2(0., 0.);
     }
@@ -345,9 +345,7 @@ __global__ void __launch_bounds__(NSEQ *(L / 16))
   if (valid) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      double *p = dst + (int64_t)reg_out_pos<L>(t, i) * a.stride_ax;
-      p[0] = v[i].x;
-      p[C] = v[i].y;
+      *(double2 *)(dst + (int64_t)reg_out_pos<L>(t, i) * a.stride_ax) = v[i];
     }
   }
 }
@@ -388,8 +386,8 @@ __global__ void __launch_bounds__(kThreads)
     int q = r * C + c;
     int part = q & 1;
     int64_t pt = (row0 + r) * nh + k;
-    double re = spec[pt * 2 * C + c];
-    double im = spec[pt * 2 * C + C + c];
+    double re = spec[pt * 2 * C + 2 * c];
+    double im = spec[pt * 2 * C + 2 * c + 1];
     double2 *Z = bufA + (q >> 1) * a.Ls;
     bool lonely = (!part) && (q + 1 == Q);
     if (k == 0 || 2 * k == L) {
@@ -469,8 +467,14 @@ __global__ void __launch_bounds__(NP *(L / 16))
     if (r < rb) {
       int64_t vox = (row0 + r) * L + j;
       double e[C];
+      if constexpr (C == 6) {
+        const double2 *xv = (const double2 *)(x + vox * 6);
+        double2 a = xv[0], b = xv[1], cc = xv[2];
+        e[0] = a.x; e[1] = a.y; e[2] = b.x; e[3] = b.y; e[4] = cc.x; e[5] = cc.y;
+      } else {
 #pragma unroll
-      for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
+        for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
+      }
       if (LOCAL) {
         local_voxel<C>(loc, vox, e, t);
       } else {
@@ -500,6 +504,25 @@ __global__ void __launch_bounds__(NP *(L / 16))
   for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
   __syncthreads();
   // separate the pairs: Xa = (Z[k] + conj Z[-k]) / 2, Xb = (Z[k] - conj Z[-k]) / 2i
+  if constexpr (C == 6) {
+    // pair p of row r holds components (c0, c0 + 1), adjacent in memory:
+    // two 16-byte stores of complex values
+    for (int w = tid; w < rb * NH * 3; w += NT) {
+      int cp = w % 3;
+      int rest = w / 3;
+      int k = rest % NH;
+      int r = rest / NH;
+      const double2 *Z = smem + (r * 3 + cp) * LS;
+      double2 zk = Z[lp(k)];
+      double2 zm = Z[lp(k == 0 ? 0 : L - k)];
+      double2 xa = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+      double2 xb = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+      double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+      *(double2 *)sp = xa;
+      *(double2 *)(sp + 2) = xb;
+    }
+    return;
+  }
   for (int w = tid; w < rb * NH * C; w += NT) {
     int c = w % C;
     int rest = w / C;
@@ -518,8 +541,8 @@ __global__ void __launch_bounds__(NP *(L / 16))
       im = -0.5 * (zk.x - zm.x);
     }
     int64_t pt = (row0 + r) * NH + k;
-    spec[pt * 2 * C + c] = re;
-    spec[pt * 2 * C + C + c] = im;
+    spec[pt * 2 * C + 2 * c] = re;
+    spec[pt * 2 * C + 2 * c + 1] = im;
   }
 }
 
@@ -542,26 +565,30 @@ __global__ void __launch_bounds__(NP *(L / 16))
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
   const int p = tid % NP, t = tid / NP;
-  // the previous iterate (and right-hand side) is loaded first so that its
-  // latency overlaps the transform
-  double old[2][16], rhs[MODE == 2 ? 2 : 1][16];
-  if (LS_UPDATE) {
-#pragma unroll
-    for (int part = 0; part < 2; ++part) {
-      const int q = 2 * p + part;
-      const int r = q / C, c = q - (q / C) * C;
-      const int64_t base = (row0 + r) * L * C + c;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        int64_t off = base + (int64_t)reg_out_pos<L>(t, i) * C;
-        old[part][i] = r < rb ? y[off] : 0.;
-        if constexpr (MODE == 2) rhs[part][i] = r < rb ? ls.rhs[off] : 0.;
-      }
-    }
-  }
   // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
   // (0 < k < L/2); DC and Nyquist slots <- (Re Xa, Re Xb), their imaginary
   // parts dropped as FFTW's and pocketfft's c2r do
+  if constexpr (C == 6) {
+    for (int w = tid; w < RB * NH * 3; w += NT) {
+      int cp = w % 3;
+      int rest = w / 3;
+      int k = rest % NH;
+      int r = rest / NH;
+      double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
+      if (r < rb) {
+        const double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+        xa = *(const double2 *)sp;
+        xb = *(const double2 *)(sp + 2);
+      }
+      double2 *Z = smem + (r * 3 + cp) * LS;
+      if (k == 0 || 2 * k == L) {
+        Z[lp(k)] = make_double2(xa.x, xb.x);
+      } else {
+        Z[lp(k)] = xa;
+        Z[lp(L - k)] = xb;
+      }
+    }
+  } else
   for (int w = tid; w < RB * NH * C; w += NT) {
     int c = w % C;
     int rest = w / C;
@@ -572,8 +599,8 @@ __global__ void __launch_bounds__(NP *(L / 16))
     double re = 0., im = 0.;
     if (r < rb) {
       int64_t pt = (row0 + r) * NH + k;
-      re = spec[pt * 2 * C + c];
-      im = spec[pt * 2 * C + C + c];
+      re = spec[pt * 2 * C + 2 * c];
+      im = spec[pt * 2 * C + 2 * c + 1];
     }
     double2 *Z = smem + (q >> 1) * LS;
     if (k == 0 || 2 * k == L)
@@ -603,24 +630,56 @@ __global__ void __launch_bounds__(NP *(L / 16))
   __syncthreads();
   reg_fft<L, 1>(v, t, seq, W);
   double acc = 0.;
-#pragma unroll
-  for (int part = 0; part < 2; ++part) {
-    const int q = 2 * p + part;
-    const int r = q / C, c = q - (q / C) * C;
+  if constexpr (C == 6) {
+    // pair p = components (c0, c0 + 1) of row r: 16-byte accesses
+    const int r = p / 3, c0 = 2 * (p - r * 3);
     if (r < rb) {
-      double *yr = y + (row0 + r) * L * C + c;
+      double2 *yr = (double2 *)(y + (row0 + r) * L * 6 + c0);
+      const double2 *hr =
+          MODE == 2 ? (const double2 *)(ls.rhs + (row0 + r) * L * 6 + c0)
+                    : nullptr;
+      const double e0 = ls.E[c0], e1 = ls.E[c0 + 1];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
-        double val = scale * (part ? v[i].y : v[i].x);
+        int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
+        double2 val = make_double2(scale * v[i].x, scale * v[i].y);
         if constexpr (LS_UPDATE) {
-          double nv = ls.E[c] - val;
-          if constexpr (MODE == 2) nv = (ls.E[c] + rhs[part][i]) - val;
-          double d = nv - old[part][i];
-          acc += d * d;
+          double2 nv = make_double2(e0 - val.x, e1 - val.y);
+          if constexpr (MODE == 2) {
+            double2 h = hr[off];
+            nv = make_double2((e0 + h.x) - val.x, (e1 + h.y) - val.y);
+          }
+          double2 o = yr[off];
+          double dx = nv.x - o.x, dy = nv.y - o.y;
+          acc += dx * dx;
+          acc += dy * dy;
           yr[off] = nv;
         } else {
           yr[off] = val;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const int q = 2 * p + part;
+      const int r = q / C, c = q - (q / C) * C;
+      if (r < rb) {
+        double *yr = y + (row0 + r) * L * C + c;
+        const double *hr = MODE == 2 ? ls.rhs + (row0 + r) * L * C + c : nullptr;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
+          double val = scale * (part ? v[i].y : v[i].x);
+          if constexpr (LS_UPDATE) {
+            double nv = ls.E[c] - val;
+            if constexpr (MODE == 2) nv = (ls.E[c] + hr[off]) - val;
+            double d = nv - yr[off];
+            acc += d * d;
+            yr[off] = nv;
+          } else {
+            yr[off] = val;
+          }
         }
       }
     }
