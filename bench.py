@@ -32,21 +32,37 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 PASS_NAMES = ['A_rows_r2c_local', 'B_axis1_fwd', 'C_axis0_gamma',
               'D_axis1_inv', 'E_rows_c2r_update', 'residual_reduce']
+PASS_NAMES_FUSED = ['A_rows_r2c_local', 'B_axis1_fwd', 'C_axis0_gamma',
+                    'D_axis1_inv', 'EA_rows_c2r_update_local_r2c',
+                    'residual_reduce']
 
 
-def pass_bytes(n, sym=6, m=1):
+def pass_bytes(n, fused, sym=6, m=1):
     """Algorithmic HBM bytes of each pass of one iteration (3D, FP64):
-    real field 8 sym B/voxel, spectrum 16 sym B/point, phase id m B/voxel."""
+    real field 8 sym B/voxel, spectrum 16 sym B/point, phase id m B/voxel.
+    Fused schedule: pass A is folded into pass E of the previous iteration."""
     N = n[0] * n[1] * n[2]
     Nh = n[0] * n[1] * (n[2] // 2 + 1)
     f = 8 * sym
     s = 16 * sym
+    if fused:
+        return [0, 2 * s * Nh, 2 * s * Nh, 2 * s * Nh,
+                # EA: read spectrum + eps_old + phase, write eps + spectrum
+                2 * s * Nh + (2 * f + m) * N, 0]
     return [(f + m) * N + s * Nh,      # A: read eps + phase, write spectrum
             2 * s * Nh,                # B
             2 * s * Nh,                # C
             2 * s * Nh,                # D
             s * Nh + 2 * f * N,        # E: read spectrum + eps_old, write eps
             0]
+
+
+def survey_bytes(n, sym=6, m=1):
+    """B_alg = (144 + m) N + 768 Nh, the 5-pass per-iteration byte count
+    the roofline target is stated against."""
+    N = n[0] * n[1] * n[2]
+    Nh = n[0] * n[1] * (n[2] // 2 + 1)
+    return (16 * sym + 8 * sym + m) * N + 8 * 16 * sym * Nh
 
 
 def build_workload(n, kind):
@@ -90,6 +106,36 @@ def cpu_baseline(n, kind, phase, table, mat0, iters=1):
     return iters / t, t
 
 
+def stream_copy_gbps(mib=2048, reps=10):
+    """Bandwidth (read + write bytes / time) of the library's 16-byte
+    stream-copy kernel: the practical HBM ceiling next to the 8 TB/s
+    data-sheet peak."""
+    import torch
+    from janus_amd import _lib
+    nbytes = mib * (1 << 20)
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    lib = _lib.lib()
+    stream = _lib.current_stream()
+
+    def copy():
+        _lib.check(lib.jx_stream_copy(a.data_ptr(), b.data_ptr(), nbytes,
+                                      stream))
+    copy()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        copy()
+    end.record()
+    torch.cuda.synchronize()
+    sec = start.elapsed_time(end) * 1e-3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / sec / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -102,6 +148,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true',
                     help='skip the CPU baseline leg')
     ap.add_argument('--cpu-iters', type=int, default=1)
+    ap.add_argument('--copy-mb', type=int, default=0,
+                    help='after the bench, copy a buffer of this size '
+                         '(reference kernel for HBM traffic counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only; the multi-GPU '
@@ -138,17 +187,23 @@ def main():
     _lib.check(_lib.lib().jx_solver_profile(
         solver._handle, eps.data_ptr(), E.ctypes.data, args.profile_iters,
         ms.ctypes.data, _lib.current_stream()))
-    nbytes = pass_bytes((n, n, n))
+    fused = solver.fused
+    names = PASS_NAMES_FUSED if fused else PASS_NAMES
+    nbytes = pass_bytes((n, n, n), fused)
     dom = int(np.argmax(ms[:5]))
     achieved = nbytes[dom] / (ms[dom] * 1e-3) / 1e9
     iter_bytes = sum(nbytes)
     iter_ms = 1e3 / its
-    passes = {PASS_NAMES[i]: {'ms': round(float(ms[i]), 4),
+    passes = {names[i]: {'ms': round(float(ms[i]), 4),
                               'bytes': int(nbytes[i]),
                               'GBps': round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)
                               if ms[i] > 0 and nbytes[i] else None}
-              for i in range(6)}
+              for i in range(6) if ms[i] > 0 or not fused or i != 0}
     print('bench: passes ' + json.dumps(passes), file=sys.stderr, flush=True)
+
+    copy_gbps = stream_copy_gbps()
+    print('bench: stream copy %.1f GB/s' % copy_gbps, file=sys.stderr,
+          flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -181,15 +236,21 @@ def main():
                         % (n, args.kind, 16.0 * n / 256.0),
             'grid': [n, n, n],
             'green': args.kind,
+            'schedule': 'fused E+A' if fused else 'five passes',
             'iteration_bytes_alg': int(iter_bytes),
             'iteration_GBps': round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
             'iteration_roofline_frac': round(
                 iter_bytes / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            'B_alg_5pass': int(survey_bytes((n, n, n))),
+            'B_alg_5pass_roofline_frac': round(
+                survey_bytes((n, n, n)) / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                4),
             'passes': passes,
+            'hbm_copy_GBps': round(copy_gbps, 1),
         },
         'roofline': {
             'bound': 'hbm',
-            'kernel': PASS_NAMES[dom],
+            'kernel': names[dom],
             'achieved': round(achieved, 1),
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
@@ -199,6 +260,16 @@ def main():
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_mb > 0:
+        # plain device-to-device copy of known size, outside the timed region
+        nel = args.copy_mb * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device='cuda')
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print('bench: copied %d MiB x 3' % args.copy_mb, file=sys.stderr,
+              flush=True)
 
 
 if __name__ == '__main__':

@@ -131,6 +131,14 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
                     const double *rhs, double tol, int max_iter,
                     double res_scale, double *res_hist, int *iters,
                     int *status, void *stream);
+/* Schedule choice (no janus counterpart).  fused = 1 (default): the last
+ * pass of an iteration also applies the local operator to the new iterate
+ * and transforms it for the next iteration, so the iterate is written once
+ * and not read back; fused = 0: five separate passes per iteration.  Both
+ * give the same iterates.  get returns the schedule actually used (the fused
+ * pass needs a last-axis length 16 ... 512, a power of two). */
+int jx_solver_set_fused(jx_solver *solver, int fused);
+int jx_solver_get_fused(jx_solver *solver);
 
 /* Adjoint contraction (no janus counterpart; implicit differentiation of
  * the fixed point of jx_solver_solve).  With eps the converged strain and
@@ -144,13 +152,21 @@ int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
                         const double *mu, double *dsph, double *ddev,
                         void *stream);
 
+/* Measurement helper (no janus counterpart): device-to-device copy of
+ * `bytes` (a multiple of 16; 16-byte aligned pointers) with one 16-byte load
+ * and store per lane -- the reference for the practical HBM bandwidth and
+ * for the calibration of traffic counters. */
+int jx_stream_copy(const void *src, void *dst, int64_t bytes, void *stream);
+
 /* Measurement helper (no janus counterpart): runs `iters` fixed-point
- * iterations exactly as jx_solver_solve does (tol ignored) and records a
- * hipEvent before and after every pass on `stream`.  ms_out[6] receives the
- * average duration in milliseconds of the passes: 0 last-axis r2c with the
- * local operator, 1 axis-1 forward (3D), 2 axis-0 forward + Gamma-hat +
- * inverse, 3 axis-1 inverse (3D), 4 last-axis c2r with the fixed-point
- * update, 5 residual reduction.  Synchronizes the stream. */
+ * iterations exactly as jx_solver_solve does (tol ignored) after one untimed
+ * iteration, and records a hipEvent before and after every pass on
+ * `stream`.  ms_out[6] receives the average duration in milliseconds of the
+ * passes: 0 last-axis r2c with the local operator (0 in the fused
+ * schedule), 1 axis-1 forward (3D), 2 axis-0 forward + Gamma-hat + inverse,
+ * 3 axis-1 inverse (3D), 4 last-axis c2r with the fixed-point update (fused:
+ * plus the local operator and r2c of the next iteration), 5 residual
+ * reduction.  Synchronizes the stream. */
 int jx_solver_profile(jx_solver *solver, double *eps, const double *E,
                       int iters, double *ms_out, void *stream);
 

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'libjanus_amd.so')
+# JANUS_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get('JANUS_AMD_LIB',
+                          os.path.join(_HERE, 'libjanus_amd.so'))
 
 JX_OK = 0
 JX_EINVAL = -1
@@ -60,6 +62,9 @@ SIGNATURES = [
       _VP, _VP, _VP, _VP]),
     ('jx_adjoint_contract', ctypes.c_int,
      [ctypes.c_int, _c_int64_p, _VP, _VP, _VP, _VP, _VP]),
+    ('jx_stream_copy', ctypes.c_int, [_VP, _VP, ctypes.c_int64, _VP]),
+    ('jx_solver_set_fused', ctypes.c_int, [_VP, ctypes.c_int]),
+    ('jx_solver_get_fused', ctypes.c_int, [_VP]),
     ('jx_solver_profile', ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]),
 ]
 

@@ -21,11 +21,14 @@ def hipcc():
     return os.path.join(rocm, 'bin', 'hipcc')
 
 
-def build_library(verbose=False):
-    """Compile every HIP source into the shared library (always rebuilds)."""
+def build_library(verbose=False, output=OUTPUT, defines=()):
+    """Compile every HIP source into the shared library (always rebuilds).
+
+    ``defines`` are extra ``NAME=VALUE`` macros (tuning variants)."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUTPUT + '.tmp'
-    cmd = [hipcc()] + HIPCC_FLAGS + ['-o', tmp] + srcs
+    tmp = output + '.tmp'
+    cmd = [hipcc()] + HIPCC_FLAGS + ['-D' + d for d in defines] + \
+        ['-o', tmp] + srcs
     if verbose:
         print(' '.join(cmd), flush=True)
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
@@ -36,9 +39,17 @@ def build_library(verbose=False):
         raise RuntimeError('hipcc failed:\n' + out)
     if verbose and out.strip():
         print(out)
-    os.replace(tmp, OUTPUT)
-    return OUTPUT
+    os.replace(tmp, output)
+    return output
 
 
 if __name__ == '__main__':
-    print(build_library(verbose='-v' in sys.argv))
+    args = [a for a in sys.argv[1:] if a != '-v']
+    out = OUTPUT
+    defs = []
+    for a in args:
+        if a.startswith('-o='):
+            out = os.path.abspath(a[3:])
+        elif a.startswith('-D'):
+            defs.append(a[2:])
+    print(build_library(verbose='-v' in sys.argv, output=out, defines=defs))

@@ -184,6 +184,7 @@ static int local_alloc_partials(jx_local *l) {
 struct jx_solver {
   jx_green *green = nullptr;
   jx_local *local = nullptr;
+  bool fused = true;  // pass E of one iteration fused with pass A of the next
   SolveState *st = nullptr;       // device
   SolveState *st_host = nullptr;  // pinned host mirror
   double *partials = nullptr;
@@ -541,6 +542,43 @@ int jx_solver_destroy(jx_solver *S) {
   return JX_OK;
 }
 
+// One fixed-point iteration.  Fused schedule: pass A runs only on the first
+// iteration; every iteration then runs B, C, D and the fused E+A pass, which
+// leaves the spectrum of the next iteration's polarization in place.
+static int solver_iteration(jx_solver *S, double *eps, const RowsInverseLs &ls,
+                            bool first, int *nb, hipStream_t s) {
+  jx_green *G = S->green;
+  const Grid &g = G->g;
+  const int C = G->sym;
+  if (!(S->fused && rows_fusable(g, C)))
+    return green_passes(G, eps, eps, S->local->dev, ls, S->st, nb, s);
+  int rc;
+  if (first && (rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev,
+                                         S->st, s)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s)))
+    return rc;
+  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s)))
+    return rc;
+  return launch_rows_update_forward(g, C, G->spec, eps, ls, S->local->dev,
+                                    S->st, nb, s);
+}
+
+int jx_solver_set_fused(jx_solver *S, int fused) {
+  if (!S) return fail(JX_EINVAL, "jx_solver_set_fused: NULL solver");
+  S->fused = fused != 0;
+  return JX_OK;
+}
+
+int jx_solver_get_fused(jx_solver *S) {
+  if (!S) return -1;
+  return (S->fused && rows_fusable(S->green->g, S->green->sym)) ? 1 : 0;
+}
+
 int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rhs,
                     double tol, int max_iter, double res_scale, double *res_hist,
                     int *iters, int *status, void *stream) {
@@ -590,7 +628,7 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
     int n = std::min(chunk, max_iter - enqueued);
     for (int i = 0; i < n; ++i) {
       int nb = 0;
-      int rc = green_passes(G, eps, eps, S->local->dev, ls, S->st, &nb, s);
+      int rc = solver_iteration(S, eps, ls, enqueued + i == 0, &nb, s);
       if (rc) return rc;
       if ((rc = launch_finalize(S->partials, nb, S->st, S->res_dev, s))) return rc;
     }
@@ -606,6 +644,14 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
   if (res_hist && it > 0)
     JX_HIP(hipMemcpy(res_hist, S->res_dev, sizeof(double) * it, hipMemcpyDeviceToHost));
   return JX_OK;
+}
+
+int jx_stream_copy(const void *src, void *dst, int64_t bytes, void *stream) {
+  if (!src || !dst || bytes < 0 || bytes % 16)
+    return fail(JX_EINVAL, "jx_stream_copy: bytes must be a multiple of 16");
+  if (((uintptr_t)src | (uintptr_t)dst) % 16)
+    return fail(JX_EINVAL, "jx_stream_copy: pointers must be 16-byte aligned");
+  return launch_stream_copy(src, dst, bytes / 16, (hipStream_t)stream);
 }
 
 int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
@@ -628,19 +674,19 @@ int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
   jx_green *G = S->green;
   const Grid &g = G->g;
   int C = G->sym;
-  if (S->res_cap < iters) {
+  if (S->res_cap < iters + 1) {
     if (S->res_dev) (void)hipFree(S->res_dev);
     S->res_dev = nullptr;
     S->res_cap = 0;
-    JX_HIP(hipMalloc((void **)&S->res_dev, sizeof(double) * iters));
-    S->res_cap = iters;
+    JX_HIP(hipMalloc((void **)&S->res_dev, sizeof(double) * (iters + 1)));
+    S->res_cap = iters + 1;
   }
   SolveState init;
   init.iter = 0;
   init.done = 0;
   init.tol = 0.;
   init.res_scale = 1.;
-  init.max_iter = iters;
+  init.max_iter = iters + 1;
   init.pad = 0;
   *S->st_host = init;
   JX_HIP(hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState), hipMemcpyHostToDevice, s));
@@ -649,14 +695,18 @@ int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
   ls.partials = S->partials;
   for (int c = 0; c < C; ++c) ls.E[c] = E[c];
   const int NP = 6;
+  const bool fused = S->fused && rows_fusable(g, C);
   std::vector<hipEvent_t> ev((size_t)iters * (NP + 1));
   for (auto &e : ev) JX_HIP(hipEventCreate(&e));
-  int rc = JX_OK;
+  // one untimed iteration (it carries pass A in the fused schedule)
+  int nb0 = 0;
+  int rc = solver_iteration(S, eps, ls, true, &nb0, s);
+  if (!rc) rc = launch_finalize(S->partials, nb0, S->st, S->res_dev, s);
   for (int it = 0; it < iters && !rc; ++it) {
     hipEvent_t *e = &ev[(size_t)it * (NP + 1)];
     int nb = 0;
     (void)hipEventRecord(e[0], s);
-    rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev, S->st, s);
+    if (!fused) rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev, S->st, s);
     (void)hipEventRecord(e[1], s);
     if (!rc && g.dim == 3) rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s);
     (void)hipEventRecord(e[2], s);
@@ -664,7 +714,10 @@ int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
     (void)hipEventRecord(e[3], s);
     if (!rc && g.dim == 3) rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s);
     (void)hipEventRecord(e[4], s);
-    if (!rc) rc = launch_rows_inverse(g, C, G->spec, eps, ls, S->st, &nb, s);
+    if (!rc)
+      rc = fused ? launch_rows_update_forward(g, C, G->spec, eps, ls,
+                                              S->local->dev, S->st, &nb, s)
+                 : launch_rows_inverse(g, C, G->spec, eps, ls, S->st, &nb, s);
     (void)hipEventRecord(e[5], s);
     if (!rc) rc = launch_finalize(S->partials, nb, S->st, S->res_dev, s);
     (void)hipEventRecord(e[6], s);

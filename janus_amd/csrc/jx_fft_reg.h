@@ -11,7 +11,8 @@
 // through LDS.  The first stage reads x[t + TPP r] (r < 16) and the last
 // one leaves X[t + TPP q + r L/R_last] in registers, so loads and stores
 // are plain strided patterns.  Stage twiddles come from the table
-// W[m] = exp(-2 pi i m / L) (one correctly rounded entry per factor).
+// W[m] = exp(-2 pi i m / L) (one correctly rounded entry per factor), read
+// from global memory or from a quarter table staged in LDS.
 #pragma once
 
 #include "jx_fft_lds.h"
@@ -82,8 +83,38 @@ __device__ __forceinline__ void dft_r(double2 *v) {
   else if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
 }
 
+// Twiddle source of the register kernels.  TQ: the kernel keeps the first
+// quarter of the table, W[m] for m < L/4, in LDS and rebuilds
+// W[m] = (-i)^(m div L/4) W[m mod L/4] (exact: a swap and sign changes);
+// otherwise W is the global table.
+template <int L, int DIR, bool TQ>
+__device__ __forceinline__ double2 reg_twiddle(const double2 *W, int m) {
+  if constexpr (TQ) {
+    constexpr int Q = L / 4;
+    int quad = m / Q;
+    double2 w = W[m - quad * Q];
+    double2 r;
+    switch (quad & 3) {
+      case 0: r = w; break;
+      case 1: r = make_double2(w.y, -w.x); break;
+      case 2: r = make_double2(-w.x, -w.y); break;
+      default: r = make_double2(-w.y, w.x); break;
+    }
+    if (DIR > 0) r.y = -r.y;
+    return r;
+  } else {
+    return twiddle<DIR>(W, m);
+  }
+}
+
+// Entries of the twiddle table a register kernel stages into LDS.
+template <int L, bool TQ>
+constexpr int reg_twiddle_entries() {
+  return TQ ? L / 4 : 0;
+}
+
 // One stage on the registers: twiddle, then 16 / R butterflies of radix R.
-template <int L, int R, int NSP, int DIR>
+template <int L, int R, int NSP, int DIR, bool TQ>
 __device__ __forceinline__ void reg_stage(double2 *v, int t, const double2 *W) {
   constexpr int TPP = L / 16;
   constexpr int tws = L / (NSP * R);
@@ -94,7 +125,7 @@ __device__ __forceinline__ void reg_stage(double2 *v, int t, const double2 *W) {
       int k = j % NSP;
 #pragma unroll
       for (int r = 1; r < R; ++r)
-        v[q * R + r] = cmul(v[q * R + r], twiddle<DIR>(W, r * k * tws));
+        v[q * R + r] = cmul(v[q * R + r], reg_twiddle<L, DIR, TQ>(W, r * k * tws));
     }
     dft_r<R, DIR>(&v[q * R]);
   }
@@ -154,23 +185,23 @@ __device__ __forceinline__ int reg_in_pos(int t, int i) {
 // v[i] = X[reg_out_pos(t, i)].  `seq` is this sequence's LDS scratch of
 // L entries; the caller's block must reach the barriers inside (every thread
 // of the block calls this function the same number of times).
-template <int L, int DIR>
+template <int L, int DIR, bool TQ>
 __device__ __forceinline__ void reg_fft(double2 *v, int t, double2 *seq,
                                         const double2 *W) {
   using P = RegPlan<L>;
-  reg_stage<L, P::r0, 1, DIR>(v, t, W);
+  reg_stage<L, P::r0, 1, DIR, TQ>(v, t, W);
   if constexpr (P::n >= 2) {
     reg_store_stage<L, P::r0, 1>(v, t, seq);
     __syncthreads();
     reg_load_stage<L, P::r1>(v, t, seq);
-    reg_stage<L, P::r1, P::r0, DIR>(v, t, W);
+    reg_stage<L, P::r1, P::r0, DIR, TQ>(v, t, W);
   }
   if constexpr (P::n >= 3) {
     __syncthreads();
     reg_store_stage<L, P::r1, P::r0>(v, t, seq);
     __syncthreads();
     reg_load_stage<L, P::r2>(v, t, seq);
-    reg_stage<L, P::r2, P::r0 * P::r1, DIR>(v, t, W);
+    reg_stage<L, P::r2, P::r0 * P::r1, DIR, TQ>(v, t, W);
   }
 }
 

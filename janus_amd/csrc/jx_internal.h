@@ -119,6 +119,12 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
                         const RowsInverseLs &ls, const SolveState *st,
                         int *nblocks_out, hipStream_t s);
 int rows_inverse_nblocks(const Grid &g, int ncomp);
+// Fused pass E of iteration k + pass A of iteration k + 1 (register path).
+bool rows_fusable(const Grid &g, int ncomp);
+int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,
+                               double *y, const RowsInverseLs &ls,
+                               const LocalDev &loc, const SolveState *st,
+                               int *nblocks_out, hipStream_t s);
 int launch_finalize(const double *partials, int nblocks, SolveState *st,
                     double *res_hist, hipStream_t s);
 int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
@@ -129,6 +135,8 @@ int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,
 int launch_local_sums(const Grid &g, int ncomp, const LocalDev &loc,
                       const double *x, double *partials, int *nblocks,
                       hipStream_t s);
+int launch_stream_copy(const void *src, void *dst, int64_t n16,
+                       hipStream_t s);
 int launch_reduce_columns(const double *partials, int nblocks, int ncol,
                           double *out, hipStream_t s);
 

@@ -7,13 +7,15 @@ sphinx/tutorials/square_basic/square_basic.py:83-97:
                       res = ||new - eps|| / (sqrt(N) ||E||);  eps = new
 
 :class:`LSSolver` runs exactly that loop on the GPU through
-``jx_solver_solve``: every iteration is five fused passes (local operator
-fused into the first FFT pass, Gamma-hat between the forward and inverse
-transforms of axis 0, ``E - .`` and the residual partial sums fused into the
-last pass), the residual is reduced on the device and the host reads the
-solver state only once per chunk of iterations.  The iteration count is the
-one the host loop would give: the loop stops after the first iteration whose
-residual is below ``tol``.
+``jx_solver_solve``.  An iteration is a chain of passes over HBM: the local
+operator fused into the last-axis r2c, the other axes' forward transforms,
+Gamma-hat recomputed between the forward and inverse transforms of axis 0,
+the inverse transforms, and the last-axis c2r fused with ``E - .``, the
+residual partial sums and -- in the default fused schedule -- the local
+operator and r2c of the next iteration.  The residual is reduced on the
+device and the host reads the solver state once per chunk of iterations.
+The iteration count is the one the host loop would give: the loop stops
+after the first iteration whose residual is below ``tol``.
 """
 
 import ctypes
@@ -62,6 +64,16 @@ class LSSolver:
         if h is not None and _lib._lib is not None:
             _lib._lib.jx_solver_destroy(h)
             self._handle = None
+
+    @property
+    def fused(self):
+        """True when the last pass of an iteration also prepares the next
+        one (the default wherever the register kernels apply)."""
+        return _lib.lib().jx_solver_get_fused(self._handle) == 1
+
+    @fused.setter
+    def fused(self, value):
+        _lib.check(_lib.lib().jx_solver_set_fused(self._handle, int(bool(value))))
 
     def solve(self, E, tol=1e-10, max_iter=200, eps=None, rhs=None,
               res_scale=None):

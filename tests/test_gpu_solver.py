@@ -121,3 +121,44 @@ def test_field_rhs_matches_oracle():
     assert r.iterations == len(res)
     np.testing.assert_allclose(r.eps.cpu().numpy(), mu, rtol=0,
                                atol=1e-12 * np.abs(mu).max())
+
+
+@pytest.mark.parametrize('dim, n, kind, with_rhs', [
+    (2, 64, 'filtered', False), (3, 32, 'truncated', False),
+    (3, 16, 'filtered', True), (2, 32, 'truncated', True)])
+def test_fused_schedule_matches_five_passes(dim, n, kind, with_rhs):
+    """The fused schedule (pass E of one iteration with pass A of the next)
+    produces the same iterates as the five separate passes, bit for bit."""
+    import torch
+    phase = ms.square_inclusion(n, 0.5) if dim == 2 else \
+        ms.spheres(n, n / 6., 0.2, seed=2)
+    mats = [material.create(1.0, 0.3, dim), material.create(7.0, 0.25, dim)]
+    mat0 = material.create(4.0, 0.3, dim)
+    solver, _ = build(phase, mats, mat0, kind)
+    assert solver.fused
+    sym = solver.sym
+    rng = np.random.default_rng(5)
+    E = rng.standard_normal(sym)
+    rhs = torch.from_numpy(rng.standard_normal(solver.shape)).cuda() \
+        if with_rhs else None
+    results = []
+    for fused in (True, False):
+        solver.fused = fused
+        r = solver.solve(E, tol=1e-12, max_iter=60, rhs=rhs)
+        results.append((r.eps.cpu().numpy(), r.residuals, r.iterations))
+    assert solver.fused is False
+    np.testing.assert_array_equal(results[0][0], results[1][0])
+    np.testing.assert_array_equal(results[0][1], results[1][1])
+    assert results[0][2] == results[1][2]
+
+
+def test_stream_copy_helper():
+    import torch
+    from janus_amd import _lib
+    a = torch.arange(1 << 20, dtype=torch.float64, device='cuda')
+    b = torch.zeros_like(a)
+    _lib.check(_lib.lib().jx_stream_copy(a.data_ptr(), b.data_ptr(),
+                                         a.numel() * 8,
+                                         _lib.current_stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
