@@ -148,6 +148,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true',
                     help='skip the CPU baseline leg')
     ap.add_argument('--cpu-iters', type=int, default=1)
+    ap.add_argument('--schedule', default='auto',
+                    choices=['auto', 'fused', 'five'],
+                    help='iteration schedule (auto: the library default)')
     ap.add_argument('--copy-mb', type=int, default=0,
                     help='after the bench, copy a buffer of this size '
                          '(reference kernel for HBM traffic counters)')
@@ -163,6 +166,8 @@ def main():
     torch.cuda.set_device(0)
     n = args.grid
     solver, phase, table, mat0 = build_workload(n, args.kind)
+    if args.schedule != 'auto':
+        solver.fused = args.schedule == 'fused'
     E = np.zeros(6)
     E[5] = 1.0
     eps = torch.empty((n, n, n, 6), dtype=torch.float64, device='cuda')

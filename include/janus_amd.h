@@ -135,7 +135,8 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
  * pass of an iteration also applies the local operator to the new iterate
  * and transforms it for the next iteration, so the iterate is written once
  * and not read back; fused = 0: five separate passes per iteration.  Both
- * give the same iterates.  get returns the schedule actually used (the fused
+ * give the same iterates up to rounding.  get returns the schedule actually
+ * used (the fused
  * pass needs a last-axis length 16 ... 512, a power of two). */
 int jx_solver_set_fused(jx_solver *solver, int fused);
 int jx_solver_get_fused(jx_solver *solver);

@@ -70,6 +70,22 @@ __device__ __forceinline__ void local_voxel(const LocalDev &loc, int64_t vox,
   iso4<C>(tr, dev, e, t);
 }
 
+// ------------------------------------------------------------ update
+// Fixed-point update of one component: (E + rhs) - scale * y, evaluated the
+// same way (no contraction into FMA) in every kernel, so that all schedules
+// produce the same iterates bit for bit.
+__device__ __forceinline__ double ls_new(double e, double y, double scale) {
+#pragma clang fp contract(off)
+  double val = scale * y;
+  return e - val;
+}
+__device__ __forceinline__ double ls_new_rhs(double e, double h, double y,
+                                             double scale) {
+#pragma clang fp contract(off)
+  double val = scale * y;
+  return (e + h) - val;
+}
+
 // ------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -449,10 +465,11 @@ __global__ void __launch_bounds__(kThreads)
     for (int c = 0; c < C; ++c) {
       int q = r * C + c;
       double2 z = res[(q >> 1) * a.Ls + j];
-      double val = a.scale * ((q & 1) ? z.y : z.x);
+      double raw = (q & 1) ? z.y : z.x;
+      double val = a.scale * raw;
       if (LS) {
-        double nv = ls.E[c] - val;
-        if (ls.rhs) nv = (ls.E[c] + ls.rhs[vox * C + c]) - val;
+        double nv = ls.rhs ? ls_new_rhs(ls.E[c], ls.rhs[vox * C + c], raw, a.scale)
+                           : ls_new(ls.E[c], raw, a.scale);
         double d = nv - y[vox * C + c];
         acc += d * d;
         y[vox * C + c] = nv;
@@ -468,6 +485,12 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ------------------------------------------------------------ passes A, E
+// Twiddle source of the last-axis kernels: quarter table in LDS (1) or the
+// global table (0, fewer registers; measured as fast or faster).
+#ifndef JX_ROWS_TQ
+#define JX_ROWS_TQ 0
+#endif
+
 // Register-resident last-axis transforms for power-of-two L.  A block
 // handles RB rows, i.e. 2 NP real sequences paired into NP complex ones
 // (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / 16 threads
@@ -487,8 +510,8 @@ __global__ void __launch_bounds__(NP *(L / 16))
   const int tid = threadIdx.x;
   const int64_t row0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
-  __shared__ double2 twl[L / 4];
-  const double2 *Wt = stage_twiddles<L, NT, true>(W, twl);
+  __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
+  const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
   // stage the rows (local operator applied) into the pair sequences
   for (int w = tid; w < RB * L; w += NT) {
     int r = w / L, j = w - r * L;
@@ -527,7 +550,7 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #pragma unroll
   for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
-  reg_fft<L, -1, true>(v, t, seq, Wt);
+  reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
@@ -594,8 +617,8 @@ __global__ void __launch_bounds__(NP *(L / 16))
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
   const int p = tid % NP, t = tid / NP;
-  __shared__ double2 twl[L / 4];
-  const double2 *Wt = stage_twiddles<L, NT, true>(W, twl);
+  __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
+  const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
   // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
   // (0 < k < L/2); DC and Nyquist slots <- (Re Xa, Re Xb), their imaginary
   // parts dropped as FFTW's and pocketfft's c2r do
@@ -659,7 +682,7 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #pragma unroll
   for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
-  reg_fft<L, 1, true>(v, t, seq, Wt);
+  reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
   double acc = 0.;
   if constexpr (C == 6) {
     // pair p = components (c0, c0 + 1) of row r: 16-byte accesses
@@ -675,10 +698,12 @@ __global__ void __launch_bounds__(NP *(L / 16))
         int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
         double2 val = make_double2(scale * v[i].x, scale * v[i].y);
         if constexpr (LS_UPDATE) {
-          double2 nv = make_double2(e0 - val.x, e1 - val.y);
+          double2 nv = make_double2(ls_new(e0, v[i].x, scale),
+                                    ls_new(e1, v[i].y, scale));
           if constexpr (MODE == 2) {
             double2 h = hr[off];
-            nv = make_double2((e0 + h.x) - val.x, (e1 + h.y) - val.y);
+            nv = make_double2(ls_new_rhs(e0, h.x, v[i].x, scale),
+                              ls_new_rhs(e1, h.y, v[i].y, scale));
           }
           double2 o = yr[off];
           double dx = nv.x - o.x, dy = nv.y - o.y;
@@ -701,10 +726,11 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
-          double val = scale * (part ? v[i].y : v[i].x);
+          double raw = part ? v[i].y : v[i].x;
+          double val = scale * raw;
           if constexpr (LS_UPDATE) {
-            double nv = ls.E[c] - val;
-            if constexpr (MODE == 2) nv = (ls.E[c] + hr[off]) - val;
+            double nv = ls_new(ls.E[c], raw, scale);
+            if constexpr (MODE == 2) nv = ls_new_rhs(ls.E[c], hr[off], raw, scale);
             double d = nv - yr[off];
             acc += d * d;
             yr[off] = nv;
@@ -726,8 +752,14 @@ __global__ void __launch_bounds__(NP *(L / 16))
 // residual partial sums, then (C - C0) : eps and the forward last-axis
 // transform of the result, written over the same spectrum rows.  The new
 // iterate never travels back from memory.  MODE 1: E only; 2: E + rhs.
+// Minimum waves per SIMD requested for the fused update pass: 3 (about 70
+// registers spilled to scratch) measured faster than 2 at 256^3.
+#ifndef JX_FUSED_MIN_WAVES
+#define JX_FUSED_MIN_WAVES 3
+#endif
+
 template <int L, int C, int NP, int MODE>
-__global__ void __launch_bounds__(NP *(L / 16))
+__global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
                               double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st) {
@@ -737,13 +769,13 @@ __global__ void __launch_bounds__(NP *(L / 16))
   constexpr int RB = 2 * NP / C;
   extern __shared__ double2 smem[];
   __shared__ double red[4];
-  __shared__ double2 twl[L / 4];
+  __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
   const int tid = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
   const int p = tid % NP, t = tid / NP;
-  const double2 *Wt = stage_twiddles<L, NT, true>(W, twl);
+  const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
 
   // ---- inverse: half spectra into the pair slots (see k_rows_inverse_reg)
   if constexpr (C == 6) {
@@ -807,7 +839,7 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #pragma unroll
   for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
-  reg_fft<L, 1, true>(v, t, seq, Wt);
+  reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
 
   // ---- update; v becomes the new iterate (part 0 in .x, part 1 in .y)
   double acc = 0.;
@@ -822,11 +854,12 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
-        double2 nv = make_double2(e0 - scale * v[i].x, e1 - scale * v[i].y);
+        double2 nv = make_double2(ls_new(e0, v[i].x, scale),
+                                  ls_new(e1, v[i].y, scale));
         if constexpr (MODE == 2) {
           double2 h = hr[off];
-          nv = make_double2((e0 + h.x) - scale * v[i].x,
-                            (e1 + h.y) - scale * v[i].y);
+          nv = make_double2(ls_new_rhs(e0, h.x, v[i].x, scale),
+                            ls_new_rhs(e1, h.y, v[i].y, scale));
         }
         double2 o = yr[off];
         double dx = nv.x - o.x, dy = nv.y - o.y;
@@ -851,9 +884,9 @@ __global__ void __launch_bounds__(NP *(L / 16))
         double nv = 0.;
         if (r < rb) {
           int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
-          double val = scale * (part ? v[i].y : v[i].x);
-          nv = ls.E[c] - val;
-          if constexpr (MODE == 2) nv = (ls.E[c] + hr[off]) - val;
+          double raw = part ? v[i].y : v[i].x;
+          nv = ls_new(ls.E[c], raw, scale);
+          if constexpr (MODE == 2) nv = ls_new_rhs(ls.E[c], hr[off], raw, scale);
           double d = nv - yr[off];
           acc += d * d;
           yr[off] = nv;
@@ -892,7 +925,7 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #pragma unroll
   for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
-  reg_fft<L, -1, true>(v, t, seq, Wt);
+  reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];

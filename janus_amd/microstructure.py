@@ -89,6 +89,8 @@ def smoothed_density(n, sigma, seed, dim=3):
     k[-1] = np.fft.rfftfreq(n)
     kk = sum(np.meshgrid(*[ki ** 2 for ki in k], indexing='ij'))
     filt = np.exp(-2. * (math.pi * sigma) ** 2 * kk)
-    rho = np.fft.irfftn(np.fft.rfftn(noise) * filt, s=(n,) * dim)
+    axes = tuple(range(dim))
+    rho = np.fft.irfftn(np.fft.rfftn(noise, axes=axes) * filt, s=(n,) * dim,
+                        axes=axes)
     lo, hi = rho.min(), rho.max()
     return (rho - lo) / (hi - lo)

@@ -128,7 +128,9 @@ def test_field_rhs_matches_oracle():
     (3, 16, 'filtered', True), (2, 32, 'truncated', True)])
 def test_fused_schedule_matches_five_passes(dim, n, kind, with_rhs):
     """The fused schedule (pass E of one iteration with pass A of the next)
-    produces the same iterates as the five separate passes, bit for bit."""
+    produces the iterates of the five separate passes.  The two schedules
+    run the last-axis transforms in different kernels, whose multiply-adds
+    the compiler may fuse differently: the fields agree to rounding."""
     import torch
     phase = ms.square_inclusion(n, 0.5) if dim == 2 else \
         ms.spheres(n, n / 6., 0.2, seed=2)
@@ -147,8 +149,11 @@ def test_fused_schedule_matches_five_passes(dim, n, kind, with_rhs):
         r = solver.solve(E, tol=1e-12, max_iter=60, rhs=rhs)
         results.append((r.eps.cpu().numpy(), r.residuals, r.iterations))
     assert solver.fused is False
-    np.testing.assert_array_equal(results[0][0], results[1][0])
-    np.testing.assert_array_equal(results[0][1], results[1][1])
+    scale = np.abs(results[1][0]).max()
+    np.testing.assert_allclose(results[0][0], results[1][0], rtol=0,
+                               atol=1e-13 * scale)
+    np.testing.assert_allclose(results[0][1], results[1][1], rtol=0,
+                               atol=1e-14)
     assert results[0][2] == results[1][2]
 
 
