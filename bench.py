@@ -136,6 +136,83 @@ def stream_copy_gbps(mib=2048, reps=10):
     return 2 * nbytes / sec / 1e9
 
 
+TRAFFIC_TAG = 'traffic_per_launch'
+
+
+def traffic_file(n, kind):
+    return os.path.join(ROOT, 'profiles',
+                        'ls_iteration_%d_%s_traffic.md' % (n, kind))
+
+
+def read_traffic(n, kind, kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (lines '<tag> <kernel> <bytes>'), or None."""
+    path = traffic_file(n, kind)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.strip().strip('`').split()
+        if len(f) == 3 and f[0] == TRAFFIC_TAG and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
+# kernel symbol prefix -> bench pass name (256^3, sym = 6)
+KERNEL_PASS = [('k_rows_update_forward_reg', 'EA_rows_c2r_update_local_r2c'),
+               ('k_rows_forward_reg', 'A_rows_r2c_local'),
+               ('k_rows_inverse_reg', 'E_rows_c2r_update'),
+               ('k_axis_reg<256, 6, 12, -1>', 'B_axis1_fwd'),
+               ('k_axis_reg<256, 6, 12, 1>', 'D_axis1_inv'),
+               ('k_axis_reg<256, 6, 12, 0>', 'C_axis0_gamma'),
+               ('k_stream_copy', 'stream_copy')]
+
+
+def traffic_summary(fetch_csv, write_csv, cmd):
+    """Markdown summary of rocprofv3 FETCH_SIZE / WRITE_SIZE runs: each
+    counter is scaled by what it reads for the stream-copy kernel, whose
+    bytes are known (16 B per grid thread)."""
+    import csv
+    import collections
+    vals = {}
+    copies = {}
+    for counter, path in (('FETCH_SIZE', fetch_csv), ('WRITE_SIZE', write_csv)):
+        agg = collections.defaultdict(list)
+        ratios = []
+        for r in csv.DictReader(open(path)):
+            name = r['Kernel_Name']
+            v = float(r['Counter_Value']) * 1024.0
+            agg[name].append(v)
+            if 'k_stream_copy' in name:
+                ratios.append(int(r['Grid_Size']) * 16 / v)
+        vals[counter] = {k: sum(v) / len(v) for k, v in agg.items()}
+        copies[counter] = sum(ratios) / len(ratios)
+    out = ['# HBM traffic per launch, 256^3 filtered LS iteration', '',
+           'Commands (one counter per run):', '',
+           '    ' + cmd.replace('COUNTER', 'FETCH_SIZE'),
+           '    ' + cmd.replace('COUNTER', 'WRITE_SIZE'), '',
+           'Calibration on k_stream_copy (bytes known): FETCH_SIZE x %.4f, '
+           'WRITE_SIZE x %.4f.' % (copies['FETCH_SIZE'], copies['WRITE_SIZE']),
+           '', '| kernel | FETCH raw MB | WRITE raw MB | calibrated read MB | '
+           'calibrated write MB | traffic MB |', '|---|---|---|---|---|---|']
+    tags = []
+    for name in vals['FETCH_SIZE']:
+        if 'jx::' not in name:
+            continue
+        f = vals['FETCH_SIZE'][name]
+        w = vals['WRITE_SIZE'].get(name, 0.)
+        fr, wr = f * copies['FETCH_SIZE'], w * copies['WRITE_SIZE']
+        short = name.split('(')[0].replace('void ', '')
+        out.append('| %s | %.1f | %.1f | %.1f | %.1f | %.1f |'
+                   % (short, f / 1e6, w / 1e6, fr / 1e6, wr / 1e6,
+                      (fr + wr) / 1e6))
+        for pre, pname in KERNEL_PASS:
+            if pre in name:
+                tags.append('    %s %s %d' % (TRAFFIC_TAG, pname, int(fr + wr)))
+                break
+    out += ['', 'Traffic per launch (bytes) read by bench.py:', ''] + tags
+    return '\n'.join(out) + '\n'
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -147,14 +224,21 @@ def main():
     ap.add_argument('--profile-iters', type=int, default=20)
     ap.add_argument('--no-cpu', action='store_true',
                     help='skip the CPU baseline leg')
-    ap.add_argument('--cpu-iters', type=int, default=1)
+    ap.add_argument('--cpu-iters', type=int, default=2)
     ap.add_argument('--schedule', default='auto',
                     choices=['auto', 'fused', 'five'],
                     help='iteration schedule (auto: the library default)')
     ap.add_argument('--copy-mb', type=int, default=0,
                     help='after the bench, copy a buffer of this size '
                          '(reference kernel for HBM traffic counters)')
+    ap.add_argument('--traffic-summary', nargs=3,
+                    metavar=('FETCH_CSV', 'WRITE_CSV', 'COMMAND'),
+                    help='print the counter summary of two rocprofv3 --pmc '
+                         'runs (no GPU work)')
     args = ap.parse_args()
+    if args.traffic_summary:
+        print(traffic_summary(*args.traffic_summary), end='')
+        return
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only; the multi-GPU '
               'slab decomposition is design-only (--gpus %d refused)'
@@ -203,7 +287,7 @@ def main():
                               'bytes': int(nbytes[i]),
                               'GBps': round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)
                               if ms[i] > 0 and nbytes[i] else None}
-              for i in range(6) if ms[i] > 0 or not fused or i != 0}
+              for i in range(6) if not (fused and i == 0)}
     print('bench: passes ' + json.dumps(passes), file=sys.stderr, flush=True)
 
     copy_gbps = stream_copy_gbps()
@@ -260,21 +344,23 @@ def main():
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4),
-            'traffic': None,
+            'traffic': read_traffic(n, args.kind, names[dom]),
         },
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mb > 0:
-        # plain device-to-device copy of known size, outside the timed region
-        nel = args.copy_mb * (1 << 20) // 8
-        a = torch.ones(nel, dtype=torch.float64, device='cuda')
+        # the library's stream copy of known size (k_stream_copy), outside
+        # the timed region: the reference for calibrating traffic counters
+        nbytes = args.copy_mb * (1 << 20)
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device='cuda')
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            _lib.check(_lib.lib().jx_stream_copy(
+                a.data_ptr(), b.data_ptr(), nbytes, _lib.current_stream()))
         torch.cuda.synchronize()
-        print('bench: copied %d MiB x 3' % args.copy_mb, file=sys.stderr,
-              flush=True)
+        print('bench: stream-copied %d MiB x 3' % args.copy_mb,
+              file=sys.stderr, flush=True)
 
 
 if __name__ == '__main__':
