@@ -1,18 +1,19 @@
-// Register-resident FFT of power-of-two length L = 16 ... 1024 for the hot
-// passes.
+// Register-resident FFT for the hot lengths.
 //
-// Each transform of length L is done by TPP = L / 16 threads, each holding
-// 16 complex values.  Stages have radix 16 except possibly the last one
-// (L = 2^a 16^b): 32 = 16 x 2, 64 = 16 x 4, 128 = 16 x 8, 256 = 16 x 16,
-// 512 = 16 x 16 x 2, 1024 = 16 x 16 x 4.  In a stage of radix R a thread
-// runs 16 / R butterflies j = t + TPP q (q < 16 / R) of the Stockham
+// A transform of length L is done by TPP = L / V threads, each holding V
+// complex values; every stage radix divides V, so in a stage of radix R a
+// thread runs V / R butterflies j = t + TPP q (q < V / R) of the Stockham
 // recurrence (inputs src[j + r L/R], twiddles w^(r k), k = j mod Ns,
 // outputs dst[(j / Ns) Ns R + k + r Ns]); consecutive stages exchange data
-// through LDS.  The first stage reads x[t + TPP r] (r < 16) and the last
-// one leaves X[t + TPP q + r L/R_last] in registers, so loads and stores
-// are plain strided patterns.  Stage twiddles come from the table
-// W[m] = exp(-2 pi i m / L) (one correctly rounded entry per factor), read
-// from global memory or from a quarter table staged in LDS.
+// through LDS.  Plans (V; radices):
+//   16 (16; 16)   32 (16; 16 2)   64 (16; 16 4)   128 (16; 16 8)
+//   256 (16; 16 16)   512 (16; 16 16 2)
+//   80 (20; 5 4 4)   96 (12; 6 4 4)   160 (20; 10 4 4)
+// The first stage reads x[t + TPP (i / R0) + (i mod R0) L / R0] and the last
+// one leaves X[t + TPP q + r L / R_last] (i = q R_last + r) in registers, so
+// loads and stores are plain strided patterns.  Stage twiddles come from the
+// table W[m] = exp(-2 pi i m / L) (one correctly rounded entry per factor),
+// read from global memory or from a quarter table staged in LDS.
 #pragma once
 
 #include "jx_fft_lds.h"
@@ -21,18 +22,33 @@ namespace jx {
 
 template <int L>
 struct RegPlan;
-template <> struct RegPlan<16> { static constexpr int n = 1, r0 = 16, r1 = 1, r2 = 1; };
-template <> struct RegPlan<32> { static constexpr int n = 2, r0 = 16, r1 = 2, r2 = 1; };
-template <> struct RegPlan<64> { static constexpr int n = 2, r0 = 16, r1 = 4, r2 = 1; };
-template <> struct RegPlan<128> { static constexpr int n = 2, r0 = 16, r1 = 8, r2 = 1; };
-template <> struct RegPlan<256> { static constexpr int n = 2, r0 = 16, r1 = 16, r2 = 1; };
-template <> struct RegPlan<512> { static constexpr int n = 3, r0 = 16, r1 = 16, r2 = 2; };
-template <> struct RegPlan<1024> { static constexpr int n = 3, r0 = 16, r1 = 16, r2 = 4; };
+#define JX_REG_PLAN(LL, VV, N, R0, R1, R2)                                  \
+  template <>                                                              \
+  struct RegPlan<LL> {                                                     \
+    static constexpr int V = VV, n = N, r0 = R0, r1 = R1, r2 = R2;         \
+  };
+JX_REG_PLAN(16, 16, 1, 16, 1, 1)
+JX_REG_PLAN(32, 16, 2, 16, 2, 1)
+JX_REG_PLAN(64, 16, 2, 16, 4, 1)
+JX_REG_PLAN(128, 16, 2, 16, 8, 1)
+JX_REG_PLAN(256, 16, 2, 16, 16, 1)
+JX_REG_PLAN(512, 16, 3, 16, 16, 2)
+JX_REG_PLAN(80, 20, 3, 5, 4, 4)
+JX_REG_PLAN(96, 12, 3, 6, 4, 4)
+JX_REG_PLAN(160, 20, 3, 10, 4, 4)
+#undef JX_REG_PLAN
 
 // Lengths the register kernels are instantiated for.
 constexpr bool reg_fft_length(int L) {
-  return L == 16 || L == 32 || L == 64 || L == 128 || L == 256 || L == 512;
+  return L == 16 || L == 32 || L == 64 || L == 128 || L == 256 || L == 512 ||
+         L == 80 || L == 96 || L == 160;
 }
+
+// Values held per thread, threads per transform.
+template <int L>
+constexpr int reg_v() { return RegPlan<L>::V; }
+template <int L>
+constexpr int reg_tpp() { return L / RegPlan<L>::V; }
 
 // (a + i b) * (c + i d) with compile-time constants
 __device__ __forceinline__ double2 cmulc(double2 a, double c, double d) {
@@ -75,11 +91,96 @@ __device__ __forceinline__ void dft16(double2 *v) {
   }
 }
 
+// 3-point DFT: y1,2 = v0 - (v1 + v2)/2 +- DIR i (sqrt3/2)(v1 - v2)
+template <int DIR>
+__device__ __forceinline__ void dft3(double2 *v) {
+  constexpr double s3 = 0.86602540378443864676;  // sqrt(3)/2
+  double2 t1 = cadd(v[1], v[2]);
+  double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+  double2 t3 = make_double2(s3 * (v[1].x - v[2].x), s3 * (v[1].y - v[2].y));
+  t3 = mul_i<DIR>(t3);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+
+// 5-point DFT with c_k = cos(2 pi k / 5), s_k = sin(2 pi k / 5):
+// a1 = v1 + v4, a2 = v2 + v3, b1 = v1 - v4, b2 = v2 - v3,
+// y1,4 = v0 + c1 a1 + c2 a2 +- DIR i (s1 b1 + s2 b2),
+// y2,3 = v0 + c2 a1 + c1 a2 +- DIR i (s2 b1 - s1 b2).
+template <int DIR>
+__device__ __forceinline__ void dft5(double2 *v) {
+  constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  constexpr double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+  double2 a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
+  double2 b1 = csub(v[1], v[4]), b2 = csub(v[2], v[3]);
+  double2 t1 = make_double2(v[0].x + c1 * a1.x + c2 * a2.x,
+                            v[0].y + c1 * a1.y + c2 * a2.y);
+  double2 t2 = make_double2(v[0].x + c2 * a1.x + c1 * a2.x,
+                            v[0].y + c2 * a1.y + c1 * a2.y);
+  double2 u1 = mul_i<DIR>(make_double2(s1 * b1.x + s2 * b2.x,
+                                       s1 * b1.y + s2 * b2.y));
+  double2 u2 = mul_i<DIR>(make_double2(s2 * b1.x - s1 * b2.x,
+                                       s2 * b1.y - s1 * b2.y));
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(t1, u1);
+  v[4] = csub(t1, u1);
+  v[2] = cadd(t2, u2);
+  v[3] = csub(t2, u2);
+}
+
+// 2 R2-point DFT (R2 = 3, 5): n = R2 n1 + n2, k = k1 + 2 k2; pairs over n1,
+// twiddles W_{2 R2}^(n2 k1), then R2-point DFTs over n2.  cw/sw hold
+// cos/sin(2 pi m / (2 R2)) for m = 0 .. R2 - 1.
+template <int R2, int DIR>
+__device__ __forceinline__ void dft2xr(double2 *v, const double *cw,
+                                       const double *sw) {
+  double2 a[R2], b[R2];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    a[n2] = cadd(v[n2], v[R2 + n2]);
+    double2 d = csub(v[n2], v[R2 + n2]);
+    b[n2] = n2 ? cmulc(d, cw[n2], DIR * sw[n2]) : d;
+  }
+  if constexpr (R2 == 3) {
+    dft3<DIR>(a);
+    dft3<DIR>(b);
+  } else {
+    dft5<DIR>(a);
+    dft5<DIR>(b);
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) {
+    v[2 * k2] = a[k2];
+    v[2 * k2 + 1] = b[k2];
+  }
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft6(double2 *v) {
+  constexpr double cw[3] = {1.0, 0.5, -0.5};
+  constexpr double sw[3] = {0.0, 0.86602540378443864676, 0.86602540378443864676};
+  dft2xr<3, DIR>(v, cw, sw);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft10(double2 *v) {
+  constexpr double cw[5] = {1.0, 0.80901699437494742410, 0.30901699437494742410,
+                            -0.30901699437494742410, -0.80901699437494742410};
+  constexpr double sw[5] = {0.0, 0.58778525229247312917, 0.95105651629515357212,
+                            0.95105651629515357212, 0.58778525229247312917};
+  dft2xr<5, DIR>(v, cw, sw);
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft_r(double2 *v) {
   if constexpr (R == 16) dft16<DIR>(v);
+  else if constexpr (R == 10) dft10<DIR>(v);
   else if constexpr (R == 8) dft8<DIR>(v);
+  else if constexpr (R == 6) dft6<DIR>(v);
+  else if constexpr (R == 5) dft5<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);
+  else if constexpr (R == 3) dft3<DIR>(v);
   else if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
 }
 
@@ -113,13 +214,14 @@ constexpr int reg_twiddle_entries() {
   return TQ ? L / 4 : 0;
 }
 
-// One stage on the registers: twiddle, then 16 / R butterflies of radix R.
+// One stage on the registers: twiddle, then V / R butterflies of radix R.
 template <int L, int R, int NSP, int DIR, bool TQ>
 __device__ __forceinline__ void reg_stage(double2 *v, int t, const double2 *W) {
-  constexpr int TPP = L / 16;
+  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<L>();
   constexpr int tws = L / (NSP * R);
 #pragma unroll
-  for (int q = 0; q < 16 / R; ++q) {
+  for (int q = 0; q < V / R; ++q) {
     if constexpr (NSP > 1) {
       int j = t + TPP * q;
       int k = j % NSP;
@@ -142,9 +244,10 @@ constexpr int reg_lds_stride(int L) { return L + L / 16 + 1; }
 template <int L, int R, int NSP>
 __device__ __forceinline__ void reg_store_stage(const double2 *v, int t,
                                                 double2 *seq) {
-  constexpr int TPP = L / 16;
+  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<L>();
 #pragma unroll
-  for (int q = 0; q < 16 / R; ++q) {
+  for (int q = 0; q < V / R; ++q) {
     int j = t + TPP * q;
     int o = (j / NSP) * NSP * R + (j % NSP);
 #pragma unroll
@@ -156,9 +259,10 @@ __device__ __forceinline__ void reg_store_stage(const double2 *v, int t,
 template <int L, int R>
 __device__ __forceinline__ void reg_load_stage(double2 *v, int t,
                                                const double2 *seq) {
-  constexpr int TPP = L / 16;
+  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<L>();
 #pragma unroll
-  for (int q = 0; q < 16 / R; ++q) {
+  for (int q = 0; q < V / R; ++q) {
     int j = t + TPP * q;
 #pragma unroll
     for (int r = 0; r < R; ++r) v[q * R + r] = seq[lp(j + r * (L / R))];
@@ -169,7 +273,7 @@ __device__ __forceinline__ void reg_load_stage(double2 *v, int t,
 template <int L>
 __device__ __forceinline__ int reg_out_pos(int t, int i) {
   using P = RegPlan<L>;
-  constexpr int TPP = L / 16;
+  constexpr int TPP = reg_tpp<L>();
   constexpr int RL = P::n == 1 ? P::r0 : (P::n == 2 ? P::r1 : P::r2);
   int q = i / RL, r = i - q * RL;
   return t + TPP * q + r * (L / RL);
@@ -178,13 +282,16 @@ __device__ __forceinline__ int reg_out_pos(int t, int i) {
 // Position of register slot i before the first stage.
 template <int L>
 __device__ __forceinline__ int reg_in_pos(int t, int i) {
-  return t + (L / 16) * i;
+  constexpr int TPP = reg_tpp<L>();
+  constexpr int R0 = RegPlan<L>::r0;
+  int q = i / R0, r = i - q * R0;
+  return t + TPP * q + r * (L / R0);
 }
 
-// Full transform.  On entry v[i] = x[reg_in_pos(t, i)]; on exit
-// v[i] = X[reg_out_pos(t, i)].  `seq` is this sequence's LDS scratch of
-// L entries; the caller's block must reach the barriers inside (every thread
-// of the block calls this function the same number of times).
+// Full transform.  On entry v[i] = x[reg_in_pos(t, i)] (i < V); on exit
+// v[i] = X[reg_out_pos(t, i)].  `seq` is this sequence's LDS scratch; the
+// caller's block must reach the barriers inside (every thread of the block
+// calls this function the same number of times).
 template <int L, int DIR, bool TQ>
 __device__ __forceinline__ void reg_fft(double2 *v, int t, double2 *seq,
                                         const double2 *W) {

@@ -324,23 +324,27 @@ __device__ __forceinline__ const double2 *stage_twiddles(const double2 *W,
 
 // Register-resident transform along a strided axis of the spectrum for
 // power-of-two L.  NSEQ = T * C complex sequences per block (T points of the
-// contiguous inner index, all C components), TPP = L / 16 threads each.
+// contiguous inner index, all C components), TPP = L / V threads each.
 // Thread tid = t * NSEQ + s: consecutive lanes read consecutive doubles of
 // the T * 2C doubles that one axis element of the tile spans.
-// Minimum waves per SIMD requested for the Gamma-hat pass (register budget
-// of the compiler).  3 (a few spilled registers) measured faster than the
-// compiler's own choice of 2 waves at 256^3.
+// Minimum waves per SIMD requested for the Gamma-hat pass of the 16-value
+// plans (register budget of the compiler).  3 (a few spilled registers)
+// measured faster than the compiler's own choice of 2 waves at 256^3; the
+// 20- and 24-value plans would spill heavily and keep the compiler's choice.
 #ifndef JX_GAMMA_PASS_MIN_WAVES
 #define JX_GAMMA_PASS_MIN_WAVES 3
 #endif
 
 template <int L, int C, int NSEQ, int MODE>
-__global__ void __launch_bounds__(NSEQ *(L / 16),
-                                  MODE == 0 ? JX_GAMMA_PASS_MIN_WAVES : 1)
+__global__ void __launch_bounds__(NSEQ * reg_tpp<L>(),
+                                  MODE == 0 && reg_v<L>() == 16
+                                      ? JX_GAMMA_PASS_MIN_WAVES
+                                      : 1)
     k_axis_reg(const double *in, double *out, AxisArgs a, const double2 *W,
                GreenDev G, const SolveState *st) {
   if (solve_done(st)) return;
-  constexpr int TPP = L / 16;
+  constexpr int TPP = reg_tpp<L>();
+  constexpr int V = reg_v<L>();
   constexpr int T = NSEQ / C;
   constexpr int LS = reg_lds_stride(L);
   extern __shared__ double2 smem[];
@@ -354,9 +358,9 @@ __global__ void __launch_bounds__(NSEQ *(L / 16),
   const bool valid = tp < tc;
   const double *src = in + o * a.stride_outer + (m0 + tp) * 2 * C + 2 * c;
   double *dst = out + o * a.stride_outer + (m0 + tp) * 2 * C + 2 * c;
-  double2 v[16];
+  double2 v[V];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
+  for (int i = 0; i < V; ++i) {
     if (valid) {
       v[i] = *(const double2 *)(src + (int64_t)reg_in_pos<L>(t, i) * a.stride_ax);
     } else {
@@ -373,7 +377,7 @@ __global__ void __launch_bounds__(NSEQ *(L / 16),
   if constexpr (MODE == 0 && (C == 3 || C == 6)) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+    for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
     __syncthreads();
     for (int w = tid; w < tc * L; w += NSEQ * TPP) {
       int p = w / L, e = w - p * L;
@@ -381,13 +385,13 @@ __global__ void __launch_bounds__(NSEQ *(L / 16),
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+    for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
     __syncthreads();
     reg_fft<L, 1, TQ>(v, t, seq, Wt);
   }
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < V; ++i) {
       *(double2 *)(dst + (int64_t)reg_out_pos<L>(t, i) * a.stride_ax) = v[i];
     }
   }
@@ -493,16 +497,17 @@ __global__ void __launch_bounds__(kThreads)
 
 // Register-resident last-axis transforms for power-of-two L.  A block
 // handles RB rows, i.e. 2 NP real sequences paired into NP complex ones
-// (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / 16 threads
+// (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / V threads
 // per pair.
 
 template <int L, int C, int NP, bool LOCAL>
-__global__ void __launch_bounds__(NP *(L / 16))
+__global__ void __launch_bounds__(NP * reg_tpp<L>())
     k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
                        int64_t nrows, const double2 *W, LocalDev loc,
                        const SolveState *st) {
   if (solve_done(st)) return;
-  constexpr int TPP = L / 16, LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<L>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   static_assert((2 * NP) % C == 0, "whole rows per block");
@@ -546,14 +551,14 @@ __global__ void __launch_bounds__(NP *(L / 16))
   __syncthreads();
   const int p = tid % NP, t = tid / NP;
   double2 *seq = smem + p * LS;
-  double2 v[16];
+  double2 v[V];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
   reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
   __syncthreads();
   // separate the pairs: Xa = (Z[k] + conj Z[-k]) / 2, Xb = (Z[k] - conj Z[-k]) / 2i
   if constexpr (C == 6) {
@@ -601,12 +606,13 @@ __global__ void __launch_bounds__(NP *(L / 16))
 // MODE 0: y = c2r(spec); 1: y <- E - c2r(spec) with residual partials;
 // 2: y <- (E + rhs) - c2r(spec) with residual partials.
 template <int L, int C, int NP, int MODE>
-__global__ void __launch_bounds__(NP *(L / 16))
+__global__ void __launch_bounds__(NP * reg_tpp<L>())
     k_rows_inverse_reg(const double *__restrict__ spec, double *y,
                        int64_t nrows, double scale, const double2 *W,
                        LsArgs ls, const SolveState *st) {
   if (solve_done(st)) return;
-  constexpr int TPP = L / 16, LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<L>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   constexpr bool LS_UPDATE = MODE > 0;
@@ -678,9 +684,9 @@ __global__ void __launch_bounds__(NP *(L / 16))
     __syncthreads();
   }
   double2 *seq = smem + p * LS;
-  double2 v[16];
+  double2 v[V];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
   reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
   double acc = 0.;
@@ -694,7 +700,7 @@ __global__ void __launch_bounds__(NP *(L / 16))
                     : nullptr;
       const double e0 = ls.E[c0], e1 = ls.E[c0 + 1];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < V; ++i) {
         int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
         double2 val = make_double2(scale * v[i].x, scale * v[i].y);
         if constexpr (LS_UPDATE) {
@@ -724,7 +730,7 @@ __global__ void __launch_bounds__(NP *(L / 16))
         double *yr = y + (row0 + r) * L * C + c;
         const double *hr = MODE == 2 ? ls.rhs + (row0 + r) * L * C + c : nullptr;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
+        for (int i = 0; i < V; ++i) {
           int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
           double raw = part ? v[i].y : v[i].x;
           double val = scale * raw;
@@ -759,12 +765,14 @@ __global__ void __launch_bounds__(NP *(L / 16))
 #endif
 
 template <int L, int C, int NP, int MODE>
-__global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
+__global__ void __launch_bounds__(NP * reg_tpp<L>(),
+                                  reg_v<L>() == 16 ? JX_FUSED_MIN_WAVES : 1)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
                               double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st) {
   if (solve_done(st)) return;
-  constexpr int TPP = L / 16, LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<L>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   extern __shared__ double2 smem[];
@@ -835,9 +843,9 @@ __global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
     __syncthreads();
   }
   double2 *seq = smem + p * LS;
-  double2 v[16];
+  double2 v[V];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
   reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
 
@@ -852,7 +860,7 @@ __global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
                     : nullptr;
       const double e0 = ls.E[c0], e1 = ls.E[c0 + 1];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < V; ++i) {
         int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
         double2 nv = make_double2(ls_new(e0, v[i].x, scale),
                                   ls_new(e1, v[i].y, scale));
@@ -870,7 +878,7 @@ __global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) v[i] = make_double2(0., 0.);
+      for (int i = 0; i < V; ++i) v[i] = make_double2(0., 0.);
     }
   } else {
 #pragma unroll
@@ -880,7 +888,7 @@ __global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
       double *yr = y + (row0 + r) * L * C + c;
       const double *hr = MODE == 2 ? ls.rhs + (row0 + r) * L * C + c : nullptr;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < V; ++i) {
         double nv = 0.;
         if (r < rb) {
           int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
@@ -901,7 +909,7 @@ __global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
   // ---- the new iterate into LDS, local operator voxel by voxel
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
   __syncthreads();
   for (int w = tid; w < rb * L; w += NT) {
     int r = w / L, j = w - r * L;
@@ -923,12 +931,12 @@ __global__ void __launch_bounds__(NP *(L / 16), JX_FUSED_MIN_WAVES)
 
   // ---- forward transform of the polarization rows (see k_rows_forward_reg)
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
   __syncthreads();
   reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
   __syncthreads();
   if constexpr (C == 6) {
     for (int w = tid; w < rb * NH * 3; w += NT) {
@@ -1136,12 +1144,23 @@ static int set_lds_attr(K kernel, size_t bytes) {
 // Rows per block of the register kernels (0 = not available).
 template <int L>
 constexpr int reg_np() {
-  return L <= 256 ? 192 / (L / 16) : 6;
+  return 192 / reg_tpp<L>();
 }
 
 static int reg_rows_per_block(int L, int C) {
   if (!reg_fft_length(L) || (C != 1 && C != 3 && C != 6)) return 0;
-  int np = L <= 256 ? 192 / (L / 16) : 6;
+  int np = 0;
+  switch (L) {
+    case 16: np = reg_np<16>(); break;
+    case 32: np = reg_np<32>(); break;
+    case 64: np = reg_np<64>(); break;
+    case 128: np = reg_np<128>(); break;
+    case 256: np = reg_np<256>(); break;
+    case 512: np = reg_np<512>(); break;
+    case 80: np = reg_np<80>(); break;
+    case 96: np = reg_np<96>(); break;
+    case 160: np = reg_np<160>(); break;
+  }
   return 2 * np / C;
 }
 
@@ -1155,7 +1174,7 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
   size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
   int rc = check_lds(lds, "rows_forward");
   if (rc) return rc;
-  dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * (L / 16));
+  dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * reg_tpp<L>());
   if (loc.mode != LOCAL_NONE) {
     if constexpr (C == 3 || C == 6) {
       set_lds_attr(k_rows_forward_reg<L, C, NP, true>, lds);
@@ -1183,7 +1202,7 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
   size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
   int rc = check_lds(lds, "rows_inverse");
   if (rc) return rc;
-  dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * (L / 16));
+  dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * reg_tpp<L>());
   double scale = 1.0 / (double)g.N;
   if (upd) {
     if constexpr (C == 3 || C == 6) {
@@ -1216,6 +1235,9 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
     case 128: JX_DISPATCH_C(128, FN, __VA_ARGS__);                         \
     case 256: JX_DISPATCH_C(256, FN, __VA_ARGS__);                         \
     case 512: JX_DISPATCH_C(512, FN, __VA_ARGS__);                         \
+    case 80: JX_DISPATCH_C(80, FN, __VA_ARGS__);                           \
+    case 96: JX_DISPATCH_C(96, FN, __VA_ARGS__);                           \
+    case 160: JX_DISPATCH_C(160, FN, __VA_ARGS__);                         \
     default: return fail(JX_EUNSUPPORTED, "no register FFT for length");   \
   }
 #define JX_DISPATCH_C(LL, FN, ...)                                         \
@@ -1252,7 +1274,7 @@ static int launch_rows_update_forward_reg_lc(const Grid &g, double *spec,
     size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
     int rc = check_lds(lds, "rows_update_forward");
     if (rc) return rc;
-    dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * (L / 16));
+    dim3 grid((unsigned)((g.rows + RB - 1) / RB)), block(NP * reg_tpp<L>());
     double scale = 1.0 / (double)g.N;
     if (la.rhs) {
       set_lds_attr(k_rows_update_forward_reg<L, C, NP, 2>, lds);
@@ -1358,7 +1380,7 @@ static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
                                const double *in, double *out,
                                const AxisArgs &a, const double2 *W,
                                const GreenDev &G, const SolveState *st) {
-  dim3 block(NSEQ * (L / 16));
+  dim3 block(NSEQ * reg_tpp<L>());
   if (dir < 0) {
     set_lds_attr(k_axis_reg<L, C, NSEQ, -1>, lds);
     hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, -1>), grid, block, lds, s, in,
@@ -1377,7 +1399,7 @@ static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
 // sequences per block: 192 threads for every length up to 512
 template <int L>
 constexpr int reg_nseq() {
-  return L <= 256 ? 192 / (L / 16) : 6;
+  return 192 / reg_tpp<L>();
 }
 
 template <int L>
@@ -1427,6 +1449,9 @@ int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
     case 128: return launch_axis_reg_l<128>(g, ncomp, axis, dir, in, out, green, st, P, s);
     case 256: return launch_axis_reg_l<256>(g, ncomp, axis, dir, in, out, green, st, P, s);
     case 512: return launch_axis_reg_l<512>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 80: return launch_axis_reg_l<80>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 96: return launch_axis_reg_l<96>(g, ncomp, axis, dir, in, out, green, st, P, s);
+    case 160: return launch_axis_reg_l<160>(g, ncomp, axis, dir, in, out, green, st, P, s);
     default: return fail(JX_EUNSUPPORTED, "no register FFT for this length");
   }
 }

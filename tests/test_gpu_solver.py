@@ -37,6 +37,9 @@ CASES = [
     (2, 33, 'filtered', (1.0, 0.3), (20.0, 0.2)),
     (3, 16, 'filtered', (1.0, 0.3), (50.0, 0.25)),
     (3, 15, 'truncated', (1.0, 0.3), (5.0, 0.3)),
+    (2, 80, 'filtered', (1.0, 0.3), (5.0, 0.3)),
+    (2, 96, 'truncated', (1.0, 0.3), (5.0, 0.3)),
+    (2, 160, 'filtered', (1.0, 0.3), (3.0, 0.3)),
 ]
 
 
