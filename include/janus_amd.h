@@ -47,6 +47,7 @@ typedef struct jx_fft jx_fft;
 typedef struct jx_green jx_green;
 typedef struct jx_local jx_local;
 typedef struct jx_solver jx_solver;
+typedef struct jx_batch jx_batch;
 
 /* Message describing the last failure on this thread ("" if none). */
 const char *jx_last_error(void);
@@ -136,10 +137,60 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
  * and transforms it for the next iteration, so the iterate is written once
  * and not read back; fused = 0: five separate passes per iteration.  Both
  * give the same iterates up to rounding.  get returns the schedule actually
- * used (the fused
- * pass needs a last-axis length 16 ... 512, a power of two). */
+ * used (the fused pass needs a last-axis length with a register FFT:
+ * 16, 32, 64, 128, 256, 512, 80, 96 or 160). */
 int jx_solver_set_fused(jx_solver *solver, int fused);
 int jx_solver_get_fused(jx_solver *solver);
+
+/* ---------------------------------------------------------------------
+ * Batched solves (no janus counterpart: janus runs one fixed point at a
+ * time from user scripts, sphinx/tutorials/square_basic/square_basic.py:
+ * 83-97).  A batch holds many right-hand sides -- the load cases of one
+ * solver, solvers on different grids -- and runs them all to their stop in
+ * one sequence of launches.  Entries on the same grid, Green operator kind
+ * and h, schedule and rhs use form a group; `width` entries of a group
+ * share each kernel launch when every axis has a register FFT length (else
+ * one launch per entry).  Every replay runs iters_per_launch iterations of
+ * each chunk that still has a running entry; after the first (eager)
+ * replay the replays are captured in a hipGraph, captured again only when
+ * a chunk has finished.  Each entry stops exactly as jx_solver_solve would
+ * and gets the same iterates and iteration count.
+ * ------------------------------------------------------------------- */
+int jx_batch_create(jx_batch **out);
+int jx_batch_destroy(jx_batch *batch);
+/* One right-hand side of `solver`; the arguments are those of
+ * jx_solver_solve (eps: the entry's own device field, initial iterate in,
+ * last iterate out; it must stay valid while the batch runs).  Allocates the
+ * entry's spectrum, partial sums and residual history on the device.
+ * index: host output, the entry's index (0, 1, ... in order of addition). */
+int jx_batch_add(jx_batch *batch, jx_solver *solver, double *eps,
+                 const double *E, const double *rhs, double tol, int max_iter,
+                 double res_scale, int *index);
+/* Groups in order of first appearance: grid (dim, shape[3], 1 for unused
+ * axes), entry count, entries per launch (after jx_batch_run: the width
+ * used) and whether the group's launches are batched (1) or per entry. */
+int jx_batch_ngroups(jx_batch *batch, int *ngroups);
+int jx_batch_group(jx_batch *batch, int group, int *dim, int64_t *shape,
+                   int *size, int *width, int *batched);
+/* Entries per launch of a group (0 = the whole group). */
+int jx_batch_set_width(jx_batch *batch, int group, int width);
+/* iters_per_launch >= 1 (default 8); graph = 1 (default) replays a
+ * captured hipGraph; autotune = 1: after the first replay, one timed replay
+ * per candidate width (all, 6, 3, 1 entries per launch) of every group whose
+ * width was not set, then the fastest is kept (default 0: whole groups). */
+int jx_batch_set_schedule(jx_batch *batch, int iters_per_launch, int graph,
+                          int autotune);
+/* Runs every entry to its stop on an internal stream after synchronizing
+ * `stream`; returns when all have stopped.  A batch may be run again: each
+ * entry restarts from the content of its eps. */
+int jx_batch_run(jx_batch *batch, void *stream);
+/* Result of entry `index` of the last run: iteration count, status and the
+ * residual history (host array of iters doubles, may be NULL). */
+int jx_batch_result(jx_batch *batch, int index, double *res_hist, int *iters,
+                    int *status);
+/* Wall time of the last run in ms, number of replays and of graph
+ * captures. */
+int jx_batch_stats(jx_batch *batch, double *ms, int *replays, int *captures);
 
 /* Adjoint contraction (no janus counterpart; implicit differentiation of
  * the fixed point of jx_solver_solve).  With eps the converged strain and

@@ -66,6 +66,19 @@ SIGNATURES = [
     ('jx_solver_set_fused', ctypes.c_int, [_VP, ctypes.c_int]),
     ('jx_solver_get_fused', ctypes.c_int, [_VP]),
     ('jx_solver_profile', ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]),
+    ('jx_batch_create', ctypes.c_int, [_PP]),
+    ('jx_batch_destroy', ctypes.c_int, [_VP]),
+    ('jx_batch_add', ctypes.c_int,
+     [_VP, _VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+      _VP]),
+    ('jx_batch_ngroups', ctypes.c_int, [_VP, _VP]),
+    ('jx_batch_group', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]),
+    ('jx_batch_set_width', ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    ('jx_batch_set_schedule', ctypes.c_int,
+     [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ('jx_batch_run', ctypes.c_int, [_VP, _VP]),
+    ('jx_batch_result', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP]),
+    ('jx_batch_stats', ctypes.c_int, [_VP, _VP, _VP, _VP]),
 ]
 
 _lib = None

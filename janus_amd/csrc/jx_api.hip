@@ -7,6 +7,7 @@
 #include <string>
 
 #include "jx_gamma.h"
+#include "jx_handles.h"
 #include "jx_internal.h"
 
 namespace jx {
@@ -150,48 +151,10 @@ int get_fft_len(int L, FftLen *out) {
 
 using namespace jx;
 
-// ================================================================ handles
-struct jx_fft {
-  Grid g;
-  double *scratch = nullptr;  // spectrum copy for c2r (input is preserved)
-};
-
-struct jx_green {
-  Grid g;
-  int sym = 0;
-  double h = 1.0;
-  double g0 = 0, nu0 = 0;
-  GreenDev dev;
-  double *tables = nullptr;   // device per-axis tables
-  std::vector<double> host_tables;
-  GreenDev host;              // same tables, host pointers
-  double *spec = nullptr;     // spectrum scratch (Nh * 2 sym)
-};
-
-struct jx_local {
-  Grid g;
-  int sym = 0;
-  LocalDev dev;
-  double *table = nullptr;     // device (tr, dev) per phase
-  double *partials = nullptr;  // device, 1024 blocks x 2 sym columns + 2 sym
-};
-
 static int local_alloc_partials(jx_local *l) {
   JX_HIP(hipMalloc((void **)&l->partials, sizeof(double) * (1024 + 1) * 2 * l->sym));
   return JX_OK;
 }
-
-struct jx_solver {
-  jx_green *green = nullptr;
-  jx_local *local = nullptr;
-  bool fused = true;  // pass E of one iteration fused with pass A of the next
-  SolveState *st = nullptr;       // device
-  SolveState *st_host = nullptr;  // pinned host mirror
-  double *partials = nullptr;
-  int nblocks = 0;
-  double *res_dev = nullptr;
-  int res_cap = 0;
-};
 
 extern "C" {
 
@@ -326,9 +289,11 @@ int jx_green_create(int dim, const int64_t *shape, double h, int kind, double g0
   return JX_OK;
 }
 
+}  // extern "C"
+
 // Device tables, spectrum scratch and FFT plans are set up on first use, so
 // that the host-side per-frequency API works without a GPU.
-static int green_ensure_device(jx_green *G) {
+int jx::green_ensure_device(jx_green *G) {
   if (G->spec) return JX_OK;
   JX_HIP(hipMalloc((void **)&G->tables, sizeof(double) * G->host_tables.size()));
   JX_HIP(hipMemcpy(G->tables, G->host_tables.data(),
@@ -343,6 +308,8 @@ static int green_ensure_device(jx_green *G) {
   return JX_OK;
 }
 
+extern "C" {
+
 int jx_green_destroy(jx_green *G) {
   if (!G) return JX_OK;
   if (G->tables) (void)hipFree(G->tables);
@@ -351,19 +318,19 @@ int jx_green_destroy(jx_green *G) {
   return JX_OK;
 }
 
-static int green_passes(jx_green *G, const double *x, double *y,
+static int green_passes(jx_green *G, double *spec, const double *x, double *y,
                         const LocalDev &loc, const RowsInverseLs &ls,
                         const SolveState *st, int *nb, hipStream_t s) {
   const Grid &g = G->g;
   int C = G->sym;
-  int rc = launch_rows_forward(g, C, x, G->spec, loc, st, s);
+  int rc = launch_rows_forward(g, C, x, spec, loc, st, s);
   if (rc) return rc;
-  if (g.dim == 3 && (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, st, s)))
+  if (g.dim == 3 && (rc = launch_axis(g, C, 1, -1, spec, spec, nullptr, st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, st, s))) return rc;
-  if (g.dim == 3 && (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, st, s)))
+  if ((rc = launch_axis(g, C, 0, 0, spec, spec, &G->dev, st, s))) return rc;
+  if (g.dim == 3 && (rc = launch_axis(g, C, 1, +1, spec, spec, nullptr, st, s)))
     return rc;
-  return launch_rows_inverse(g, C, G->spec, y, ls, st, nb, s);
+  return launch_rows_inverse(g, C, spec, y, ls, st, nb, s);
 }
 
 int jx_green_apply(jx_green *G, const double *x, double *y, void *stream) {
@@ -372,7 +339,8 @@ int jx_green_apply(jx_green *G, const double *x, double *y, void *stream) {
   if (rc) return rc;
   LocalDev none;
   RowsInverseLs plain;
-  return green_passes(G, x, y, none, plain, nullptr, nullptr, (hipStream_t)stream);
+  return green_passes(G, G->spec, x, y, none, plain, nullptr, nullptr,
+                      (hipStream_t)stream);
 }
 
 int jx_green_matrix_at(jx_green *G, const int32_t *b, double *out) {
@@ -542,31 +510,34 @@ int jx_solver_destroy(jx_solver *S) {
   return JX_OK;
 }
 
-// One fixed-point iteration.  Fused schedule: pass A runs only on the first
-// iteration; every iteration then runs B, C, D and the fused E+A pass, which
-// leaves the spectrum of the next iteration's polarization in place.
-static int solver_iteration(jx_solver *S, double *eps, const RowsInverseLs &ls,
-                            bool first, int *nb, hipStream_t s) {
+}  // extern "C"
+
+// One fixed-point iteration on iterate eps, spectrum buffer spec and solve
+// state st.  Fused schedule: pass A runs only on the first iteration; every
+// iteration then runs B, C, D and the fused E+A pass, which leaves the
+// spectrum of the next iteration's polarization in place.
+int jx::solver_iteration(jx_solver *S, double *eps, double *spec,
+                         SolveState *st, const RowsInverseLs &ls, bool first,
+                         int *nb, hipStream_t s) {
   jx_green *G = S->green;
   const Grid &g = G->g;
   const int C = G->sym;
   if (!(S->fused && rows_fusable(g, C)))
-    return green_passes(G, eps, eps, S->local->dev, ls, S->st, nb, s);
+    return green_passes(G, spec, eps, eps, S->local->dev, ls, st, nb, s);
   int rc;
-  if (first && (rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev,
-                                         S->st, s)))
+  if (first &&
+      (rc = launch_rows_forward(g, C, eps, spec, S->local->dev, st, s)))
     return rc;
-  if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s)))
+  if (g.dim == 3 && (rc = launch_axis(g, C, 1, -1, spec, spec, nullptr, st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s)))
+  if ((rc = launch_axis(g, C, 0, 0, spec, spec, &G->dev, st, s))) return rc;
+  if (g.dim == 3 && (rc = launch_axis(g, C, 1, +1, spec, spec, nullptr, st, s)))
     return rc;
-  if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s)))
-    return rc;
-  return launch_rows_update_forward(g, C, G->spec, eps, ls, S->local->dev,
-                                    S->st, nb, s);
+  return launch_rows_update_forward(g, C, spec, eps, ls, S->local->dev, st, nb,
+                                    s);
 }
+
+extern "C" {
 
 int jx_solver_set_fused(jx_solver *S, int fused) {
   if (!S) return fail(JX_EINVAL, "jx_solver_set_fused: NULL solver");
@@ -610,6 +581,7 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
   init.res_scale = res_scale;
   init.max_iter = max_iter;
   init.pad = 0;
+  for (int c = 0; c < 6; ++c) init.E[c] = c < C ? E[c] : 0.;
   *S->st_host = init;
   JX_HIP(hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState), hipMemcpyHostToDevice, s));
   JX_HIP(hipStreamSynchronize(s));
@@ -618,7 +590,6 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
   ls.enabled = true;
   ls.rhs = rhs;
   ls.partials = S->partials;
-  for (int c = 0; c < C; ++c) ls.E[c] = E[c];
   // Enqueue iterations in chunks; kernels of iterations past convergence
   // return at once (the device state says done), so the iteration count is
   // exact while the host only synchronizes once per chunk.
@@ -628,7 +599,8 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
     int n = std::min(chunk, max_iter - enqueued);
     for (int i = 0; i < n; ++i) {
       int nb = 0;
-      int rc = solver_iteration(S, eps, ls, enqueued + i == 0, &nb, s);
+      int rc = solver_iteration(S, eps, G->spec, S->st, ls, enqueued + i == 0,
+                                &nb, s);
       if (rc) return rc;
       if ((rc = launch_finalize(S->partials, nb, S->st, S->res_dev, s))) return rc;
     }
@@ -688,19 +660,19 @@ int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
   init.res_scale = 1.;
   init.max_iter = iters + 1;
   init.pad = 0;
+  for (int c = 0; c < 6; ++c) init.E[c] = c < C ? E[c] : 0.;
   *S->st_host = init;
   JX_HIP(hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState), hipMemcpyHostToDevice, s));
   RowsInverseLs ls;
   ls.enabled = true;
   ls.partials = S->partials;
-  for (int c = 0; c < C; ++c) ls.E[c] = E[c];
   const int NP = 6;
   const bool fused = S->fused && rows_fusable(g, C);
   std::vector<hipEvent_t> ev((size_t)iters * (NP + 1));
   for (auto &e : ev) JX_HIP(hipEventCreate(&e));
   // one untimed iteration (it carries pass A in the fused schedule)
   int nb0 = 0;
-  int rc = solver_iteration(S, eps, ls, true, &nb0, s);
+  int rc = solver_iteration(S, eps, G->spec, S->st, ls, true, &nb0, s);
   if (!rc) rc = launch_finalize(S->partials, nb0, S->st, S->res_dev, s);
   for (int it = 0; it < iters && !rc; ++it) {
     hipEvent_t *e = &ev[(size_t)it * (NP + 1)];

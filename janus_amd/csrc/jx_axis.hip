@@ -1,0 +1,363 @@
+// Passes B, C and D of the Lippmann-Schwinger iteration: complex transforms
+// along the strided axes of the spectrum, with Gamma-hat applied between the
+// forward and inverse transforms of axis 0 (pass C).
+//
+// Spectrum layout (rows, nh, C, 2) as in pass A (jx_rows_fwd.hip).
+
+#include "jx_device.h"
+
+namespace jx {
+
+// ------------------------------------------------------------ passes B, C, D
+struct AxisArgs {
+  int L, Ls, T, ntiles;
+  int64_t stride_ax, n_outer, stride_outer, n_inner;
+  int nh;  // last-axis half length, decodes the inner index in 3D pass C
+};
+
+template <int C, int MODE>
+__global__ void __launch_bounds__(kThreads)
+    k_axis(const double *in, double *out, AxisArgs a, FftLen P, GreenDev G,
+           const SolveState *st) {
+  if (solve_done(st)) return;
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t o = blockIdx.x / a.ntiles;
+  const int64_t m0 = (int64_t)(blockIdx.x % a.ntiles) * a.T;
+  const int tc = (int)min((int64_t)a.T, a.n_inner - m0);
+  const int W2 = tc * 2 * C;
+  const int nseq = tc * C;
+  const int64_t base = o * a.stride_outer + m0 * 2 * C;
+  double2 *bufA = smem;
+  double2 *bufB = smem + (size_t)a.T * C * a.Ls;
+  const int L = a.L;
+
+  for (int v = tid; v < L * W2; v += nth) {
+    int e = v / W2;
+    int w = v - e * W2;
+    int t = w / (2 * C);
+    int rem = w - t * 2 * C;
+    int part = rem & 1;
+    int c = rem >> 1;
+    double val = in[base + (int64_t)e * a.stride_ax + w];
+    ((double *)&bufA[(t * C + c) * a.Ls + e])[part] = val;
+  }
+  __syncthreads();
+
+  double2 *res;
+  if (MODE == 0) {
+    res = lds_fft<-1>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+    if constexpr (C == 3 || C == 6) {
+      constexpr int DIM = C == 3 ? 2 : 3;
+      constexpr int NC = C == 3 ? 6 : 21;
+      for (int v = tid; v < tc * L; v += nth) {
+        int t = v / L, e = v - t * L;
+        int64_t m = m0 + t;
+        int b[3];
+        b[0] = e;
+        if (DIM == 2) {
+          b[1] = (int)m;
+        } else {
+          b[1] = (int)(m / a.nh);
+          b[2] = (int)(m - (int64_t)b[1] * a.nh);
+        }
+        double g[NC];
+        gamma_hat_moments<DIM>(G, b, g);
+        double xr[C], xi[C], yr[C], yi[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          double2 z = res[(t * C + c) * a.Ls + e];
+          xr[c] = z.x;
+          xi[c] = z.y;
+        }
+        sym_matvec<C>(g, xr, yr);
+        sym_matvec<C>(g, xi, yi);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          res[(t * C + c) * a.Ls + e] = make_double2(yr[c], yi[c]);
+      }
+    }
+    __syncthreads();
+    double2 *other = (res == bufA) ? bufB : bufA;
+    res = lds_fft<+1>(res, other, nseq, P, a.Ls, tid, nth);
+  } else {
+    res = lds_fft<MODE>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+  }
+
+  for (int v = tid; v < L * W2; v += nth) {
+    int e = v / W2;
+    int w = v - e * W2;
+    int t = w / (2 * C);
+    int rem = w - t * 2 * C;
+    int part = rem & 1;
+    int c = rem >> 1;
+    out[base + (int64_t)e * a.stride_ax + w] =
+        ((const double *)&res[(t * C + c) * a.Ls + e])[part];
+  }
+}
+
+// Gamma-hat at frequency (b0 = e, inner index m) applied to the C complex
+// values z[c * zs] (real and imaginary parts separately).
+template <int C>
+__device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
+                                            int nh, double2 *z, int zs) {
+  constexpr int DIM = C == 3 ? 2 : 3;
+  constexpr int NC = C == 3 ? 6 : 21;
+  int b[3];
+  b[0] = e;
+  if (DIM == 2) {
+    b[1] = (int)m;
+  } else {
+    b[1] = (int)(m / nh);
+    b[2] = (int)(m - (int64_t)b[1] * nh);
+  }
+  double g[NC];
+  gamma_hat_moments<DIM>(G, b, g);
+  double xr[C], xi[C], yr[C], yi[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    double2 v = z[c * zs];
+    xr[c] = v.x;
+    xi[c] = v.y;
+  }
+  sym_matvec<C>(g, xr, yr);
+  sym_matvec<C>(g, xi, yi);
+#pragma unroll
+  for (int c = 0; c < C; ++c) z[c * zs] = make_double2(yr[c], yi[c]);
+}
+
+// Register-resident transform along a strided axis of the spectrum for the
+// register FFT lengths.  NSEQ = T * C complex sequences per block (T points
+// of the contiguous inner index, all C components), TPP = L / V threads
+// each.  Thread tid = t * NSEQ + s: consecutive lanes read consecutive
+// doubles of the T * 2C doubles that one axis element of the tile spans.
+// Batched: the entry's spectrum (in place), Green constants and state come
+// from its slot.
+//
+// Minimum waves per SIMD requested for the Gamma-hat pass of the 16-value
+// plans (register budget of the compiler).  3 (a few spilled registers)
+// measured faster than the compiler's own choice of 2 waves at 256^3; the
+// 20- and 24-value plans would spill heavily and keep the compiler's choice.
+#ifndef JX_GAMMA_PASS_MIN_WAVES
+#define JX_GAMMA_PASS_MIN_WAVES 3
+#endif
+
+template <int L, int C, int NSEQ, int MODE>
+__global__ void __launch_bounds__(NSEQ * reg_tpp<L>(),
+                                  MODE == 0 && reg_v<L>() == 16
+                                      ? JX_GAMMA_PASS_MIN_WAVES
+                                      : 1)
+    k_axis_reg(const double *in, double *out, AxisArgs a, const double2 *W,
+               GreenDev G, const SolveState *st, BatchRef br) {
+  const Slot *sl;
+  const int64_t blk = batch_block(br, sl);
+  if (sl) {
+    in = sl->spec;
+    out = sl->spec;
+    G.daux1 = sl->daux[0];
+    G.daux2 = sl->daux[1];
+    G.daux3 = sl->daux[2];
+    G.daux4 = sl->daux[3];
+    st = sl->st;
+  }
+  if (solve_done(st)) return;
+  constexpr int TPP = reg_tpp<L>();
+  constexpr int V = reg_v<L>();
+  constexpr int T = NSEQ / C;
+  constexpr int LS = reg_lds_stride(L);
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x;
+  const int s = tid % NSEQ, t = tid / NSEQ;
+  const int tp = s / C, c = s - tp * C;
+  const int64_t o = blk / a.ntiles;
+  const int64_t m0 = (blk - o * a.ntiles) * T;
+  const int tc = (int)min((int64_t)T, a.n_inner - m0);
+  const bool valid = tp < tc;
+  const double *src = in + o * a.stride_outer + (m0 + tp) * 2 * C + 2 * c;
+  double *dst = out + o * a.stride_outer + (m0 + tp) * 2 * C + 2 * c;
+  double2 v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    if (valid) {
+      v[i] = *(const double2 *)(src + (int64_t)reg_in_pos<L>(t, i) * a.stride_ax);
+    } else {
+      v[i] = make_double2(0., 0.);
+    }
+  }
+  // the Gamma-hat pass reads the global table: its LDS budget is kept for
+  // three blocks per CU
+  constexpr bool TQ = MODE != 0;
+  __shared__ double2 twl[TQ ? L / 4 : 1];
+  const double2 *Wt = stage_twiddles<L, NSEQ * TPP, TQ>(W, twl);
+  double2 *seq = smem + s * LS;
+  reg_fft<L, (MODE > 0 ? 1 : -1), TQ>(v, t, seq, Wt);
+  if constexpr (MODE == 0 && (C == 3 || C == 6)) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+    __syncthreads();
+    for (int w = tid; w < tc * L; w += NSEQ * TPP) {
+      int p = w / L, e = w - p * L;
+      gamma_point<C>(G, e, m0 + p, a.nh, smem + (p * C) * LS + lp(e), LS);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+    __syncthreads();
+    reg_fft<L, 1, TQ>(v, t, seq, Wt);
+  }
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      *(double2 *)(dst + (int64_t)reg_out_pos<L>(t, i) * a.stride_ax) = v[i];
+    }
+  }
+}
+
+// ============================================================ launchers
+
+template <int C>
+static void launch_axis_c(int dir, dim3 grid, size_t lds, hipStream_t s,
+                          const double *in, double *out, const AxisArgs &a,
+                          const FftLen &P, const GreenDev &G,
+                          const SolveState *st) {
+  if (dir < 0) {
+    set_lds_attr(k_axis<C, -1>, lds);
+    hipLaunchKernelGGL((k_axis<C, -1>), grid, dim3(kThreads), lds, s, in, out,
+                       a, P, G, st);
+  } else if (dir > 0) {
+    set_lds_attr(k_axis<C, 1>, lds);
+    hipLaunchKernelGGL((k_axis<C, 1>), grid, dim3(kThreads), lds, s, in, out,
+                       a, P, G, st);
+  } else {
+    set_lds_attr(k_axis<C, 0>, lds);
+    hipLaunchKernelGGL((k_axis<C, 0>), grid, dim3(kThreads), lds, s, in, out,
+                       a, P, G, st);
+  }
+}
+
+template <int L, int C, int NSEQ>
+static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
+                               const double *in, double *out,
+                               const AxisArgs &a, const double2 *W,
+                               const GreenDev &G, const SolveState *st,
+                               const BatchRef &br) {
+  dim3 block(NSEQ * reg_tpp<L>());
+  if (dir < 0) {
+    set_lds_attr(k_axis_reg<L, C, NSEQ, -1>, lds);
+    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, -1>), grid, block, lds, s, in,
+                       out, a, W, G, st, br);
+  } else if (dir > 0) {
+    set_lds_attr(k_axis_reg<L, C, NSEQ, 1>, lds);
+    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, 1>), grid, block, lds, s, in,
+                       out, a, W, G, st, br);
+  } else {
+    set_lds_attr(k_axis_reg<L, C, NSEQ, 0>, lds);
+    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, 0>), grid, block, lds, s, in,
+                       out, a, W, G, st, br);
+  }
+}
+
+template <int L>
+static int launch_axis_reg_l(const Grid &g, int ncomp, int axis, int dir,
+                             const double *in, double *out,
+                             const GreenDev *green, const SolveState *st,
+                             const FftLen &P, hipStream_t s, const Batch *b) {
+  constexpr int NSEQ = reg_nseq<L>();
+  static_assert(NSEQ % 6 == 0, "sequences per block must hold whole points");
+  AxisArgs a;
+  a.L = L;
+  a.Ls = reg_lds_stride(L);
+  a.nh = (int)g.nh;
+  int64_t inner = g.nh;
+  for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
+  int64_t outer = 1;
+  for (int d = 0; d < axis; ++d) outer *= g.n[d];
+  a.n_inner = inner;
+  a.n_outer = outer;
+  a.stride_ax = inner * 2 * ncomp;
+  a.stride_outer = (int64_t)L * inner * 2 * ncomp;
+  const int T = NSEQ / ncomp;
+  a.T = T;
+  a.ntiles = (int)((inner + T - 1) / T);
+  size_t lds = (size_t)NSEQ * reg_lds_stride(L) * sizeof(double2);
+  int rc = check_lds(lds, "axis");
+  if (rc) return rc;
+  const int64_t nbe = outer * a.ntiles;
+  const BatchRef br = batch_ref(b, nbe);
+  dim3 grid((unsigned)(nbe * batch_count(b)));
+  GreenDev G = green ? *green : GreenDev();
+  switch (ncomp) {
+    case 1: launch_axis_reg_lc<L, 1, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st, br); break;
+    case 3: launch_axis_reg_lc<L, 3, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st, br); break;
+    case 6: launch_axis_reg_lc<L, 6, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st, br); break;
+    default: return fail(JX_EINVAL, "launch_axis: ncomp must be 1, 3 or 6");
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+static int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
+                           const double *in, double *out,
+                           const GreenDev *green, const SolveState *st,
+                           const FftLen &P, hipStream_t s, const Batch *b) {
+  switch (P.L) {
+    case 16: return launch_axis_reg_l<16>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 32: return launch_axis_reg_l<32>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 64: return launch_axis_reg_l<64>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 128: return launch_axis_reg_l<128>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 256: return launch_axis_reg_l<256>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 512: return launch_axis_reg_l<512>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 80: return launch_axis_reg_l<80>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 96: return launch_axis_reg_l<96>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    case 160: return launch_axis_reg_l<160>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+    default: return fail(JX_EUNSUPPORTED, "no register FFT for this length");
+  }
+}
+
+int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
+                double *out, const GreenDev *green, const SolveState *st,
+                hipStream_t s, const Batch *b) {
+  if (axis < 0 || axis >= g.dim - 1)
+    return fail(JX_EINVAL, "launch_axis: axis out of range");
+  if (dir == 0 && (axis != 0 || green == nullptr || ncomp == 1))
+    return fail(JX_EINVAL, "launch_axis: Gamma-hat pass needs axis 0");
+  FftLen P;
+  int rc = get_fft_len((int)g.n[axis], &P);
+  if (rc) return rc;
+  if (reg_fft_length((int)g.n[axis]))
+    return launch_axis_reg(g, ncomp, axis, dir, in, out, green, st, P, s, b);
+  if (b) return fail(JX_EUNSUPPORTED, "axis: batch needs a register FFT length");
+  AxisArgs a;
+  a.L = (int)g.n[axis];
+  a.Ls = lds_stride(a.L);
+  a.nh = (int)g.nh;
+  // points after `axis` (all inner axes, last one halved) are contiguous
+  int64_t inner = g.nh;
+  for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
+  int64_t outer = 1;
+  for (int d = 0; d < axis; ++d) outer *= g.n[d];
+  a.n_inner = inner;
+  a.n_outer = outer;
+  a.stride_ax = inner * 2 * ncomp;
+  a.stride_outer = (int64_t)a.L * inner * 2 * ncomp;
+  int T = std::max(1, 1536 / (a.Ls * ncomp));
+  if (T > inner) T = (int)inner;
+  a.T = T;
+  a.ntiles = (int)((inner + T - 1) / T);
+  size_t lds = 2 * (size_t)T * ncomp * a.Ls * sizeof(double2);
+  if ((rc = check_lds(lds, "axis"))) return rc;
+  int64_t nb = outer * a.ntiles;
+  dim3 grid((unsigned)nb);
+  GreenDev G = green ? *green : GreenDev();
+  switch (ncomp) {
+    case 1: launch_axis_c<1>(dir, grid, lds, s, in, out, a, P, G, st); break;
+    case 3: launch_axis_c<3>(dir, grid, lds, s, in, out, a, P, G, st); break;
+    case 6: launch_axis_c<6>(dir, grid, lds, s, in, out, a, P, G, st); break;
+    default: return fail(JX_EINVAL, "launch_axis: ncomp must be 1, 3 or 6");
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+}  // namespace jx

@@ -1,0 +1,556 @@
+// Batched fixed-point solves: many right-hand sides (load cases of one
+// solver, solvers of different grids) run to convergence in one sequence of
+// launches.
+//
+// Entries on the same grid, Green operator kind and h, with the same
+// schedule and rhs use, form a group.  A group is cut into chunks of `width`
+// entries; every pass of a chunk is one launch of the batched register
+// kernels (the block index selects the entry's slot), so small grids fill
+// the GPU.  A replay runs `iters_per_launch` iterations of every chunk that
+// still has a running entry, chunk after chunk, so that a chunk's working
+// set stays in cache across its passes and iterations.  The first replay
+// runs eagerly; with autotuning the next replays time the candidate widths
+// of each group (hipEvents around the group's launches) and keep the
+// fastest; the others are captured once in a hipGraph and replayed, and
+// captured again only when a chunk has finished.  The host reads the solve
+// states once per replay.  Each entry runs exactly the kernels and
+// arithmetic of jx_solver_solve, and its converged blocks skip their work,
+// so the iterates and iteration counts are those of the sequential solves.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "jx_handles.h"
+#include "jx_internal.h"
+
+using namespace jx;
+
+namespace {
+
+struct Entry {
+  jx_solver *S = nullptr;
+  double *eps = nullptr;
+  const double *rhs = nullptr;
+  double E[6] = {};
+  double tol = 0, res_scale = 0;
+  int max_iter = 0;
+  double *spec = nullptr, *partials = nullptr, *res_hist = nullptr;
+  int slot = -1;  // position in the slot table of the last run
+  int iters = 0, status = 0;
+};
+
+struct Group {
+  std::vector<int> members;  // entry indices, in slot order
+  int off = 0;               // first slot
+  bool batched = false;      // register kernels take the whole chunk
+  bool fused = false;
+  int width = 0;             // entries per launch (0 = all)
+  bool user_width = false;
+};
+
+}  // namespace
+
+struct jx_batch {
+  std::vector<Entry> ents;
+  std::vector<Group> groups;
+  std::map<int, int> width_set;  // group -> width chosen by the caller
+  int iters_per_launch = 8;
+  bool graph = true;
+  bool autotune = false;
+  // device state of the last run
+  Slot *tab = nullptr;
+  SolveState *st = nullptr;
+  SolveState *st_host = nullptr;
+  int cap = 0;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // statistics of the last run
+  double ms = 0;
+  int replays = 0, captures = 0;
+};
+
+namespace {
+
+using Key = std::tuple<int, int64_t, int64_t, int64_t, int, uint64_t, bool,
+                       bool>;
+
+Key entry_key(const Entry &e) {
+  const jx_green *G = e.S->green;
+  uint64_t hb = 0;
+  std::memcpy(&hb, &G->h, sizeof(hb));
+  bool fused = e.S->fused && rows_fusable(G->g, G->sym);
+  return Key(G->g.dim, G->g.n[0], G->g.n[1], G->g.n[2], G->dev.kind, hb,
+             e.rhs != nullptr, fused);
+}
+
+void build_groups(jx_batch *B) {
+  std::map<Key, int> index;
+  B->groups.clear();
+  for (int i = 0; i < (int)B->ents.size(); ++i) {
+    Key k = entry_key(B->ents[i]);
+    auto it = index.find(k);
+    if (it == index.end()) {
+      it = index.emplace(k, (int)B->groups.size()).first;
+      Group gr;
+      const jx_green *G = B->ents[i].S->green;
+      gr.batched = batchable(G->g, G->sym);
+      gr.fused = std::get<7>(k);
+      B->groups.push_back(gr);
+    }
+    B->groups[it->second].members.push_back(i);
+  }
+  int off = 0;
+  for (int g = 0; g < (int)B->groups.size(); ++g) {
+    Group &gr = B->groups[g];
+    gr.off = off;
+    off += (int)gr.members.size();
+    auto w = B->width_set.find(g);
+    gr.user_width = w != B->width_set.end();
+    gr.width = gr.user_width ? w->second : 0;
+    if (!gr.batched) gr.width = 1;
+  }
+}
+
+int chunk_width(const Group &gr, int width) {
+  int n = (int)gr.members.size();
+  if (!gr.batched) return 1;
+  return (width <= 0 || width > n) ? n : width;
+}
+
+// One iteration of the chunk of group gr starting at member c0 with n
+// entries.
+int enqueue_chunk_iteration(jx_batch *B, const Group &gr, int c0, int n,
+                            bool first, hipStream_t s) {
+  int rc;
+  if (!gr.batched) {
+    for (int m = c0; m < c0 + n; ++m) {
+      Entry &e = B->ents[gr.members[m]];
+      RowsInverseLs ls;
+      ls.enabled = true;
+      ls.rhs = e.rhs;
+      ls.partials = e.partials;
+      SolveState *st = B->st + e.slot;
+      int nb = 0;
+      if ((rc = solver_iteration(e.S, e.eps, e.spec, st, ls, first, &nb, s)))
+        return rc;
+      if ((rc = launch_finalize(e.partials, nb, st, e.res_hist, s))) return rc;
+    }
+    return JX_OK;
+  }
+  const Entry &e0 = B->ents[gr.members[c0]];
+  const jx_green *G = e0.S->green;
+  const Grid &g = G->g;
+  const int C = G->sym;
+  const LocalDev &loc = e0.S->local->dev;
+  Batch b;
+  b.tab = B->tab + gr.off + c0;
+  b.n = n;
+  RowsInverseLs ls;
+  ls.enabled = true;
+  ls.rhs = e0.rhs;  // selects the kernel variant; the slots hold the fields
+  int nb = 0;
+  if ((!gr.fused || first) &&
+      (rc = launch_rows_forward(g, C, nullptr, nullptr, loc, nullptr, s, &b)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, -1, nullptr, nullptr, nullptr, nullptr, s, &b)))
+    return rc;
+  if ((rc = launch_axis(g, C, 0, 0, nullptr, nullptr, &G->dev, nullptr, s, &b)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, +1, nullptr, nullptr, nullptr, nullptr, s, &b)))
+    return rc;
+  if (gr.fused)
+    rc = launch_rows_update_forward(g, C, nullptr, nullptr, ls, loc, nullptr,
+                                    &nb, s, &b);
+  else
+    rc = launch_rows_inverse(g, C, nullptr, nullptr, ls, nullptr, &nb, s, &b);
+  if (rc) return rc;
+  return launch_finalize(nullptr, nb, nullptr, nullptr, s, &b);
+}
+
+bool entry_running(const jx_batch *B, int i) {
+  return B->st_host[B->ents[i].slot].done == 0;
+}
+
+// iters iterations of every chunk of group gr that has a running entry.
+int enqueue_group(jx_batch *B, const Group &gr, int width, int iters,
+                  bool first, hipStream_t s) {
+  const int n = (int)gr.members.size();
+  const int w = chunk_width(gr, width);
+  for (int c0 = 0; c0 < n; c0 += w) {
+    int cn = std::min(w, n - c0);
+    bool running = false;
+    for (int m = c0; m < c0 + cn; ++m) running |= entry_running(B, gr.members[m]);
+    if (!running) continue;
+    for (int k = 0; k < iters; ++k) {
+      int rc = enqueue_chunk_iteration(B, gr, c0, cn, first && k == 0, s);
+      if (rc) return rc;
+    }
+  }
+  return JX_OK;
+}
+
+// Chunks with a running entry, as (group, first member) pairs: a new graph
+// is captured when this set changes.
+std::vector<std::pair<int, int>> running_chunks(const jx_batch *B) {
+  std::vector<std::pair<int, int>> out;
+  for (int g = 0; g < (int)B->groups.size(); ++g) {
+    const Group &gr = B->groups[g];
+    const int n = (int)gr.members.size();
+    const int w = chunk_width(gr, gr.width);
+    for (int c0 = 0; c0 < n; c0 += w) {
+      int cn = std::min(w, n - c0);
+      for (int m = c0; m < c0 + cn; ++m)
+        if (entry_running(B, gr.members[m])) {
+          out.emplace_back(g, c0);
+          break;
+        }
+    }
+  }
+  return out;
+}
+
+int read_states(jx_batch *B, hipStream_t s) {
+  const int n = (int)B->ents.size();
+  JX_HIP(hipMemcpyAsync(B->st_host, B->st, sizeof(SolveState) * n,
+                        hipMemcpyDeviceToHost, s));
+  JX_HIP(hipStreamSynchronize(s));
+  return JX_OK;
+}
+
+void drop_graph(jx_batch *B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  B->exec = nullptr;
+}
+
+// Captures one replay (all running chunks) into B->exec.
+int capture(jx_batch *B, hipStream_t s) {
+  drop_graph(B);
+  JX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  int rc = JX_OK;
+  for (const Group &gr : B->groups) {
+    rc = enqueue_group(B, gr, gr.width, B->iters_per_launch, false, s);
+    if (rc) break;
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "jx_batch_run: hipStreamEndCapture");
+  e = hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    B->exec = nullptr;
+    return hip_fail(e, "jx_batch_run: hipGraphInstantiate");
+  }
+  ++B->captures;
+  return JX_OK;
+}
+
+std::vector<int> candidate_widths(const Group &gr) {
+  const int n = (int)gr.members.size();
+  std::vector<int> c;
+  if (!gr.batched || gr.user_width || n <= 1) return c;
+  for (int w : {n, 6, 3, 1})
+    if (w <= n && std::find(c.begin(), c.end(), w) == c.end()) c.push_back(w);
+  return c;
+}
+
+void free_entry(Entry &e) {
+  if (e.spec) (void)hipFree(e.spec);
+  if (e.partials) (void)hipFree(e.partials);
+  if (e.res_hist) (void)hipFree(e.res_hist);
+  e.spec = e.partials = e.res_hist = nullptr;
+}
+
+void free_run_state(jx_batch *B) {
+  drop_graph(B);
+  if (B->tab) (void)hipFree(B->tab);
+  if (B->st) (void)hipFree(B->st);
+  if (B->st_host) (void)hipHostFree(B->st_host);
+  B->tab = nullptr;
+  B->st = nullptr;
+  B->st_host = nullptr;
+  B->cap = 0;
+}
+
+int run(jx_batch *B, hipStream_t caller) {
+  const int n = (int)B->ents.size();
+  if (n == 0) return JX_OK;
+  build_groups(B);
+  if (B->cap < n) {
+    free_run_state(B);
+    JX_HIP(hipMalloc((void **)&B->tab, sizeof(Slot) * n));
+    JX_HIP(hipMalloc((void **)&B->st, sizeof(SolveState) * n));
+    JX_HIP(hipHostMalloc((void **)&B->st_host, sizeof(SolveState) * n, 0));
+    B->cap = n;
+  }
+  drop_graph(B);
+  if (!B->stream)
+    JX_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+  hipStream_t s = B->stream;
+  JX_HIP(hipStreamSynchronize(caller));
+
+  std::vector<Slot> tab(n);
+  for (Group &gr : B->groups)
+    for (int m = 0; m < (int)gr.members.size(); ++m) {
+      Entry &e = B->ents[gr.members[m]];
+      e.slot = gr.off + m;
+      Slot &sl = tab[e.slot];
+      sl.eps = e.eps;
+      sl.spec = e.spec;
+      sl.rhs = e.rhs;
+      sl.partials = e.partials;
+      sl.res_hist = e.res_hist;
+      sl.st = B->st + e.slot;
+      sl.loc = e.S->local->dev;
+      const GreenDev &D = e.S->green->dev;
+      sl.daux[0] = D.daux1;
+      sl.daux[1] = D.daux2;
+      sl.daux[2] = D.daux3;
+      sl.daux[3] = D.daux4;
+      SolveState &st = B->st_host[e.slot];
+      st.iter = 0;
+      st.done = 0;
+      st.tol = e.tol;
+      st.res_scale = e.res_scale;
+      st.max_iter = e.max_iter;
+      st.pad = 0;
+      for (int c = 0; c < 6; ++c) st.E[c] = e.E[c];
+    }
+  JX_HIP(hipMemcpyAsync(B->tab, tab.data(), sizeof(Slot) * n,
+                        hipMemcpyHostToDevice, s));
+  JX_HIP(hipMemcpyAsync(B->st, B->st_host, sizeof(SolveState) * n,
+                        hipMemcpyHostToDevice, s));
+  JX_HIP(hipStreamSynchronize(s));
+  B->replays = 0;
+  B->captures = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  const int K = B->iters_per_launch;
+  int rc = JX_OK;
+
+  // replay 0: eager, carries pass A of the fused groups
+  for (Group &gr : B->groups) {
+    std::vector<int> cand = candidate_widths(gr);
+    if (!cand.empty()) gr.width = cand[0];
+    if ((rc = enqueue_group(B, gr, gr.width, K, true, s))) return rc;
+  }
+  if ((rc = read_states(B, s))) return rc;
+  ++B->replays;
+
+  // autotuning replays: candidate r of every group, timed per group
+  if (B->autotune) {
+    const int ng = (int)B->groups.size();
+    std::vector<std::vector<int>> cand(ng);
+    std::vector<std::vector<double>> cost(ng);
+    size_t rounds = 0;
+    for (int g = 0; g < ng; ++g) {
+      cand[g] = candidate_widths(B->groups[g]);
+      rounds = std::max(rounds, cand[g].size());
+    }
+    std::vector<hipEvent_t> ev(2 * ng);
+    for (auto &e : ev) JX_HIP(hipEventCreate(&e));
+    for (size_t r = 0; r < rounds && !rc; ++r) {
+      std::vector<int> running(ng, 0);
+      for (int g = 0; g < ng; ++g)
+        for (int i : B->groups[g].members) running[g] += entry_running(B, i);
+      for (int g = 0; g < ng && !rc; ++g) {
+        Group &gr = B->groups[g];
+        int w = r < cand[g].size() ? cand[g][r] : gr.width;
+        (void)hipEventRecord(ev[2 * g], s);
+        rc = enqueue_group(B, gr, w, K, false, s);
+        (void)hipEventRecord(ev[2 * g + 1], s);
+      }
+      if (!rc) rc = read_states(B, s);
+      ++B->replays;
+      for (int g = 0; g < ng && !rc; ++g) {
+        if (r >= cand[g].size()) continue;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[2 * g], ev[2 * g + 1]);
+        // time per running entry and iteration
+        cost[g].push_back(running[g] ? ms / (double)running[g] : 1e30);
+      }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int g = 0; g < ng; ++g) {
+      if (cost[g].empty()) continue;
+      size_t best = 0;
+      for (size_t i = 1; i < cost[g].size(); ++i)
+        if (cost[g][i] < cost[g][best]) best = i;
+      B->groups[g].width = cand[g][best];
+    }
+  }
+
+  // steady state: captured replays until every entry has stopped
+  std::vector<std::pair<int, int>> live = running_chunks(B);
+  while (!live.empty()) {
+    if (B->graph) {
+      if (!B->exec && (rc = capture(B, s))) return rc;
+      JX_HIP(hipGraphLaunch(B->exec, s));
+    } else {
+      for (const Group &gr : B->groups)
+        if ((rc = enqueue_group(B, gr, gr.width, K, false, s))) return rc;
+    }
+    if ((rc = read_states(B, s))) return rc;
+    ++B->replays;
+    std::vector<std::pair<int, int>> now = running_chunks(B);
+    if (now != live) drop_graph(B);
+    live.swap(now);
+  }
+  drop_graph(B);
+  B->ms = std::chrono::duration<double, std::milli>(
+              std::chrono::steady_clock::now() - t0).count();
+  for (Entry &e : B->ents) {
+    const SolveState &st = B->st_host[e.slot];
+    e.iters = st.iter;
+    e.status = st.done ? st.done : JX_SOLVE_MAXITER;
+  }
+  return JX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jx_batch_create(jx_batch **out) {
+  if (!out) return fail(JX_EINVAL, "jx_batch_create: out is NULL");
+  *out = new jx_batch();
+  return JX_OK;
+}
+
+int jx_batch_destroy(jx_batch *B) {
+  if (!B) return JX_OK;
+  free_run_state(B);
+  for (Entry &e : B->ents) free_entry(e);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+  return JX_OK;
+}
+
+int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
+                 const double *rhs, double tol, int max_iter, double res_scale,
+                 int *index) {
+  if (!B || !S || !eps || !E)
+    return fail(JX_EINVAL, "jx_batch_add: NULL argument");
+  if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
+  const jx_green *G = S->green;
+  const int C = G->sym;
+  Entry e;
+  e.S = S;
+  e.eps = eps;
+  e.rhs = rhs;
+  for (int c = 0; c < C; ++c) e.E[c] = E[c];
+  e.tol = tol;
+  e.max_iter = max_iter;
+  if (res_scale <= 0.) {
+    double nE = 0.;
+    for (int c = 0; c < C; ++c) nE += E[c] * E[c];
+    nE = std::sqrt(nE);
+    if (nE == 0.) return fail(JX_EINVAL, "res_scale must be given when E = 0");
+    res_scale = 1.0 / std::sqrt((double)G->g.N) / nE;  // square_basic.py:86
+  }
+  e.res_scale = res_scale;
+  for (const Entry &o : B->ents)
+    if (o.eps == eps)
+      return fail(JX_EINVAL, "jx_batch_add: each entry needs its own eps");
+  int nb = rows_inverse_nblocks(G->g, C);
+  hipError_t he = hipMalloc((void **)&e.spec, sizeof(double) * 2 * C * G->g.Nh);
+  if (he == hipSuccess) he = hipMalloc((void **)&e.partials, sizeof(double) * nb);
+  if (he == hipSuccess)
+    he = hipMalloc((void **)&e.res_hist, sizeof(double) * max_iter);
+  if (he != hipSuccess) {
+    free_entry(e);
+    return hip_fail(he, "jx_batch_add");
+  }
+  B->ents.push_back(e);
+  if (index) *index = (int)B->ents.size() - 1;
+  return JX_OK;
+}
+
+int jx_batch_ngroups(jx_batch *B, int *ngroups) {
+  if (!B || !ngroups) return fail(JX_EINVAL, "jx_batch_ngroups: NULL argument");
+  std::vector<Group> keep = B->groups;
+  build_groups(B);
+  // keep the widths chosen by the last run
+  if (keep.size() == B->groups.size())
+    for (size_t g = 0; g < keep.size(); ++g)
+      if (keep[g].members == B->groups[g].members)
+        B->groups[g].width = keep[g].width;
+  *ngroups = (int)B->groups.size();
+  return JX_OK;
+}
+
+int jx_batch_group(jx_batch *B, int g, int *dim, int64_t *shape, int *size,
+                   int *width, int *batched) {
+  if (!B) return fail(JX_EINVAL, "jx_batch_group: NULL batch");
+  if (g < 0 || g >= (int)B->groups.size())
+    return fail(JX_EINVAL, "jx_batch_group: group out of range");
+  const Group &gr = B->groups[g];
+  const Grid &G = B->ents[gr.members[0]].S->green->g;
+  if (dim) *dim = G.dim;
+  if (shape)
+    for (int d = 0; d < 3; ++d) shape[d] = d < G.dim ? G.n[d] : 1;
+  if (size) *size = (int)gr.members.size();
+  if (width) *width = chunk_width(gr, gr.width);
+  if (batched) *batched = gr.batched ? 1 : 0;
+  return JX_OK;
+}
+
+int jx_batch_set_width(jx_batch *B, int g, int width) {
+  if (!B || g < 0) return fail(JX_EINVAL, "jx_batch_set_width: invalid argument");
+  if (width < 0) return fail(JX_EINVAL, "width must be >= 0");
+  B->width_set[g] = width;
+  return JX_OK;
+}
+
+int jx_batch_set_schedule(jx_batch *B, int iters_per_launch, int graph,
+                          int autotune) {
+  if (!B) return fail(JX_EINVAL, "jx_batch_set_schedule: NULL batch");
+  if (iters_per_launch < 1) return fail(JX_EINVAL, "iters_per_launch must be >= 1");
+  B->iters_per_launch = iters_per_launch;
+  B->graph = graph != 0;
+  B->autotune = autotune != 0;
+  return JX_OK;
+}
+
+int jx_batch_run(jx_batch *B, void *stream) {
+  if (!B) return fail(JX_EINVAL, "jx_batch_run: NULL batch");
+  int rc = run(B, (hipStream_t)stream);
+  if (rc) {
+    drop_graph(B);
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+  }
+  return rc;
+}
+
+int jx_batch_result(jx_batch *B, int index, double *res_hist, int *iters,
+                    int *status) {
+  if (!B || index < 0 || index >= (int)B->ents.size())
+    return fail(JX_EINVAL, "jx_batch_result: invalid entry");
+  const Entry &e = B->ents[index];
+  if (iters) *iters = e.iters;
+  if (status) *status = e.status;
+  if (res_hist && e.iters > 0)
+    JX_HIP(hipMemcpy(res_hist, e.res_hist, sizeof(double) * e.iters,
+                     hipMemcpyDeviceToHost));
+  return JX_OK;
+}
+
+int jx_batch_stats(jx_batch *B, double *ms, int *replays, int *captures) {
+  if (!B) return fail(JX_EINVAL, "jx_batch_stats: NULL batch");
+  if (ms) *ms = B->ms;
+  if (replays) *replays = B->replays;
+  if (captures) *captures = B->captures;
+  return JX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,258 @@
+// Device helpers and launch plumbing shared by the kernel sources.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+
+#include "jx_fft_lds.h"
+#include "jx_fft_reg.h"
+#include "jx_gamma.h"
+#include "jx_internal.h"
+
+namespace jx {
+
+// LDS row stride of a sequence: one complex pad for even lengths breaks the
+// power-of-two strides between sequences.
+static inline int lds_stride(int L) { return (L % 2 == 0) ? L + 1 : L; }
+
+// ------------------------------------------------------------ local operator
+// (sph J + dev K) : e in Mandel notation, janus/operators.pyx:315-324 (2D)
+// and :352-365 (3D), tr = (sph - dev) / dim (:299).  Contraction into FMA is
+// disabled so that the result is the one of the separate products and sums.
+template <int C>
+__device__ __forceinline__ void iso4(double tr, double dev, const double *e,
+                                     double *t) {
+#pragma clang fp contract(off)
+  if (C == 3) {
+    double aux = tr * (e[0] + e[1]);
+    t[0] = aux + dev * e[0];
+    t[1] = aux + dev * e[1];
+    t[2] = dev * e[2];
+  } else {
+    double aux = tr * (e[0] + e[1] + e[2]);
+    t[0] = aux + dev * e[0];
+    t[1] = aux + dev * e[1];
+    t[2] = aux + dev * e[2];
+    t[3] = dev * e[3];
+    t[4] = dev * e[4];
+    t[5] = dev * e[5];
+  }
+}
+
+// Phase tables hold (tr, dev) per phase; field mode computes tr per voxel.
+template <int C>
+__device__ __forceinline__ void local_voxel(const LocalDev &loc, int64_t vox,
+                                            const double *e, double *t) {
+  double tr, dev;
+  if (loc.mode == LOCAL_PHASES) {
+    int p = loc.phase[vox];
+    tr = loc.table[2 * p];
+    dev = loc.table[2 * p + 1];
+  } else {
+    double sph = loc.sph[vox];
+    dev = loc.dev[vox];
+    tr = (sph - dev) / (C == 3 ? 2.0 : 3.0);
+  }
+  iso4<C>(tr, dev, e, t);
+}
+
+// ------------------------------------------------------------ update
+// Fixed-point update of one component: (E + rhs) - scale * y, evaluated the
+// same way (no contraction into FMA) in every kernel, so that all schedules
+// produce the same iterates bit for bit.
+__device__ __forceinline__ double ls_new(double e, double y, double scale) {
+#pragma clang fp contract(off)
+  double val = scale * y;
+  return e - val;
+}
+__device__ __forceinline__ double ls_new_rhs(double e, double h, double y,
+                                             double scale) {
+#pragma clang fp contract(off)
+  double val = scale * y;
+  return (e + h) - val;
+}
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum (fixed order); result valid in thread 0.
+static __device__ double block_sum(double v, double *red) {
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += red[w];
+  }
+  return s;
+}
+
+struct RowsArgs {
+  int64_t nrows;
+  int L, nh, Ls, RB;
+  double scale;
+};
+
+__device__ __forceinline__ bool solve_done(const SolveState *st) {
+  return st != nullptr && *((volatile const int *)&st->done) != 0;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that tiles sharing cache lines meet in the
+// same L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t nx = 8;
+  int64_t q = nb / nx, r = nb - q * nx;
+  int64_t x = b - (b / nx) * nx, slot = b / nx;
+  return x * q + (x < r ? x : r) + slot;
+}
+
+// Stage the twiddle entries a register kernel reads into LDS (none when the
+// global table is used).  The caller synchronizes before the first use.
+template <int L, int NT, bool TQ>
+__device__ __forceinline__ const double2 *stage_twiddles(const double2 *W,
+                                                         double2 *tw) {
+  constexpr int NTW = reg_twiddle_entries<L, TQ>();
+  if constexpr (NTW == 0) {
+    return W;
+  } else {
+    for (int i = threadIdx.x; i < NTW; i += NT) tw[i] = W[i];
+    return tw;
+  }
+}
+
+// Register-resident transform along a strided axis of the spectrum for
+// power-of-two L.  NSEQ = T * C complex sequences per block (T points of the
+// contiguous inner index, all C components), TPP = L / V threads each.
+
+// ------------------------------------------------------------ batches
+// Kernel-side view of a batch: tab == nullptr for a single launch (pointer
+// arguments by value), else nbe blocks per entry of the slot table.
+struct BatchRef {
+  const Slot *tab;
+  int64_t nbe;
+};
+
+// Index of the block within its entry (XCD-aware order over the whole
+// launch); *sl is the entry's slot in a batched launch, else nullptr.
+__device__ __forceinline__ int64_t batch_block(const BatchRef &br,
+                                               const Slot *&sl) {
+  int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  sl = nullptr;
+  if (br.tab) {
+    const int64_t e = blk / br.nbe;
+    blk -= e * br.nbe;
+    sl = br.tab + e;
+  }
+  return blk;
+}
+
+inline BatchRef batch_ref(const Batch *b, int64_t nbe) {
+  BatchRef r;
+  r.tab = b ? b->tab : nullptr;
+  r.nbe = nbe;
+  return r;
+}
+
+inline int64_t batch_count(const Batch *b) { return b ? b->n : 1; }
+
+// Fixed-point update arguments (the macroscopic strain is read from the
+// solve state).
+struct LsArgs {
+  const double *rhs;
+  double *partials;
+};
+
+// Twiddle source of the last-axis kernels: quarter table in LDS (1) or the
+// global table (0, fewer registers; measured as fast or faster).
+#ifndef JX_ROWS_TQ
+#define JX_ROWS_TQ 0
+#endif
+
+// ------------------------------------------------------------ launch helpers
+static inline int check_lds(size_t bytes, const char *what) {
+  if ((long long)bytes > lds_limit_bytes())
+    return fail(JX_EUNSUPPORTED,
+                std::string(what) + ": transform length too large for LDS (" +
+                    std::to_string(bytes) + " bytes needed)");
+  return JX_OK;
+}
+
+template <typename K>
+static inline void set_lds_attr(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute((const void *)kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+}
+
+// Pairs per block of the register last-axis kernels and sequences per block
+// of the register axis kernels: 192 threads for every length.
+template <int L>
+constexpr int reg_np() {
+  return 192 / reg_tpp<L>();
+}
+template <int L>
+constexpr int reg_nseq() {
+  return 192 / reg_tpp<L>();
+}
+
+// Rows per block of the register last-axis kernels (0 = not available).
+static inline int reg_rows_per_block(int L, int C) {
+  if (!reg_fft_length(L) || (C != 1 && C != 3 && C != 6)) return 0;
+  int np = 0;
+  switch (L) {
+    case 16: np = reg_np<16>(); break;
+    case 32: np = reg_np<32>(); break;
+    case 64: np = reg_np<64>(); break;
+    case 128: np = reg_np<128>(); break;
+    case 256: np = reg_np<256>(); break;
+    case 512: np = reg_np<512>(); break;
+    case 80: np = reg_np<80>(); break;
+    case 96: np = reg_np<96>(); break;
+    case 160: np = reg_np<160>(); break;
+  }
+  return 2 * np / C;
+}
+
+// Rows per block of the generic last-axis kernels and their LDS bytes.
+static inline int rows_per_block(int L, int C) {
+  int Ls = lds_stride(L);
+  // aim at about 1536 complex entries per ping-pong buffer
+  int nseq = std::max(1, 1536 / Ls);
+  int rb = std::max(1, (2 * nseq) / C);
+  return rb;
+}
+
+static inline size_t rows_lds_bytes(int L, int C, int RB) {
+  int nseq_max = (RB * C + 1) / 2;
+  return 2 * (size_t)nseq_max * lds_stride(L) * sizeof(double2);
+}
+
+#define JX_DISPATCH_LC(FN, ...)                                            \
+  switch (L) {                                                             \
+    case 16: JX_DISPATCH_C(16, FN, __VA_ARGS__);                           \
+    case 32: JX_DISPATCH_C(32, FN, __VA_ARGS__);                           \
+    case 64: JX_DISPATCH_C(64, FN, __VA_ARGS__);                           \
+    case 128: JX_DISPATCH_C(128, FN, __VA_ARGS__);                         \
+    case 256: JX_DISPATCH_C(256, FN, __VA_ARGS__);                         \
+    case 512: JX_DISPATCH_C(512, FN, __VA_ARGS__);                         \
+    case 80: JX_DISPATCH_C(80, FN, __VA_ARGS__);                           \
+    case 96: JX_DISPATCH_C(96, FN, __VA_ARGS__);                           \
+    case 160: JX_DISPATCH_C(160, FN, __VA_ARGS__);                         \
+    default: return fail(JX_EUNSUPPORTED, "no register FFT for length");   \
+  }
+#define JX_DISPATCH_C(LL, FN, ...)                                         \
+  switch (C) {                                                             \
+    case 1: return FN<LL, 1>(__VA_ARGS__);                                 \
+    case 3: return FN<LL, 3>(__VA_ARGS__);                                 \
+    case 6: return FN<LL, 6>(__VA_ARGS__);                                 \
+    default: return fail(JX_EINVAL, "ncomp must be 1, 3 or 6");            \
+  }
+
+}  // namespace jx

@@ -1,0 +1,54 @@
+// Handle structures behind the opaque pointers of include/janus_amd.h.
+#pragma once
+
+#include <vector>
+
+#include "jx_internal.h"
+
+struct jx_fft {
+  jx::Grid g;
+  double *scratch = nullptr;  // spectrum copy for c2r (input is preserved)
+};
+
+struct jx_green {
+  jx::Grid g;
+  int sym = 0;
+  double h = 1.0;
+  double g0 = 0, nu0 = 0;
+  jx::GreenDev dev;
+  double *tables = nullptr;   // device per-axis tables
+  std::vector<double> host_tables;
+  jx::GreenDev host;          // same tables, host pointers
+  double *spec = nullptr;     // spectrum scratch (Nh * 2 sym)
+};
+
+struct jx_local {
+  jx::Grid g;
+  int sym = 0;
+  jx::LocalDev dev;
+  double *table = nullptr;     // device (tr, dev) per phase
+  double *partials = nullptr;  // device, 1024 blocks x 2 sym columns + 2 sym
+};
+
+struct jx_solver {
+  jx_green *green = nullptr;
+  jx_local *local = nullptr;
+  bool fused = true;  // pass E of one iteration fused with pass A of the next
+  jx::SolveState *st = nullptr;       // device
+  jx::SolveState *st_host = nullptr;  // pinned host mirror
+  double *partials = nullptr;
+  int nblocks = 0;
+  double *res_dev = nullptr;
+  int res_cap = 0;
+};
+
+namespace jx {
+// Device tables, spectrum scratch and FFT plans of a Green operator, set up
+// on first use.
+int green_ensure_device(jx_green *G);
+// One fixed-point iteration of solver S on iterate eps, spectrum buffer
+// spec and solve state st (jx_api.hip).
+int solver_iteration(jx_solver *S, double *eps, double *spec, SolveState *st,
+                     const RowsInverseLs &ls, bool first, int *nb,
+                     hipStream_t s);
+}  // namespace jx

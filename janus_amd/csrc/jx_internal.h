@@ -70,6 +70,27 @@ struct SolveState {
   double res_scale;
   int max_iter;
   int pad;
+  double E[6];   // macroscopic strain (Mandel), read by the update pass
+};
+
+// One right-hand side of a batched solve.  A batched launch covers n
+// consecutive entries of a device table of slots, all on the same grid;
+// each block finds its entry from its index and takes every per-entry
+// pointer from the slot.
+struct Slot {
+  double *eps = nullptr;        // iterate, real field (rows, L, C)
+  double *spec = nullptr;       // spectrum (rows, nh, C, 2)
+  const double *rhs = nullptr;  // optional field added to E
+  double *partials = nullptr;   // residual partial sums, one per block
+  double *res_hist = nullptr;   // residual history (max_iter entries)
+  SolveState *st = nullptr;
+  LocalDev loc;
+  double daux[4] = {};          // Green operator constants of C0
+};
+
+struct Batch {
+  const Slot *tab = nullptr;  // device
+  int n = 0;
 };
 
 // Geometry of a grid with ncomp components per point.
@@ -92,41 +113,46 @@ int get_fft_len(int L, FftLen *out);
 int lds_limit_bytes();
 
 // ------------------------------------------------------------- launchers
+// Every solver pass takes an optional batch: when b != nullptr the launch
+// covers b->n right-hand sides and the pointer arguments (x, spec, y, loc,
+// st, ls pointers) are taken from the slots instead.  Batched launches need
+// register FFT lengths on every axis (batchable()).
+bool batchable(const Grid &g, int ncomp);
 // Forward transform of the last axis of a real field with optional local
 // operator, writing the half spectrum.
 int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
                         const LocalDev &loc, const SolveState *st,
-                        hipStream_t s);
+                        hipStream_t s, const Batch *b = nullptr);
 // Complex transform along axis `axis` (< dim - 1) of a spectrum; dir = -1
 // forward, +1 inverse, 0 = forward + Gamma-hat + inverse (axis must be 0).
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 double *out, const GreenDev *green, const SolveState *st,
-                hipStream_t s);
-// Register-FFT variant of launch_axis for power-of-two lengths 16 ... 512.
-int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
-                    const double *in, double *out, const GreenDev *green,
-                    const SolveState *st, const FftLen &P, hipStream_t s);
-// Inverse transform of the last axis (scaled by 1/N).  When E != nullptr
-// the fixed-point update eps <- E + rhs - y is applied in place on y and
-// per-block partial sums of (eps_new - eps_old)^2 go to partials.
+                hipStream_t s, const Batch *b = nullptr);
+// Inverse transform of the last axis (scaled by 1/N).  When enabled, the
+// fixed-point update eps <- E + rhs - y (E from the solve state) is applied
+// in place on y and per-block partial sums of (eps_new - eps_old)^2 go to
+// partials.
 struct RowsInverseLs {
   const double *rhs = nullptr;  // optional field
-  double E[6] = {};
   double *partials = nullptr;
   bool enabled = false;
 };
 int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
                         const RowsInverseLs &ls, const SolveState *st,
-                        int *nblocks_out, hipStream_t s);
+                        int *nblocks_out, hipStream_t s,
+                        const Batch *b = nullptr);
 int rows_inverse_nblocks(const Grid &g, int ncomp);
 // Fused pass E of iteration k + pass A of iteration k + 1 (register path).
 bool rows_fusable(const Grid &g, int ncomp);
 int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,
                                double *y, const RowsInverseLs &ls,
                                const LocalDev &loc, const SolveState *st,
-                               int *nblocks_out, hipStream_t s);
+                               int *nblocks_out, hipStream_t s,
+                               const Batch *b = nullptr);
+// Residual of the iteration from the partial sums; batched: one block per
+// entry.
 int launch_finalize(const double *partials, int nblocks, SolveState *st,
-                    double *res_hist, hipStream_t s);
+                    double *res_hist, hipStream_t s, const Batch *b = nullptr);
 int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
                        const double *x, double *y, hipStream_t s);
 int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,

@@ -1,0 +1,310 @@
+// Fused pass E + pass A of the Lippmann-Schwinger iteration (register FFT
+// lengths): the inverse last-axis transform and fixed-point update of
+// iteration k followed by the local operator and forward last-axis
+// transform of iteration k + 1.  Layouts as in pass A (jx_rows_fwd.hip).
+
+#include "jx_device.h"
+
+namespace jx {
+
+// Fused pass E of iteration k and pass A of iteration k + 1: inverse
+// last-axis transform, fixed-point update eps <- E (+ rhs) - y with the
+// residual partial sums, then (C - C0) : eps and the forward last-axis
+// transform of the result, written over the same spectrum rows.  The new
+// iterate never travels back from memory.  MODE 1: E only; 2: E + rhs.
+// Minimum waves per SIMD requested for the fused update pass: 3 (about 70
+// registers spilled to scratch) measured faster than 2 at 256^3.
+#ifndef JX_FUSED_MIN_WAVES
+#define JX_FUSED_MIN_WAVES 3
+#endif
+
+template <int L, int C, int NP, int MODE>
+__global__ void __launch_bounds__(NP * reg_tpp<L>(),
+                                  reg_v<L>() == 16 ? JX_FUSED_MIN_WAVES : 1)
+    k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
+                              double scale, const double2 *W, LsArgs ls,
+                              LocalDev loc, const SolveState *st,
+                              BatchRef br) {
+  const Slot *sl;
+  const int64_t blk = batch_block(br, sl);
+  if (sl) {
+    spec = sl->spec;
+    y = sl->eps;
+    ls.rhs = sl->rhs;
+    ls.partials = sl->partials;
+    loc = sl->loc;
+    st = sl->st;
+  }
+  if (solve_done(st)) return;
+  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<L>();
+  constexpr int NT = NP * TPP;
+  constexpr int RB = 2 * NP / C;
+  extern __shared__ double2 smem[];
+  __shared__ double red[4];
+  __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
+  const int tid = threadIdx.x;
+  const int64_t row0 = blk * RB;
+  const int rb = (int)min((int64_t)RB, nrows - row0);
+  const int p = tid % NP, t = tid / NP;
+  const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
+
+  // ---- inverse: half spectra into the pair slots (see k_rows_inverse_reg)
+  if constexpr (C == 6) {
+    for (int w = tid; w < RB * NH * 3; w += NT) {
+      int cp = w % 3;
+      int rest = w / 3;
+      int k = rest % NH;
+      int r = rest / NH;
+      double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
+      if (r < rb) {
+        const double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+        xa = *(const double2 *)sp;
+        xb = *(const double2 *)(sp + 2);
+      }
+      double2 *Z = smem + (r * 3 + cp) * LS;
+      if (k == 0 || 2 * k == L) {
+        Z[lp(k)] = make_double2(xa.x, xb.x);
+      } else {
+        Z[lp(k)] = xa;
+        Z[lp(L - k)] = xb;
+      }
+    }
+  } else {
+    for (int w = tid; w < RB * NH * C; w += NT) {
+      int c = w % C;
+      int rest = w / C;
+      int k = rest % NH;
+      int r = rest / NH;
+      int q = r * C + c;
+      int part = q & 1;
+      double re = 0., im = 0.;
+      if (r < rb) {
+        int64_t pt = (row0 + r) * NH + k;
+        re = spec[pt * 2 * C + 2 * c];
+        im = spec[pt * 2 * C + 2 * c + 1];
+      }
+      double2 *Z = smem + (q >> 1) * LS;
+      if (k == 0 || 2 * k == L)
+        ((double *)&Z[lp(k)])[part] = re;
+      else if (!part)
+        Z[lp(k)] = make_double2(re, im);
+      else
+        Z[lp(L - k)] = make_double2(re, im);
+    }
+  }
+  __syncthreads();
+  constexpr int HALF = L / 2 - 1;
+  if (HALF > 0) {
+    for (int w = tid; w < NP * HALF; w += NT) {
+      int pp = w / HALF;
+      int k = 1 + (w - pp * HALF);
+      double2 *Z = smem + pp * LS;
+      double2 A = Z[lp(k)], B = Z[lp(L - k)];
+      Z[lp(k)] = make_double2(A.x - B.y, A.y + B.x);
+      Z[lp(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();
+  }
+  double2 *seq = smem + p * LS;
+  double2 v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  __syncthreads();
+  reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
+
+  // ---- update; v becomes the new iterate (part 0 in .x, part 1 in .y)
+  double acc = 0.;
+  if constexpr (C == 6) {
+    const int r = p / 3, c0 = 2 * (p - r * 3);
+    if (r < rb) {
+      double2 *yr = (double2 *)(y + (row0 + r) * L * 6 + c0);
+      const double2 *hr =
+          MODE == 2 ? (const double2 *)(ls.rhs + (row0 + r) * L * 6 + c0)
+                    : nullptr;
+      const double e0 = st->E[c0], e1 = st->E[c0 + 1];
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
+        double2 nv = make_double2(ls_new(e0, v[i].x, scale),
+                                  ls_new(e1, v[i].y, scale));
+        if constexpr (MODE == 2) {
+          double2 h = hr[off];
+          nv = make_double2(ls_new_rhs(e0, h.x, v[i].x, scale),
+                            ls_new_rhs(e1, h.y, v[i].y, scale));
+        }
+        double2 o = yr[off];
+        double dx = nv.x - o.x, dy = nv.y - o.y;
+        acc += dx * dx;
+        acc += dy * dy;
+        yr[off] = nv;
+        v[i] = nv;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = make_double2(0., 0.);
+    }
+  } else {
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const int q = 2 * p + part;
+      const int r = q / C, c = q - (q / C) * C;
+      double *yr = y + (row0 + r) * L * C + c;
+      const double *hr = MODE == 2 ? ls.rhs + (row0 + r) * L * C + c : nullptr;
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        double nv = 0.;
+        if (r < rb) {
+          int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
+          double raw = part ? v[i].y : v[i].x;
+          nv = ls_new(st->E[c], raw, scale);
+          if constexpr (MODE == 2) nv = ls_new_rhs(st->E[c], hr[off], raw, scale);
+          double d = nv - yr[off];
+          acc += d * d;
+          yr[off] = nv;
+        }
+        if (part) v[i].y = nv; else v[i].x = nv;
+      }
+    }
+  }
+  double sum = block_sum(acc, red);
+  if (tid == 0) ls.partials[blk] = sum;
+
+  // ---- the new iterate into LDS, local operator voxel by voxel
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  __syncthreads();
+  for (int w = tid; w < rb * L; w += NT) {
+    int r = w / L, j = w - r * L;
+    int64_t vox = (row0 + r) * L + j;
+    double e[C], tt[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      e[c] = ((const double *)&smem[(q >> 1) * LS + lp(j)])[q & 1];
+    }
+    local_voxel<C>(loc, vox, e, tt);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      ((double *)&smem[(q >> 1) * LS + lp(j)])[q & 1] = tt[c];
+    }
+  }
+  __syncthreads();
+
+  // ---- forward transform of the polarization rows (see k_rows_forward_reg)
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  __syncthreads();
+  reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  __syncthreads();
+  if constexpr (C == 6) {
+    for (int w = tid; w < rb * NH * 3; w += NT) {
+      int cp = w % 3;
+      int rest = w / 3;
+      int k = rest % NH;
+      int r = rest / NH;
+      const double2 *Z = smem + (r * 3 + cp) * LS;
+      double2 zk = Z[lp(k)];
+      double2 zm = Z[lp(k == 0 ? 0 : L - k)];
+      double2 xa = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+      double2 xb = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+      double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+      *(double2 *)sp = xa;
+      *(double2 *)(sp + 2) = xb;
+    }
+  } else {
+    for (int w = tid; w < rb * NH * C; w += NT) {
+      int c = w % C;
+      int rest = w / C;
+      int k = rest % NH;
+      int r = rest / NH;
+      int q = r * C + c;
+      const double2 *Z = smem + (q >> 1) * LS;
+      double2 zk = Z[lp(k)];
+      double2 zm = Z[lp(k == 0 ? 0 : L - k)];
+      double re, im;
+      if (!(q & 1)) {
+        re = 0.5 * (zk.x + zm.x);
+        im = 0.5 * (zk.y - zm.y);
+      } else {
+        re = 0.5 * (zk.y + zm.y);
+        im = -0.5 * (zk.x - zm.x);
+      }
+      int64_t pt = (row0 + r) * NH + k;
+      spec[pt * 2 * C + 2 * c] = re;
+      spec[pt * 2 * C + 2 * c + 1] = im;
+    }
+  }
+}
+// ============================================================ launchers
+
+template <int L, int C>
+static int launch_rows_update_forward_reg_lc(const Grid &g, double *spec,
+                                             double *y, const LsArgs &la,
+                                             const LocalDev &loc,
+                                             const SolveState *st,
+                                             const double2 *W, hipStream_t s,
+                                             const Batch *b) {
+  if constexpr (C == 3 || C == 6) {
+    constexpr int NP = reg_np<L>();
+    constexpr int RB = 2 * NP / C;
+    size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
+    int rc = check_lds(lds, "rows_update_forward");
+    if (rc) return rc;
+    const int64_t nbe = (g.rows + RB - 1) / RB;
+    const BatchRef br = batch_ref(b, nbe);
+    dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<L>());
+    double scale = 1.0 / (double)g.N;
+    if (la.rhs) {
+      set_lds_attr(k_rows_update_forward_reg<L, C, NP, 2>, lds);
+      hipLaunchKernelGGL((k_rows_update_forward_reg<L, C, NP, 2>), grid, block,
+                         lds, s, spec, y, g.rows, scale, W, la, loc, st, br);
+    } else {
+      set_lds_attr(k_rows_update_forward_reg<L, C, NP, 1>, lds);
+      hipLaunchKernelGGL((k_rows_update_forward_reg<L, C, NP, 1>), grid, block,
+                         lds, s, spec, y, g.rows, scale, W, la, loc, st, br);
+    }
+    JX_HIP(hipGetLastError());
+    return JX_OK;
+  } else {
+    return fail(JX_EINVAL, "fixed point needs sym components");
+  }
+}
+
+static int rows_update_forward_reg(const Grid &g, int C, double *spec,
+                                   double *y, const LsArgs &la,
+                                   const LocalDev &loc, const SolveState *st,
+                                   const double2 *W, hipStream_t s,
+                                   const Batch *b) {
+  const int L = (int)g.L;
+  JX_DISPATCH_LC(launch_rows_update_forward_reg_lc, g, spec, y, la, loc, st, W, s, b)
+}
+
+bool rows_fusable(const Grid &g, int ncomp) {
+  return (ncomp == 3 || ncomp == 6) && reg_rows_per_block((int)g.L, ncomp) > 0;
+}
+
+int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,
+                               double *y, const RowsInverseLs &ls,
+                               const LocalDev &loc, const SolveState *st,
+                               int *nblocks_out, hipStream_t s,
+                               const Batch *b) {
+  if (!rows_fusable(g, ncomp))
+    return fail(JX_EUNSUPPORTED, "fused update pass needs a register FFT length");
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  LsArgs la;
+  la.rhs = ls.rhs;
+  la.partials = ls.partials;
+  int RB = reg_rows_per_block((int)g.L, ncomp);
+  if (nblocks_out) *nblocks_out = (int)((g.rows + RB - 1) / RB);
+  return rows_update_forward_reg(g, ncomp, spec, y, la, loc, st, P.W, s, b);
+}
+
+}  // namespace jx

@@ -1,0 +1,281 @@
+// Pass A of the Lippmann-Schwinger iteration: forward real transform of the
+// last axis, with the local operator (C - C0) applied on the way in.
+//
+// Data layouts (FP64, C order):
+//   real field  (rows, L, C)        rows = n0 [* n1], L = n_last, C comps
+//   spectrum    (rows, nh, C, 2)    nh = L / 2 + 1; complex values, the C
+//                                   components of a frequency side by side
+//                                   (for C = 1: janus's r2c layout,
+//                                   _serial_fft.pyx:57-74, 181-200)
+// Real sequences are paired two by two as the real and imaginary parts of
+// one complex sequence, which is then transformed once.
+
+#include "jx_device.h"
+
+namespace jx {
+
+// ------------------------------------------------------------ pass A
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_rows_forward(const double *__restrict__ x, double *__restrict__ spec,
+                   RowsArgs a, FftLen P, LocalDev loc, const SolveState *st) {
+  if (solve_done(st)) return;
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t row0 = (int64_t)blockIdx.x * a.RB;
+  const int rb = (int)min((int64_t)a.RB, a.nrows - row0);
+  const int Q = rb * C;
+  const int nseq = (Q + 1) / 2;
+  const int nseq_max = (a.RB * C + 1) / 2;
+  double2 *bufA = smem;
+  double2 *bufB = smem + (size_t)nseq_max * a.Ls;
+  const int L = a.L;
+
+  for (int v = tid; v < rb * L; v += nth) {
+    int r = v / L, j = v - r * L;
+    int64_t vox = (row0 + r) * L + j;
+    double e[C], t[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
+    if (loc.mode != LOCAL_NONE) {
+      local_voxel<C>(loc, vox, e, t);
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = e[c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      double *slot = (double *)&bufA[(q >> 1) * a.Ls + j];
+      slot[q & 1] = t[c];
+    }
+    if ((Q & 1) && r == rb - 1) bufA[((Q - 1) >> 1) * a.Ls + j].y = 0.;
+  }
+  __syncthreads();
+  double2 *res = lds_fft<-1>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+
+  const int nh = a.nh;
+  for (int v = tid; v < rb * nh * C; v += nth) {
+    int c = v % C;
+    int rest = v / C;
+    int k = rest % nh;
+    int r = rest / nh;
+    int q = r * C + c;
+    const double2 *Z = res + (q >> 1) * a.Ls;
+    double2 zk = Z[k];
+    double2 zm = Z[k == 0 ? 0 : L - k];
+    double re, im;
+    if (!(q & 1)) {
+      re = 0.5 * (zk.x + zm.x);
+      im = 0.5 * (zk.y - zm.y);
+    } else {
+      re = 0.5 * (zk.y + zm.y);
+      im = -0.5 * (zk.x - zm.x);
+    }
+    int64_t pt = (row0 + r) * nh + k;
+    spec[pt * 2 * C + 2 * c] = re;
+    spec[pt * 2 * C + 2 * c + 1] = im;
+  }
+}
+
+
+// Register-resident last-axis transform for the register FFT lengths.  A
+// block handles RB rows, i.e. 2 NP real sequences paired into NP complex
+// ones (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / V
+// threads per pair.  Batched: the entry's iterate, spectrum, local operator
+// and state come from its slot.
+template <int L, int C, int NP, bool LOCAL>
+__global__ void __launch_bounds__(NP * reg_tpp<L>())
+    k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
+                       int64_t nrows, const double2 *W, LocalDev loc,
+                       const SolveState *st, BatchRef br) {
+  const Slot *sl;
+  const int64_t blk = batch_block(br, sl);
+  if (sl) {
+    x = sl->eps;
+    spec = sl->spec;
+    loc = sl->loc;
+    st = sl->st;
+  }
+  if (solve_done(st)) return;
+  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<L>();
+  constexpr int NT = NP * TPP;
+  constexpr int RB = 2 * NP / C;
+  static_assert((2 * NP) % C == 0, "whole rows per block");
+  extern __shared__ double2 smem[];
+  const int tid = threadIdx.x;
+  const int64_t row0 = blk * RB;
+  const int rb = (int)min((int64_t)RB, nrows - row0);
+  __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
+  const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
+  // stage the rows (local operator applied) into the pair sequences
+  for (int w = tid; w < RB * L; w += NT) {
+    int r = w / L, j = w - r * L;
+    double t[C];
+    if (r < rb) {
+      int64_t vox = (row0 + r) * L + j;
+      double e[C];
+      if constexpr (C == 6) {
+        const double2 *xv = (const double2 *)(x + vox * 6);
+        double2 a = xv[0], b = xv[1], cc = xv[2];
+        e[0] = a.x; e[1] = a.y; e[2] = b.x; e[3] = b.y; e[4] = cc.x; e[5] = cc.y;
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
+      }
+      if (LOCAL) {
+        local_voxel<C>(loc, vox, e, t);
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) t[c] = e[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) t[c] = 0.;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      ((double *)&smem[(q >> 1) * LS + lp(j)])[q & 1] = t[c];
+    }
+  }
+  __syncthreads();
+  const int p = tid % NP, t = tid / NP;
+  double2 *seq = smem + p * LS;
+  double2 v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  __syncthreads();
+  reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  __syncthreads();
+  // separate the pairs: Xa = (Z[k] + conj Z[-k]) / 2, Xb = (Z[k] - conj Z[-k]) / 2i
+  if constexpr (C == 6) {
+    // pair p of row r holds components (c0, c0 + 1), adjacent in memory:
+    // two 16-byte stores of complex values
+    for (int w = tid; w < rb * NH * 3; w += NT) {
+      int cp = w % 3;
+      int rest = w / 3;
+      int k = rest % NH;
+      int r = rest / NH;
+      const double2 *Z = smem + (r * 3 + cp) * LS;
+      double2 zk = Z[lp(k)];
+      double2 zm = Z[lp(k == 0 ? 0 : L - k)];
+      double2 xa = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+      double2 xb = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+      double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+      *(double2 *)sp = xa;
+      *(double2 *)(sp + 2) = xb;
+    }
+    return;
+  }
+  for (int w = tid; w < rb * NH * C; w += NT) {
+    int c = w % C;
+    int rest = w / C;
+    int k = rest % NH;
+    int r = rest / NH;
+    int q = r * C + c;
+    const double2 *Z = smem + (q >> 1) * LS;
+    double2 zk = Z[lp(k)];
+    double2 zm = Z[lp(k == 0 ? 0 : L - k)];
+    double re, im;
+    if (!(q & 1)) {
+      re = 0.5 * (zk.x + zm.x);
+      im = 0.5 * (zk.y - zm.y);
+    } else {
+      re = 0.5 * (zk.y + zm.y);
+      im = -0.5 * (zk.x - zm.x);
+    }
+    int64_t pt = (row0 + r) * NH + k;
+    spec[pt * 2 * C + 2 * c] = re;
+    spec[pt * 2 * C + 2 * c + 1] = im;
+  }
+}
+
+// ============================================================ launchers
+
+template <int L, int C>
+static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
+                                      double *spec, const LocalDev &loc,
+                                      const SolveState *st, const double2 *W,
+                                      hipStream_t s, const Batch *b) {
+  constexpr int NP = reg_np<L>();
+  constexpr int RB = 2 * NP / C;
+  size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
+  int rc = check_lds(lds, "rows_forward");
+  if (rc) return rc;
+  const int64_t nbe = (g.rows + RB - 1) / RB;
+  const BatchRef br = batch_ref(b, nbe);
+  dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<L>());
+  // a batch always applies the local operator of its slots
+  if (b || loc.mode != LOCAL_NONE) {
+    if constexpr (C == 3 || C == 6) {
+      set_lds_attr(k_rows_forward_reg<L, C, NP, true>, lds);
+      hipLaunchKernelGGL((k_rows_forward_reg<L, C, NP, true>), grid, block, lds,
+                         s, x, spec, g.rows, W, loc, st, br);
+    } else {
+      return fail(JX_EINVAL, "local operator needs sym components");
+    }
+  } else {
+    set_lds_attr(k_rows_forward_reg<L, C, NP, false>, lds);
+    hipLaunchKernelGGL((k_rows_forward_reg<L, C, NP, false>), grid, block, lds,
+                       s, x, spec, g.rows, W, loc, st, br);
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+static int rows_forward_reg(const Grid &g, int C, const double *x, double *spec,
+                            const LocalDev &loc, const SolveState *st,
+                            const double2 *W, hipStream_t s, const Batch *b) {
+  const int L = (int)g.L;
+  JX_DISPATCH_LC(launch_rows_forward_reg_lc, g, x, spec, loc, st, W, s, b)
+}
+
+int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
+                        const LocalDev &loc, const SolveState *st,
+                        hipStream_t s, const Batch *b) {
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  if (reg_rows_per_block((int)g.L, ncomp))
+    return rows_forward_reg(g, ncomp, x, spec, loc, st, P.W, s, b);
+  if (b) return fail(JX_EUNSUPPORTED, "rows_forward: batch needs a register FFT length");
+  RowsArgs a;
+  a.nrows = g.rows;
+  a.L = (int)g.L;
+  a.nh = (int)g.nh;
+  a.Ls = lds_stride(a.L);
+  a.RB = rows_per_block(a.L, ncomp);
+  a.scale = 1.0;
+  size_t lds = rows_lds_bytes(a.L, ncomp, a.RB);
+  if ((rc = check_lds(lds, "rows_forward"))) return rc;
+  int64_t nb = (g.rows + a.RB - 1) / a.RB;
+  dim3 grid((unsigned)nb), block(kThreads);
+  switch (ncomp) {
+    case 1:
+      set_lds_attr(k_rows_forward<1>, lds);
+      hipLaunchKernelGGL(k_rows_forward<1>, grid, block, lds, s, x, spec, a, P,
+                         loc, st);
+      break;
+    case 3:
+      set_lds_attr(k_rows_forward<3>, lds);
+      hipLaunchKernelGGL(k_rows_forward<3>, grid, block, lds, s, x, spec, a, P,
+                         loc, st);
+      break;
+    case 6:
+      set_lds_attr(k_rows_forward<6>, lds);
+      hipLaunchKernelGGL(k_rows_forward<6>, grid, block, lds, s, x, spec, a, P,
+                         loc, st);
+      break;
+    default:
+      return fail(JX_EINVAL, "rows_forward: ncomp must be 1, 3 or 6");
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+}  // namespace jx

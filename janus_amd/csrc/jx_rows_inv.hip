@@ -1,0 +1,359 @@
+// Pass E of the Lippmann-Schwinger iteration: inverse real transform of the
+// last axis (scaled by 1/N), optionally with the fixed-point update
+// eps <- E (+ rhs) - y and the residual partial sums.  Layouts as in pass A
+// (jx_rows_fwd.hip).
+
+#include "jx_device.h"
+
+namespace jx {
+
+// ------------------------------------------------------------ pass E
+
+template <int C, bool LS>
+__global__ void __launch_bounds__(kThreads)
+    k_rows_inverse(const double *__restrict__ spec, double *y, RowsArgs a,
+                   FftLen P, LsArgs ls, const SolveState *st) {
+  if (solve_done(st)) return;
+  extern __shared__ double2 smem[];
+  __shared__ double red[kThreads / 64];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t row0 = (int64_t)blockIdx.x * a.RB;
+  const int rb = (int)min((int64_t)a.RB, a.nrows - row0);
+  const int Q = rb * C;
+  const int nseq = (Q + 1) / 2;
+  const int nseq_max = (a.RB * C + 1) / 2;
+  double2 *bufA = smem;
+  double2 *bufB = smem + (size_t)nseq_max * a.Ls;
+  const int L = a.L, nh = a.nh;
+
+  // Load the half spectra: pair p = (Xa, Xb).  Slot k holds Xa[k] and slot
+  // L - k holds Xb[k] (0 < k < L/2); the DC and Nyquist slots hold
+  // (Re Xa, Re Xb) -- their imaginary parts are dropped, as in FFTW's and
+  // pocketfft's c2r.
+  for (int v = tid; v < rb * nh * C; v += nth) {
+    int c = v % C;
+    int rest = v / C;
+    int k = rest % nh;
+    int r = rest / nh;
+    int q = r * C + c;
+    int part = q & 1;
+    int64_t pt = (row0 + r) * nh + k;
+    double re = spec[pt * 2 * C + 2 * c];
+    double im = spec[pt * 2 * C + 2 * c + 1];
+    double2 *Z = bufA + (q >> 1) * a.Ls;
+    bool lonely = (!part) && (q + 1 == Q);
+    if (k == 0 || 2 * k == L) {
+      ((double *)&Z[k])[part] = re;
+      if (lonely) Z[k].y = 0.;
+    } else if (!part) {
+      Z[k] = make_double2(re, im);
+      if (lonely) Z[L - k] = make_double2(0., 0.);
+    } else {
+      Z[L - k] = make_double2(re, im);
+    }
+  }
+  __syncthreads();
+  // Z[k] = Xa[k] + i Xb[k], Z[L-k] = conj(Xa[k]) + i conj(Xb[k])
+  const int half = (L - 1) / 2;
+  for (int v = tid; v < nseq * half; v += nth) {
+    int p = v / half;
+    int k = 1 + (v - p * half);
+    double2 *Z = bufA + p * a.Ls;
+    double2 A = Z[k], B = Z[L - k];
+    Z[k] = make_double2(A.x - B.y, A.y + B.x);
+    Z[L - k] = make_double2(A.x + B.y, B.x - A.y);
+  }
+  __syncthreads();
+  double2 *res = lds_fft<+1>(bufA, bufB, nseq, P, a.Ls, tid, nth);
+
+  double acc = 0.;
+  for (int v = tid; v < rb * L; v += nth) {
+    int r = v / L, j = v - r * L;
+    int64_t vox = (row0 + r) * L + j;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      int q = r * C + c;
+      double2 z = res[(q >> 1) * a.Ls + j];
+      double raw = (q & 1) ? z.y : z.x;
+      double val = a.scale * raw;
+      if (LS) {
+        double nv = ls.rhs ? ls_new_rhs(st->E[c], ls.rhs[vox * C + c], raw, a.scale)
+                           : ls_new(st->E[c], raw, a.scale);
+        double d = nv - y[vox * C + c];
+        acc += d * d;
+        y[vox * C + c] = nv;
+      } else {
+        y[vox * C + c] = val;
+      }
+    }
+  }
+  if (LS) {
+    double s = block_sum(acc, red);
+    if (tid == 0) ls.partials[blockIdx.x] = s;
+  }
+}
+
+// Register-FFT variant for the register lengths.  MODE 0: y = c2r(spec);
+// 1: y <- E - c2r(spec) with residual partials; 2: y <- (E + rhs) -
+// c2r(spec) with residual partials.  Batched: the entry's spectrum,
+// iterate, rhs, partials and state come from its slot.
+template <int L, int C, int NP, int MODE>
+__global__ void __launch_bounds__(NP * reg_tpp<L>())
+    k_rows_inverse_reg(const double *__restrict__ spec, double *y,
+                       int64_t nrows, double scale, const double2 *W,
+                       LsArgs ls, const SolveState *st, BatchRef br) {
+  const Slot *sl;
+  const int64_t blk = batch_block(br, sl);
+  if (sl) {
+    spec = sl->spec;
+    y = sl->eps;
+    ls.rhs = sl->rhs;
+    ls.partials = sl->partials;
+    st = sl->st;
+  }
+  if (solve_done(st)) return;
+  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<L>();
+  constexpr int NT = NP * TPP;
+  constexpr int RB = 2 * NP / C;
+  constexpr bool LS_UPDATE = MODE > 0;
+  extern __shared__ double2 smem[];
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  const int64_t row0 = blk * RB;
+  const int rb = (int)min((int64_t)RB, nrows - row0);
+  const int p = tid % NP, t = tid / NP;
+  __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
+  const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
+  // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
+  // (0 < k < L/2); DC and Nyquist slots <- (Re Xa, Re Xb), their imaginary
+  // parts dropped as FFTW's and pocketfft's c2r do
+  if constexpr (C == 6) {
+    for (int w = tid; w < RB * NH * 3; w += NT) {
+      int cp = w % 3;
+      int rest = w / 3;
+      int k = rest % NH;
+      int r = rest / NH;
+      double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
+      if (r < rb) {
+        const double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+        xa = *(const double2 *)sp;
+        xb = *(const double2 *)(sp + 2);
+      }
+      double2 *Z = smem + (r * 3 + cp) * LS;
+      if (k == 0 || 2 * k == L) {
+        Z[lp(k)] = make_double2(xa.x, xb.x);
+      } else {
+        Z[lp(k)] = xa;
+        Z[lp(L - k)] = xb;
+      }
+    }
+  } else
+  for (int w = tid; w < RB * NH * C; w += NT) {
+    int c = w % C;
+    int rest = w / C;
+    int k = rest % NH;
+    int r = rest / NH;
+    int q = r * C + c;
+    int part = q & 1;
+    double re = 0., im = 0.;
+    if (r < rb) {
+      int64_t pt = (row0 + r) * NH + k;
+      re = spec[pt * 2 * C + 2 * c];
+      im = spec[pt * 2 * C + 2 * c + 1];
+    }
+    double2 *Z = smem + (q >> 1) * LS;
+    if (k == 0 || 2 * k == L)
+      ((double *)&Z[lp(k)])[part] = re;
+    else if (!part)
+      Z[lp(k)] = make_double2(re, im);
+    else
+      Z[lp(L - k)] = make_double2(re, im);
+  }
+  __syncthreads();
+  constexpr int HALF = L / 2 - 1;
+  if (HALF > 0) {
+    for (int w = tid; w < NP * HALF; w += NT) {
+      int pp = w / HALF;
+      int k = 1 + (w - pp * HALF);
+      double2 *Z = smem + pp * LS;
+      double2 A = Z[lp(k)], B = Z[lp(L - k)];
+      Z[lp(k)] = make_double2(A.x - B.y, A.y + B.x);
+      Z[lp(L - k)] = make_double2(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();
+  }
+  double2 *seq = smem + p * LS;
+  double2 v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  __syncthreads();
+  reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
+  double acc = 0.;
+  if constexpr (C == 6) {
+    // pair p = components (c0, c0 + 1) of row r: 16-byte accesses
+    const int r = p / 3, c0 = 2 * (p - r * 3);
+    if (r < rb) {
+      double2 *yr = (double2 *)(y + (row0 + r) * L * 6 + c0);
+      const double2 *hr =
+          MODE == 2 ? (const double2 *)(ls.rhs + (row0 + r) * L * 6 + c0)
+                    : nullptr;
+      const double e0 = LS_UPDATE ? st->E[c0] : 0.;
+      const double e1 = LS_UPDATE ? st->E[c0 + 1] : 0.;
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
+        double2 val = make_double2(scale * v[i].x, scale * v[i].y);
+        if constexpr (LS_UPDATE) {
+          double2 nv = make_double2(ls_new(e0, v[i].x, scale),
+                                    ls_new(e1, v[i].y, scale));
+          if constexpr (MODE == 2) {
+            double2 h = hr[off];
+            nv = make_double2(ls_new_rhs(e0, h.x, v[i].x, scale),
+                              ls_new_rhs(e1, h.y, v[i].y, scale));
+          }
+          double2 o = yr[off];
+          double dx = nv.x - o.x, dy = nv.y - o.y;
+          acc += dx * dx;
+          acc += dy * dy;
+          yr[off] = nv;
+        } else {
+          yr[off] = val;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const int q = 2 * p + part;
+      const int r = q / C, c = q - (q / C) * C;
+      if (r < rb) {
+        double *yr = y + (row0 + r) * L * C + c;
+        const double *hr = MODE == 2 ? ls.rhs + (row0 + r) * L * C + c : nullptr;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
+          double raw = part ? v[i].y : v[i].x;
+          double val = scale * raw;
+          if constexpr (LS_UPDATE) {
+            double nv = ls_new(st->E[c], raw, scale);
+            if constexpr (MODE == 2) nv = ls_new_rhs(st->E[c], hr[off], raw, scale);
+            double d = nv - yr[off];
+            acc += d * d;
+            yr[off] = nv;
+          } else {
+            yr[off] = val;
+          }
+        }
+      }
+    }
+  }
+  if (LS_UPDATE) {
+    double sum = block_sum(acc, red);
+    if (tid == 0) ls.partials[blk] = sum;
+  }
+}
+
+// ============================================================ launchers
+
+template <int L, int C>
+static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
+                                      double *y, const LsArgs &la, bool upd,
+                                      const SolveState *st, const double2 *W,
+                                      hipStream_t s, const Batch *b) {
+  constexpr int NP = reg_np<L>();
+  constexpr int RB = 2 * NP / C;
+  size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
+  int rc = check_lds(lds, "rows_inverse");
+  if (rc) return rc;
+  const int64_t nbe = (g.rows + RB - 1) / RB;
+  const BatchRef br = batch_ref(b, nbe);
+  dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<L>());
+  double scale = 1.0 / (double)g.N;
+  if (upd) {
+    if constexpr (C == 3 || C == 6) {
+      if (la.rhs) {
+        set_lds_attr(k_rows_inverse_reg<L, C, NP, 2>, lds);
+        hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 2>), grid, block,
+                           lds, s, spec, y, g.rows, scale, W, la, st, br);
+      } else {
+        set_lds_attr(k_rows_inverse_reg<L, C, NP, 1>, lds);
+        hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 1>), grid, block,
+                           lds, s, spec, y, g.rows, scale, W, la, st, br);
+      }
+    } else {
+      return fail(JX_EINVAL, "fixed point needs sym components");
+    }
+  } else {
+    set_lds_attr(k_rows_inverse_reg<L, C, NP, 0>, lds);
+    hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 0>), grid, block, lds,
+                       s, spec, y, g.rows, scale, W, la, st, br);
+  }
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+static int rows_inverse_reg(const Grid &g, int C, const double *spec, double *y,
+                            const LsArgs &la, bool upd, const SolveState *st,
+                            const double2 *W, hipStream_t s, const Batch *b) {
+  const int L = (int)g.L;
+  JX_DISPATCH_LC(launch_rows_inverse_reg_lc, g, spec, y, la, upd, st, W, s, b)
+}
+
+int rows_inverse_nblocks(const Grid &g, int ncomp) {
+  int RB = reg_rows_per_block((int)g.L, ncomp);
+  if (!RB) RB = rows_per_block((int)g.L, ncomp);
+  return (int)((g.rows + RB - 1) / RB);
+}
+
+int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
+                        const RowsInverseLs &ls, const SolveState *st,
+                        int *nblocks_out, hipStream_t s, const Batch *b) {
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  if (int RBr = reg_rows_per_block((int)g.L, ncomp)) {
+    LsArgs la;
+    la.rhs = ls.rhs;
+    la.partials = ls.partials;
+    if (nblocks_out) *nblocks_out = (int)((g.rows + RBr - 1) / RBr);
+    return rows_inverse_reg(g, ncomp, spec, y, la, ls.enabled, st, P.W, s, b);
+  }
+  if (b) return fail(JX_EUNSUPPORTED, "rows_inverse: batch needs a register FFT length");
+  RowsArgs a;
+  a.nrows = g.rows;
+  a.L = (int)g.L;
+  a.nh = (int)g.nh;
+  a.Ls = lds_stride(a.L);
+  a.RB = rows_per_block(a.L, ncomp);
+  a.scale = 1.0 / (double)g.N;
+  size_t lds = rows_lds_bytes(a.L, ncomp, a.RB);
+  if ((rc = check_lds(lds, "rows_inverse"))) return rc;
+  int64_t nb = (g.rows + a.RB - 1) / a.RB;
+  if (nblocks_out) *nblocks_out = (int)nb;
+  LsArgs la;
+  la.rhs = ls.rhs;
+  la.partials = ls.partials;
+  dim3 grid((unsigned)nb), block(kThreads);
+#define JX_RI(CC, LSV)                                                        \
+  do {                                                                        \
+    set_lds_attr(k_rows_inverse<CC, LSV>, lds);                               \
+    hipLaunchKernelGGL((k_rows_inverse<CC, LSV>), grid, block, lds, s, spec, \
+                       y, a, P, la, st);                                      \
+  } while (0)
+  if (ls.enabled) {
+    if (ncomp == 3) JX_RI(3, true);
+    else if (ncomp == 6) JX_RI(6, true);
+    else return fail(JX_EINVAL, "rows_inverse: fixed point needs sym comps");
+  } else {
+    if (ncomp == 1) JX_RI(1, false);
+    else if (ncomp == 3) JX_RI(3, false);
+    else if (ncomp == 6) JX_RI(6, false);
+    else return fail(JX_EINVAL, "rows_inverse: ncomp must be 1, 3 or 6");
+  }
+#undef JX_RI
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+}  // namespace jx
