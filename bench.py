@@ -14,6 +14,16 @@ over the whole grid, one load case):
 Inputs are resident in HBM before the timed region.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Other workloads (one JSON line each, --workload NAME):
+  plain2d    2D 256^2 square inclusion, truncated, C0 = matrix: the stated
+             configuration diverges; reported with the midpoint C0 run
+  stiffness  128^3 spheres r=8, contrast 50: full 6x6 C_eff at 1e-10
+  batched    32 microstructures on grids {64,80,96,128,160}^3 x 6 loads:
+             one batch vs 192 sequential solves at 1e-10
+  adjoint    128^3 SIMP density: adjoint gradient time / forward solve time
+  large      512^3 three-phase ellipsoids: 6-load batch iterations/s and
+             an adjoint gradient with respect to the per-phase mu
 """
 
 import argparse
@@ -213,15 +223,579 @@ def traffic_summary(fetch_csv, write_csv, cmd):
     return '\n'.join(out) + '\n'
 
 
+# ---------------------------------------------------------------- workloads
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _log(msg):
+    print('bench: ' + msg, file=sys.stderr, flush=True)
+
+
+def _two_phase_solver(phase, mat_m, mat_i, mat0, kind):
+    import janus_amd.fft.serial as fft
+    from janus_amd.green import filtered, truncated
+    from janus_amd.operators import BlockDiagonalOperator2D, \
+        BlockDiagonalOperator3D
+    from janus_amd.solver import LSSolver, delta_stiffness
+    dim = phase.ndim
+    table = np.array([delta_stiffness(m, mat0) for m in (mat_m, mat_i)])
+    cls = BlockDiagonalOperator2D if dim == 2 else BlockDiagonalOperator3D
+    factory = filtered if kind == 'filtered' else truncated
+    green = factory(mat0.green_operator(), phase.shape, 1.0,
+                    fft.create_real(phase.shape))
+    return LSSolver(green, cls.from_phases(phase, table)), table
+
+
+def _unit(sym, j):
+    E = np.zeros(sym)
+    E[j] = 1.0
+    return E
+
+
+def _line(metric, value, unit, config, higher=True, **extra):
+    d = {'metric': metric, 'value': value, 'unit': unit, 'n_gpus': 1,
+         'higher_is_better': higher, 'dtype': 'f64', 'data': 'synthetic',
+         'config': config}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def workload_plain2d(args):
+    """2D 256^2, centred square inclusion (vf 0.25), matrix mu 1 / inclusion
+    mu 10 (nu 0.3), truncated Gamma0, E = (0, 0, 1), 1e-10 or 200
+    iterations.  With C0 = matrix the basic scheme diverges (the iteration
+    operator has eigenvalues up to about mu_i / mu_0 - 1 = 9); the GPU
+    iterates are checked against the oracle's over the first iterations, then
+    the same problem runs with the midpoint C0."""
+    import janus_amd.material.elastic.linear.isotropic as material
+    from janus_amd import microstructure as ms
+    from oracle import janus_oracle as jo
+    n = 256
+    phase = ms.square_inclusion(n, 0.5)
+    mat_m = material.create(1.0, 0.3, 2)
+    mat_i = material.create(10.0, 0.3, 2)
+    E = np.array([0., 0., 1.])
+    out = {}
+    for label, g0 in (('C0=matrix', 1.0), ('C0=midpoint', 5.5)):
+        mat0 = material.create(g0, 0.3, 2)
+        solver, table = _two_phase_solver(phase, mat_m, mat_i, mat0,
+                                          'truncated')
+        _sync()
+        t0 = time.perf_counter()
+        r = solver.solve(E, tol=1e-10, max_iter=200)
+        _sync()
+        t = time.perf_counter() - t0
+        ref_iters = 12
+        eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1],
+                                           E, 'truncated', g0, 0.3, tol=0.,
+                                           max_iter=ref_iters)
+        rp = solver.solve(E, tol=0., max_iter=ref_iters)
+        e = rp.eps.cpu().numpy()
+        rel = float(np.abs(e - eps_ref).max() / np.abs(eps_ref).max())
+        res_rel = float(np.max(np.abs(rp.residuals - res_ref) / res_ref))
+        tau, _ = solver.local.means(r.eps)
+        out[label] = {
+            'mu0': g0, 'status': r.status, 'iterations': r.iterations,
+            'residual_first': float(r.residuals[0]),
+            'residual_last': float(r.residuals[-1]),
+            'seconds': round(t, 4),
+            'it_per_s': round(r.iterations / t, 1),
+            'C_1212': float(g0 + 0.5 * tau[2] / E[2])
+            if r.status == 'converged' else None,
+            'oracle_%d_iterations' % ref_iters: {
+                'strain_rel_err': rel, 'residual_rel_err': res_rel}}
+        _log('%s: %s after %d iterations, residual %.3e'
+             % (label, r.status, r.iterations, r.residuals[-1]))
+    _line('2D 256^2 basic scheme (square inclusion, contrast 10)',
+          out['C0=midpoint']['it_per_s'], 'iterations/s',
+          {'workload': '2D 256^2 centred square vf 0.25, mu 1/10 nu 0.3, '
+                       'truncated Gamma0, E=(0,0,1), tol 1e-10, max 200',
+           'runs': out})
+
+
+def workload_stiffness(args):
+    """128^3 Boolean spheres r = 8, vf ~ 0.2, seed 2014; matrix mu 1 nu 0.3,
+    inclusions mu 50 nu 0.25; filtered Gamma0; the six Mandel unit loads
+    one after another and as one batch at 1e-10 -> the 6x6 C_eff.  C0 =
+    matrix diverges (reported over 30 iterations); the solves use the
+    midpoint mu0 = 25.5."""
+    import torch
+    import janus_amd.material.elastic.linear.isotropic as material
+    from janus_amd import microstructure as ms
+    from janus_amd.batch import effective_stiffness_batched
+    from janus_amd.solver import effective_stiffness
+    from oracle import janus_oracle as jo
+    n = args.grid or 128
+    phase = ms.spheres(n, 8.0 * n / 128, 0.2, seed=2014)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(50.0, 0.25, 3)
+    mat_div = material.create(1.0, 0.3, 3)
+    s_div, _ = _two_phase_solver(phase, mat_m, mat_i, mat_div, 'filtered')
+    r_div = s_div.solve(_unit(6, 5), tol=1e-10, max_iter=30)
+    del s_div
+    mat0 = material.create(25.5, 0.3, 3)
+    solver, table = _two_phase_solver(phase, mat_m, mat_i, mat0, 'filtered')
+    if not args.no_autotune:
+        solver.autotune(iters=3)
+    # parity: first iterations vs the oracle (load e_xy)
+    k = 3
+    eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1],
+                                       _unit(6, 5), 'filtered', 25.5, 0.3,
+                                       tol=0., max_iter=k)
+    rp = solver.solve(_unit(6, 5), tol=0., max_iter=k)
+    rel = float(np.abs(rp.eps.cpu().numpy() - eps_ref).max() /
+                np.abs(eps_ref).max())
+    solver.solve(_unit(6, 0), tol=0., max_iter=3)  # warm-up
+    _sync()
+    t0 = time.perf_counter()
+    C_seq, its, sts = effective_stiffness(solver, mat0, tol=1e-10,
+                                          max_iter=5000)
+    _sync()
+    t_seq = time.perf_counter() - t0
+    _log('sequential 6 loads: %.3f s, iterations %s' % (t_seq, its))
+    t0 = time.perf_counter()
+    Cs, its_b, sts_b, batch = effective_stiffness_batched(
+        [solver], [mat0], tol=1e-10, max_iter=5000, autotune=True)
+    _sync()
+    t_b = time.perf_counter() - t0
+    C_b = Cs[0]
+    _log('batched 6 loads: %.3f s, widths %s' % (t_b, batch.groups()))
+    total_it = int(sum(its))
+    B = survey_bytes((n, n, n))
+    _line('6x6 effective stiffness at %d^3 (six load cases, 1e-10)' % n,
+          round(t_seq, 4), 's', {
+              'workload': '%d^3 spheres r=%g vf~0.2 seed 2014, mu 1/50 nu '
+                          '0.3/0.25, filtered Gamma0, C0 mu0=25.5 nu0=0.3, '
+                          'Mandel unit loads, tol 1e-10' % (n, 8.0 * n / 128),
+              'C_eff': np.round(C_seq, 10).tolist(),
+              'iterations': its, 'status': sts,
+              'seconds_sequential': round(t_seq, 4),
+              'seconds_batched': round(t_b, 4),
+              'batched_widths': [g[2] for g in batch.groups()],
+              'batched_equal_bitwise': bool(np.array_equal(C_b, C_seq)),
+              'batched_iterations': its_b[0],
+              'iterations_per_s': round(total_it / t_seq, 1),
+              'iteration_GBps_B_alg': round(B * total_it / t_seq / 1e9, 1),
+              'symmetry_rel': float(np.abs(C_seq - C_seq.T).max() /
+                                    np.abs(C_seq).max()),
+              'C0_matrix_divergence': {
+                  'status': r_div.status,
+                  'residuals': [float(x) for x in r_div.residuals[::5]]},
+              'oracle_%d_iterations_strain_rel_err' % k: rel},
+          higher=False)
+
+
+def _config3_list(seed=3, count=32):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice([64, 80, 96, 128, 160]))
+        vf = float(rng.uniform(0.1, 0.35))
+        c = float(np.exp(rng.uniform(np.log(2.0), np.log(100.0))))
+        out.append((n, vf, c, 1000 + i))
+    return out
+
+
+def workload_batched(args):
+    """32 seeded microstructures, n ~ U{64, 80, 96, 128, 160}, Boolean
+    spheres r = 8, vf ~ U[0.1, 0.35], inclusion/matrix contrast ~
+    logU[2, 100] (matrix mu 1 nu 0.3, inclusions nu 0.25), filtered Gamma0,
+    midpoint C0, six Mandel unit loads each: 192 right-hand sides at 1e-10,
+    one autotuned batch vs 192 sequential solves."""
+    import torch
+    import janus_amd.material.elastic.linear.isotropic as material
+    from janus_amd import microstructure as ms
+    from janus_amd.batch import BatchSolver
+    cases = _config3_list(count=args.batch_count)
+    solvers = []
+    for n, vf, c, seed in cases:
+        phase = ms.spheres(n, 8.0, vf, seed=seed)
+        mat_m = material.create(1.0, 0.3, 3)
+        mat_i = material.create(c, 0.25, 3)
+        mat0 = material.create(0.5 * (1.0 + c), 0.3, 3)
+        S = _two_phase_solver(phase, mat_m, mat_i, mat0, 'filtered')[0]
+        if not args.no_autotune:
+            S.autotune(iters=3)
+        solvers.append(S)
+    _log('built %d microstructures' % len(solvers))
+    rhs = [(i, j) for i in range(len(solvers)) for j in range(6)]
+
+    def fields():
+        out = []
+        for i, j in rhs:
+            e = torch.empty(solvers[i].shape, dtype=torch.float64,
+                            device='cuda')
+            e[...] = torch.from_numpy(_unit(6, j)).cuda()
+            out.append(e)
+        return out
+    # warm-up: every grid size once
+    seen = set()
+    for S in solvers:
+        if S.shape not in seen:
+            seen.add(S.shape)
+            S.solve(_unit(6, 0), tol=0., max_iter=2)
+    eps_seq = fields()
+    _sync()
+    t0 = time.perf_counter()
+    seq = []
+    for (i, j), e in zip(rhs, eps_seq):
+        r = solvers[i].solve(_unit(6, j), tol=1e-10, max_iter=args.max_iter,
+                             eps=e)
+        seq.append((r.iterations, r.status))
+    _sync()
+    t_seq = time.perf_counter() - t0
+    _log('sequential: %.3f s, %d iterations' % (t_seq,
+                                                sum(x[0] for x in seq)))
+    eps_b = fields()
+    batch = BatchSolver(iters_per_launch=args.iters_per_launch, graph=True,
+                        autotune=True)
+    for (i, j), e in zip(rhs, eps_b):
+        batch.add(solvers[i], _unit(6, j), tol=1e-10, max_iter=args.max_iter,
+                  eps=e)
+    _sync()
+    t0 = time.perf_counter()
+    res = batch.run()
+    _sync()
+    t_b = time.perf_counter() - t0
+    ms_run, replays, captures = batch.stats()
+    _log('batched: %.3f s (%d replays, %d captures)' % (t_b, replays,
+                                                         captures))
+    per_grid = {}
+    if args.per_group:
+        # each grid size alone: sequential solves vs its own batch
+        for n in sorted(set(c[0] for c in cases)):
+            idx = [k for k, (i, j) in enumerate(rhs) if cases[i][0] == n]
+            t_s = 0.
+            for k in idx:
+                i, j = rhs[k]
+                eps_seq[k][...] = torch.from_numpy(_unit(6, j)).cuda()
+                _sync()
+                t0 = time.perf_counter()
+                solvers[i].solve(_unit(6, j), tol=1e-10,
+                                 max_iter=args.max_iter, eps=eps_seq[k])
+                _sync()
+                t_s += time.perf_counter() - t0
+            bg = BatchSolver(iters_per_launch=args.iters_per_launch,
+                             graph=True, autotune=True)
+            for k in idx:
+                i, j = rhs[k]
+                eps_seq[k][...] = torch.from_numpy(_unit(6, j)).cuda()
+                bg.add(solvers[i], _unit(6, j), tol=1e-10,
+                       max_iter=args.max_iter, eps=eps_seq[k])
+            _sync()
+            t0 = time.perf_counter()
+            bg.run()
+            _sync()
+            t_g = time.perf_counter() - t0
+            its_g = sum(seq[k][0] for k in idx)
+            per_grid[n] = {'entries': len(idx), 'iterations': its_g,
+                           'seconds_sequential': round(t_s, 4),
+                           'seconds_batched': round(t_g, 4),
+                           'speedup': round(t_s / t_g, 3),
+                           'widths': [g[2] for g in bg.groups()],
+                           'ms_per_entry_iteration_sequential':
+                               round(1e3 * t_s / its_g, 4),
+                           'ms_per_entry_iteration_batched':
+                               round(1e3 * t_g / its_g, 4)}
+            _log('grid %d: %s' % (n, per_grid[n]))
+            del bg
+    equal = sum(bool(torch.equal(a, b)) for a, b in zip(eps_seq, eps_b))
+    maxdiff = max(float((a - b).abs().max()) for a, b in zip(eps_seq, eps_b))
+    same_iters = all(r.iterations == s[0] for r, s in zip(res, seq))
+    its = [s[0] for s in seq]
+    conv = sum(s[1] == 'converged' for s in seq)
+    groups = [{'grid': list(g[0]), 'entries': g[1], 'width': g[2]}
+              for g in batch.groups()]
+    _line('batched vs sequential plain solves, %d right-hand sides (%d '
+          'microstructures x 6 loads) at 1e-10' % (len(rhs), len(solvers)),
+          round(t_seq / t_b, 3), 'x (speedup)', {
+              'workload': '%d seeded microstructures, n~U{64,80,96,128,160}, '
+                          'spheres r=8, vf~U[0.1,0.35], contrast~logU[2,100], '
+                          'filtered Gamma0, midpoint C0, 6 Mandel unit loads, '
+                          'tol 1e-10, max_iter %d'
+                          % (len(solvers), args.max_iter),
+              'grids': [c[0] for c in cases],
+              'contrasts': [round(c[2], 3) for c in cases],
+              'seconds_sequential': round(t_seq, 4),
+              'seconds_batched': round(t_b, 4),
+              'iterations_total': int(sum(its)),
+              'iterations_min_max': [int(min(its)), int(max(its))],
+              'converged': conv,
+              'replays': replays, 'graph_captures': captures,
+              'iters_per_launch': args.iters_per_launch,
+              'groups': groups,
+              'per_grid': per_grid,
+              'parity': {'entries_bitwise_equal': equal,
+                         'entries': len(rhs),
+                         'max_abs_diff': maxdiff,
+                         'same_iteration_counts': same_iters}})
+
+
+def workload_adjoint(args):
+    """128^3 SIMP density: rho = periodic Gaussian-smoothed noise (sigma 4
+    voxels, seed 2014) mapped to [0, 1], mu = 1 + 19 rho^3, nu = 0.3,
+    midpoint C0 (mu0 = 10.5), filtered Gamma0, E = e_xy, L = (C_1212 - 3)^2.
+    Time of the backward (Gamma0 g, adjoint solve, contraction) over the
+    forward solve, both at 1e-10; dL/drho at 20 seeded voxels against
+    central differences (step 1e-6, solves at a fixed iteration count past
+    1e-14)."""
+    import torch
+    import janus_amd.fft.serial as fft
+    import janus_amd.material.elastic.linear.isotropic as material
+    from janus_amd import microstructure as ms
+    from janus_amd.autograd import iso_delta, ls_solve, mean_stress
+    from janus_amd.green import filtered
+    n = args.grid or 128
+    rho_np = ms.smoothed_density(n, 4.0, seed=2014)
+    mat0 = material.create(10.5, 0.3, 3)
+    green = filtered(mat0.green_operator(), (n,) * 3, 1.0,
+                     fft.create_real((n,) * 3))
+    E = torch.zeros(6, dtype=torch.float64, device='cuda')
+    E[5] = 1.0
+    if not args.no_autotune:
+        # tune the Green operator's plans on the same grid
+        from janus_amd.operators import BlockDiagonalOperator3D
+        from janus_amd.solver import LSSolver
+        z = torch.zeros((n,) * 3, dtype=torch.float64, device='cuda')
+        LSSolver(green, BlockDiagonalOperator3D.from_fields(z, z)).autotune(3)
+
+    def loss_fn(rho, tol, max_iter, info=None):
+        g = 1.0 + rho ** 3 * 19.0
+        nu = torch.full_like(rho, 0.3)
+        sph, dev = iso_delta(g, nu, mat0, 3)
+        eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
+                       info=info)
+        sigma = mean_stress(sph, dev, eps, E, mat0)
+        return (0.5 * sigma[5] / E[5] - 3.0) ** 2
+
+    # warm-up
+    with torch.no_grad():
+        loss_fn(torch.tensor(rho_np, device='cuda'), 0., 3)
+    ratios = []
+    for rep in range(args.adjoint_reps):
+        rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
+        info = {}
+        _sync()
+        t0 = time.perf_counter()
+        loss = loss_fn(rho, 1e-10, 5000, info)
+        _sync()
+        t1 = time.perf_counter()
+        loss.backward()
+        _sync()
+        t2 = time.perf_counter()
+        ratios.append((t1 - t0, t2 - t1))
+    t_fwd, t_bwd = min(r[0] for r in ratios), min(r[1] for r in ratios)
+    fwd, adj = info['forward'], info['adjoint']
+    _log('forward %.3f s (%d it), backward %.3f s (adjoint %d it)'
+         % (t_fwd, fwd.iterations, t_bwd, adj.iterations))
+    grad = rho.grad.cpu().numpy()
+    fd = None
+    if args.fd_voxels > 0:
+        info2 = {}
+        rho2 = torch.tensor(rho_np, device='cuda', requires_grad=True)
+        l2 = loss_fn(rho2, 1e-14, 5000, info2)
+        l2.backward()
+        grad14 = rho2.grad.cpu().numpy()
+        iters = info2['forward'].iterations + 50
+        rng = np.random.default_rng(20)
+        h = 1e-6
+        errs = []
+        for _ in range(args.fd_voxels):
+            v = tuple(rng.integers(0, n, 3))
+            rp = rho_np.copy()
+            rp[v] += h
+            rm = rho_np.copy()
+            rm[v] -= h
+            with torch.no_grad():
+                lp = loss_fn(torch.tensor(rp, device='cuda'), 0., iters).item()
+                lm = loss_fn(torch.tensor(rm, device='cuda'), 0., iters).item()
+            d = (lp - lm) / (2 * h)
+            errs.append(abs(d - grad14[v]) / abs(grad14[v]))
+        fd = {'voxels': args.fd_voxels, 'step': h,
+              'fd_solve_iterations': iters,
+              'max_rel_err': float(max(errs)),
+              'median_rel_err': float(np.median(errs)),
+              'grad_1e10_vs_1e14_max_rel': float(
+                  np.abs(grad - grad14).max() / np.abs(grad14).max())}
+    _line('adjoint gradient time / forward solve time at %d^3 (1e-10)' % n,
+          round(t_bwd / t_fwd, 4), 'ratio', {
+              'workload': '%d^3 SIMP rho smoothed noise sigma 4 seed 2014, '
+                          'mu=1+19 rho^3 nu 0.3, filtered Gamma0, mu0 10.5, '
+                          'E=e_xy, L=(C_1212-3)^2, tol 1e-10' % n,
+              'seconds_forward': round(t_fwd, 4),
+              'seconds_backward': round(t_bwd, 4),
+              'forward_iterations': fwd.iterations,
+              'adjoint_iterations': adj.iterations,
+              'loss': float(loss.item()),
+              'finite_differences': fd},
+          higher=False)
+
+
+def _ellipsoids_gpu(n, semi_axes, count, seed, labels=(1, 2)):
+    """Same draws and voxel rule as microstructure.ellipsoids, evaluated on
+    the GPU (the numpy version takes minutes at 512^3)."""
+    import torch
+    from janus_amd.microstructure import _random_rotation
+    rng = np.random.default_rng(seed)
+    phase = torch.zeros((n, n, n), dtype=torch.uint8, device='cuda')
+    r = max(semi_axes)
+    ax = torch.tensor(semi_axes, dtype=torch.float64, device='cuda')
+    for i in range(count):
+        c = rng.uniform(0., n, size=3)
+        rot = torch.from_numpy(_random_rotation(rng)).cuda()
+        idx = [torch.arange(int(np.floor(c[d] - r - 0.5)),
+                            int(np.ceil(c[d] + r + 0.5)) + 1, device='cuda')
+               for d in range(3)]
+        g = torch.meshgrid(*idx, indexing='ij')
+        x = torch.stack([g[d].double() + 0.5 - c[d] for d in range(3)],
+                        dim=-1) @ rot
+        inside = ((x / ax) ** 2).sum(dim=-1) <= 1.0
+        sel = tuple(torch.remainder(g[d][inside], n) for d in range(3))
+        phase[sel] = labels[i % len(labels)]
+    return phase
+
+
+def workload_large(args):
+    """512^3, three phases: matrix mu 1 and randomly oriented 1:1:4
+    ellipsoids (semi-axes 8, 8, 32) of mu 10 and 0.1 (nu 0.3), filtered
+    Gamma0, midpoint C0 (mu0 = 5.05).  Iterations/s of the six Mandel loads
+    as one batch, and one adjoint gradient of L = C_eff[5, 5] (E = e_xy)
+    with respect to the three phase moduli, checked against a central
+    difference on the mu of phase 1."""
+    import torch
+    import janus_amd.fft.serial as fft
+    import janus_amd.material.elastic.linear.isotropic as material
+    from janus_amd.autograd import iso_delta, ls_solve, mean_stress
+    from janus_amd.batch import BatchSolver
+    from janus_amd.green import filtered
+    from janus_amd.operators import BlockDiagonalOperator3D
+    from janus_amd.solver import LSSolver, delta_stiffness
+    n = args.grid or 512
+    vol = 4. / 3. * np.pi * 8 * 8 * 32 * (n / 512.) ** 3
+    count = int(-np.log(1 - 0.3) * n ** 3 / vol)
+    axes = tuple(a * n / 512. for a in (8., 8., 32.))
+    t0 = time.perf_counter()
+    phase_d = _ellipsoids_gpu(n, axes, count, seed=2014)
+    frac = torch.bincount(phase_d.flatten().long(), minlength=3).double() \
+        / phase_d.numel()
+    _log('phase map %d^3 (%d ellipsoids) in %.1f s, fractions %s'
+         % (n, count, time.perf_counter() - t0, frac.tolist()))
+    phase = phase_d.cpu().numpy()
+    mus = [1.0, 10.0, 0.1]
+    mats = [material.create(m, 0.3, 3) for m in mus]
+    mat0 = material.create(0.5 * (10.0 + 0.1), 0.3, 3)
+    table = np.array([delta_stiffness(m, mat0) for m in mats])
+    green = filtered(mat0.green_operator(), (n,) * 3, 1.0,
+                     fft.create_real((n,) * 3))
+    local = BlockDiagonalOperator3D.from_phases(phase, table)
+    solver = LSSolver(green, local)
+    if not args.no_autotune:
+        _log('autotune %s' % json.dumps(solver.autotune(iters=3)))
+    # six loads as one batch: fixed iteration count, iterations/s
+    k = args.large_iters
+    batch = BatchSolver(iters_per_launch=k, graph=False)
+    fields = []
+    for j in range(6):
+        e = torch.empty(solver.shape, dtype=torch.float64, device='cuda')
+        fields.append(e)
+        batch.add(solver, _unit(6, j), tol=0., max_iter=k, eps=e)
+    t_b = None
+    for rep in range(2):  # warm-up, then timed; each restarts from E
+        for j, e in enumerate(fields):
+            e[...] = torch.from_numpy(_unit(6, j)).cuda()
+        _sync()
+        t0 = time.perf_counter()
+        res = batch.run()
+        _sync()
+        t_b = time.perf_counter() - t0
+    assert all(r.iterations == k for r in res)
+    its = 6 * k / t_b
+    B = survey_bytes((n, n, n))
+    _log('6-load batch: %d iterations each in %.3f s (%.2f load-it/s)'
+         % (k, t_b, its))
+    del batch, fields, res
+    torch.cuda.empty_cache()
+    # adjoint gradient w.r.t. the phase moduli through the field operator
+    ph = torch.from_numpy(phase.astype(np.int64)).cuda()
+    E = torch.zeros(6, dtype=torch.float64, device='cuda')
+    E[5] = 1.0
+
+    def loss_fn(mu_t, tol, max_iter, info=None):
+        g = mu_t[ph]
+        nu = torch.full_like(g, 0.3)
+        sph, dev = iso_delta(g, nu, mat0, 3)
+        eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
+                       info=info)
+        return mean_stress(sph, dev, eps, E, mat0)[5]
+
+    mu_t = torch.tensor(mus, dtype=torch.float64, device='cuda',
+                        requires_grad=True)
+    info = {}
+    _sync()
+    t0 = time.perf_counter()
+    L = loss_fn(mu_t, 1e-10, args.max_iter, info)
+    _sync()
+    t1 = time.perf_counter()
+    L.backward()
+    _sync()
+    t2 = time.perf_counter()
+    grad = mu_t.grad.cpu().numpy()
+    _log('forward %.2f s (%d it), backward %.2f s (%d it), grad %s'
+         % (t1 - t0, info['forward'].iterations, t2 - t1,
+            info['adjoint'].iterations, grad))
+    fd = None
+    if args.fd_voxels > 0:
+        h = 1e-4
+        vals = []
+        for sgn in (1, -1):
+            m = torch.tensor(mus, dtype=torch.float64, device='cuda')
+            m[1] += sgn * h
+            with torch.no_grad():
+                vals.append(loss_fn(m, 1e-10, args.max_iter).item())
+        d = (vals[0] - vals[1]) / (2 * h)
+        fd = {'phase': 1, 'step': h, 'fd': d, 'adjoint': float(grad[1]),
+              'rel_err': abs(d - grad[1]) / abs(grad[1])}
+    _line('6-load batch LS iterations/s at %d^3 FP64 (three phases)' % n,
+          round(its, 3), 'load-iterations/s', {
+              'workload': '%d^3, %d ellipsoids 1:1:4 (semi-axes %g %g %g), '
+                          'mu 1/10/0.1 nu 0.3, filtered Gamma0, mu0 5.05, '
+                          '6 Mandel unit loads as one batch, %d iterations'
+                          % (n, count, axes[0], axes[1], axes[2], k),
+              'phase_fractions': [round(x, 4) for x in frac.tolist()],
+              'B_alg_per_load_iteration': int(B),
+              'GBps_B_alg': round(B * its / 1e9, 1),
+              'roofline_frac_B_alg': round(B * its / 1e9 / HBM_PEAK_GBS, 4),
+              'adjoint': {
+                  'loss': 'C_eff[5,5] under E=e_xy',
+                  'value': float(L.item()),
+                  'dL_dmu_phase': grad.tolist(),
+                  'seconds_forward': round(t1 - t0, 3),
+                  'seconds_backward': round(t2 - t1, 3),
+                  'forward_iterations': info['forward'].iterations,
+                  'adjoint_iterations': info['adjoint'].iterations,
+                  'fd_check': fd}})
+
+
+WORKLOADS = {'plain2d': workload_plain2d, 'stiffness': workload_stiffness,
+             'batched': workload_batched, 'adjoint': workload_adjoint,
+             'large': workload_large}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=10)
-    ap.add_argument('--grid', type=int, default=256)
+    ap.add_argument('--grid', type=int, default=None,
+                    help='grid size (default: 256 for the metric, the '
+                         "workload's own size otherwise)")
     ap.add_argument('--kind', default='filtered',
                     choices=['filtered', 'truncated'])
     ap.add_argument('--profile-iters', type=int, default=20)
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='keep the default plan variants and schedule')
     ap.add_argument('--no-cpu', action='store_true',
                     help='skip the CPU baseline leg')
     ap.add_argument('--cpu-iters', type=int, default=2)
@@ -231,6 +805,20 @@ def main():
     ap.add_argument('--copy-mb', type=int, default=0,
                     help='after the bench, copy a buffer of this size '
                          '(reference kernel for HBM traffic counters)')
+    ap.add_argument('--workload', default='iteration',
+                    choices=['iteration'] + sorted(WORKLOADS),
+                    help='iteration: the 256^3 metric (default); others: '
+                         'the configurations listed in the module docstring')
+    ap.add_argument('--max-iter', type=int, default=5000,
+                    help='iteration cap of the converged workloads')
+    ap.add_argument('--batch-count', type=int, default=32,
+                    help='microstructures of the batched workload')
+    ap.add_argument('--iters-per-launch', type=int, default=8)
+    ap.add_argument('--per-group', action='store_true',
+                    help='batched workload: also time each grid size alone')
+    ap.add_argument('--adjoint-reps', type=int, default=2)
+    ap.add_argument('--fd-voxels', type=int, default=20)
+    ap.add_argument('--large-iters', type=int, default=20)
     ap.add_argument('--traffic-summary', nargs=3,
                     metavar=('FETCH_CSV', 'WRITE_CSV', 'COMMAND'),
                     help='print the counter summary of two rocprofv3 --pmc '
@@ -248,8 +836,18 @@ def main():
     import torch
     from janus_amd import _lib
     torch.cuda.set_device(0)
-    n = args.grid
+    if args.workload != 'iteration':
+        WORKLOADS[args.workload](args)
+        return
+    n = args.grid or 256
     solver, phase, table, mat0 = build_workload(n, args.kind)
+    tuned = None
+    if not args.no_autotune:
+        # the library's autotuner: plan variant per pass, then schedule
+        tuned = solver.autotune(iters=5)
+        print('bench: autotune %s -> plans %s, fused %s'
+              % (json.dumps(tuned), solver.green.plans(), solver.fused),
+              file=sys.stderr, flush=True)
     if args.schedule != 'auto':
         solver.fused = args.schedule == 'fused'
     E = np.zeros(6)
@@ -326,6 +924,8 @@ def main():
             'grid': [n, n, n],
             'green': args.kind,
             'schedule': 'fused E+A' if fused else 'five passes',
+            'plans': solver.green.plans(),
+            'autotune_ms': tuned,
             'iteration_bytes_alg': int(iter_bytes),
             'iteration_GBps': round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
             'iteration_roofline_frac': round(

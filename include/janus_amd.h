@@ -142,6 +142,25 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
 int jx_solver_set_fused(jx_solver *solver, int fused);
 int jx_solver_get_fused(jx_solver *solver);
 
+/* Register FFT plans (no janus counterpart).  Lengths 64, 128, 256 and 512
+ * have two plans: variant 0 holds 16 values per thread, variant 1 holds 8
+ * (fewer registers and more waves per SIMD for one more LDS exchange).
+ * Each pass of an operator picks its own: pass 0 last-axis r2c, 1 axis-1
+ * transforms, 2 axis-0 transforms with Gamma-hat, 3 last-axis c2r (with the
+ * update), 4 the fused update pass.  Variants change rounding, not the
+ * operator.  Defaults come from timings at 256^3; jx_solver_autotune
+ * measures them for the solver's grid. */
+int jx_green_set_plan(jx_green *green, int pass, int variant);
+int jx_green_get_plan(jx_green *green, int pass);
+/* Times every pass of the solver's grid with each plan variant (`iters`
+ * launches each, on scratch fields) and keeps the fastest in the solver's
+ * Green operator, then picks the fused schedule when its pass is faster
+ * than the separate last-axis passes.  ms_out: NULL or 10 doubles, the
+ * average time of pass p with variant k at [2 p + k] (0 when not run).
+ * Synchronizes the stream. */
+int jx_solver_autotune(jx_solver *solver, int iters, double *ms_out,
+                       void *stream);
+
 /* ---------------------------------------------------------------------
  * Batched solves (no janus counterpart: janus runs one fixed point at a
  * time from user scripts, sphinx/tutorials/square_basic/square_basic.py:
@@ -176,8 +195,9 @@ int jx_batch_group(jx_batch *batch, int group, int *dim, int64_t *shape,
 int jx_batch_set_width(jx_batch *batch, int group, int width);
 /* iters_per_launch >= 1 (default 8); graph = 1 (default) replays a
  * captured hipGraph; autotune = 1: after the first replay, one timed replay
- * per candidate width (all, 6, 3, 1 entries per launch) of every group whose
- * width was not set, then the fastest is kept (default 0: whole groups). */
+ * per candidate width (all, 12, 6, 3, 2, 1 entries per launch) of every
+ * group whose width was not set, then the width with the least time per
+ * entry-iteration run is kept (default 0: whole groups). */
 int jx_batch_set_schedule(jx_batch *batch, int iters_per_launch, int graph,
                           int autotune);
 /* Runs every entry to its stop on an internal stream after synchronizing

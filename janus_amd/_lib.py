@@ -66,6 +66,9 @@ SIGNATURES = [
     ('jx_solver_set_fused', ctypes.c_int, [_VP, ctypes.c_int]),
     ('jx_solver_get_fused', ctypes.c_int, [_VP]),
     ('jx_solver_profile', ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]),
+    ('jx_green_set_plan', ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    ('jx_green_get_plan', ctypes.c_int, [_VP, ctypes.c_int]),
+    ('jx_solver_autotune', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
     ('jx_batch_create', ctypes.c_int, [_PP]),
     ('jx_batch_destroy', ctypes.c_int, [_VP]),
     ('jx_batch_add', ctypes.c_int,

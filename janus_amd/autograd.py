@@ -46,15 +46,19 @@ def contract(eps, mu):
     return dsph, ddev
 
 
-def adjoint_solve(green, local, g, tol, max_iter):
-    """mu solving mu = G g - G[local : mu]; returns (mu, SolveResult)."""
+def adjoint_solve(green, local, g, tol, max_iter, solver=None):
+    """mu solving mu = G g - G[local : mu]; returns (mu, SolveResult).
+
+    ``solver``: an LSSolver on (green, local) to reuse (e.g. the forward
+    solve's)."""
     t = _torch()
     gg = green.apply(g.contiguous())
     norm = float(t.linalg.vector_norm(gg))
     if norm == 0.0:
         return t.zeros_like(gg), None
     mu = gg.clone()
-    solver = LSSolver(green, local)
+    if solver is None:
+        solver = LSSolver(green, local)
     r = solver.solve(np.zeros(gg.shape[-1]), tol=tol, max_iter=max_iter,
                      eps=mu, rhs=gg, res_scale=1.0 / norm)
     return mu, r
@@ -82,6 +86,9 @@ def _make_function():
                 info['forward'] = r
             ctx.save_for_backward(sph, dev, r.eps)
             ctx.green = green
+            # the backward reuses the forward's operators (same sph, dev)
+            ctx.local = local
+            ctx.solver = solver
             ctx.tol = tol
             ctx.max_iter = max_iter
             ctx.info = info
@@ -90,8 +97,9 @@ def _make_function():
         @staticmethod
         def backward(ctx, g):
             sph, dev, eps = ctx.saved_tensors
-            local = _local_from_fields(sph, dev)
-            mu, r = adjoint_solve(ctx.green, local, g, ctx.tol, ctx.max_iter)
+            local = ctx.local
+            mu, r = adjoint_solve(ctx.green, local, g, ctx.tol, ctx.max_iter,
+                                  solver=ctx.solver)
             if ctx.info is not None:
                 ctx.info['adjoint'] = r
             dsph, ddev = contract(eps, mu)

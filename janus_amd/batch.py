@@ -14,7 +14,8 @@ different grids -- and runs them together through ``jx_batch_run``:
   still has a running entry, and after the first replay the whole replay is a
   captured hipGraph, captured again only when a chunk has finished;
 * with ``autotune=True`` the first replays time the candidate widths (all,
-  6, 3, 1 entries per launch) of every group and keep the fastest.
+  12, 6, 3, 2, 1 entries per launch) of every group and keep the one with
+  the least time per entry-iteration.
 
 Each entry stops after the first iteration whose residual is below its
 ``tol``, exactly as :meth:`LSSolver.solve` would, with the same iterates.

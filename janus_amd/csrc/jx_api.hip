@@ -6,6 +6,7 @@
 #include <mutex>
 #include <string>
 
+#include "jx_fft_reg.h"
 #include "jx_gamma.h"
 #include "jx_handles.h"
 #include "jx_internal.h"
@@ -40,6 +41,10 @@ Grid make_grid(int dim, const int64_t *shape) {
   g.rows = g.N / g.L;
   g.nh = g.L / 2 + 1;
   g.Nh = g.rows * g.nh;
+  // default register FFT plans, from timings at 256^3 (6 components): the
+  // fused update pass is faster with the 8-value plan (no spilled
+  // registers, 4 waves per SIMD), every other pass with the 16-value one
+  if (g.L == 256) g.plan[PASS_FUSED] = 1;
   return g;
 }
 
@@ -548,6 +553,129 @@ int jx_solver_set_fused(jx_solver *S, int fused) {
 int jx_solver_get_fused(jx_solver *S) {
   if (!S) return -1;
   return (S->fused && rows_fusable(S->green->g, S->green->sym)) ? 1 : 0;
+}
+
+int jx_green_set_plan(jx_green *G, int pass, int variant) {
+  if (!G) return fail(JX_EINVAL, "jx_green_set_plan: NULL operator");
+  if (pass < 0 || pass >= kPlanPasses)
+    return fail(JX_EINVAL, "pass must be in [0, 5)");
+  if (variant < 0 || variant > 1)
+    return fail(JX_EINVAL, "variant must be 0 or 1");
+  G->g.plan[pass] = variant;
+  return JX_OK;
+}
+
+int jx_green_get_plan(jx_green *G, int pass) {
+  if (!G || pass < 0 || pass >= kPlanPasses) return -1;
+  return G->g.plan[pass];
+}
+
+}  // extern "C"
+
+namespace {
+
+// Average duration (ms) of `iters` launches of one pass of solver S on
+// scratch fields, after one untimed launch.
+int time_pass(jx_solver *S, int pass, double *eps, int iters, hipStream_t s,
+              hipEvent_t e0, hipEvent_t e1, double *ms) {
+  jx_green *G = S->green;
+  const Grid &g = G->g;
+  const int C = G->sym;
+  RowsInverseLs ls;
+  ls.enabled = true;
+  ls.partials = S->partials;
+  int rc = JX_OK, nb = 0;
+  for (int it = 0; it <= iters && !rc; ++it) {
+    if (it == 1) (void)hipEventRecord(e0, s);
+    switch (pass) {
+      case PASS_ROWS_FWD:
+        rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev, S->st, s);
+        break;
+      case PASS_AXIS:
+        rc = launch_axis(g, C, g.dim == 3 ? 1 : 0, -1, G->spec, G->spec,
+                         nullptr, S->st, s);
+        break;
+      case PASS_GAMMA:
+        rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s);
+        break;
+      case PASS_ROWS_INV:
+        rc = launch_rows_inverse(g, C, G->spec, eps, ls, S->st, &nb, s);
+        break;
+      default:
+        rc = launch_rows_update_forward(g, C, G->spec, eps, ls, S->local->dev,
+                                        S->st, &nb, s);
+    }
+  }
+  (void)hipEventRecord(e1, s);
+  hipError_t e = hipEventSynchronize(e1);
+  if (!rc && e != hipSuccess) rc = hip_fail(e, "jx_solver_autotune");
+  float t = 0.f;
+  if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+  *ms = t / iters;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jx_solver_autotune(jx_solver *S, int iters, double *ms_out, void *stream) {
+  if (!S || iters < 1) return fail(JX_EINVAL, "jx_solver_autotune: invalid argument");
+  hipStream_t s = (hipStream_t)stream;
+  jx_green *G = S->green;
+  Grid &g = G->g;
+  const int C = G->sym;
+  if (ms_out)
+    for (int i = 0; i < 2 * kPlanPasses; ++i) ms_out[i] = 0.;
+  // scratch iterate (zero: every pass stays finite) and a running state
+  double *eps = nullptr;
+  JX_HIP(hipMalloc((void **)&eps, sizeof(double) * C * g.N));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = JX_OK;
+  hipError_t he = hipMemsetAsync(eps, 0, sizeof(double) * C * g.N, s);
+  if (he == hipSuccess) he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he != hipSuccess) rc = hip_fail(he, "jx_solver_autotune");
+  SolveState init = {};
+  init.tol = 0.;
+  init.res_scale = 1.;
+  init.max_iter = 1 << 30;
+  *S->st_host = init;
+  if (!rc) {
+    he = hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState),
+                        hipMemcpyHostToDevice, s);
+    if (he != hipSuccess) rc = hip_fail(he, "jx_solver_autotune");
+  }
+  const bool fusable = rows_fusable(g, C);
+  double best[kPlanPasses];
+  for (int p = 0; p < kPlanPasses && !rc; ++p) {
+    best[p] = 0.;
+    if (p == PASS_AXIS && g.dim != 3) continue;
+    if (p == PASS_FUSED && !fusable) continue;
+    const int len = (p == PASS_AXIS) ? (int)g.n[1]
+                    : (p == PASS_GAMMA) ? (int)g.n[0] : (int)g.L;
+    const int nv = reg_fft_length(len) ? reg_variants(len) : 1;
+    int keep = g.plan[p];
+    for (int k = 0; k < nv && !rc; ++k) {
+      g.plan[p] = k;
+      double ms = 0.;
+      rc = time_pass(S, p, eps, iters, s, e0, e1, &ms);
+      if (ms_out) ms_out[2 * p + k] = ms;
+      if (!rc && (k == 0 || ms < best[p])) {
+        best[p] = ms;
+        keep = k;
+      }
+    }
+    g.plan[p] = keep;
+  }
+  // schedule: fused pass against the separate passes A and E
+  if (!rc && fusable)
+    S->fused = best[PASS_FUSED] <= best[PASS_ROWS_FWD] + best[PASS_ROWS_INV];
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(eps);
+  return rc;
 }
 
 int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rhs,

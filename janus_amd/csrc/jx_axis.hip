@@ -126,6 +126,53 @@ __device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
   for (int c = 0; c < C; ++c) z[c * zs] = make_double2(yr[c], yi[c]);
 }
 
+// Gamma-hat at axis-0 frequencies e and L - e of inner index m, evaluated
+// once.  The truncated wave vector and the filtered alias set of L - e are
+// those of e with the axis-0 component negated (and the filtered weights
+// swapped), so Gamma-hat(L - e) = S Gamma-hat(e) S with S = -1 on the
+// Mandel components holding axis 0 once (zx, xy in 3D; xy in 2D) and +1
+// elsewhere.  This holds for every e except 0 and L / 2 (even L), which are
+// their own mirrors; the caller passes z2 = nullptr for those.  Applied to
+// the C complex values at z0[c * zs] and, when z2 != nullptr, z2[c * zs].
+template <int C>
+__device__ __forceinline__ void gamma_point_pair(const GreenDev &G, int e,
+                                                 int64_t m, int nh, double2 *z0,
+                                                 double2 *z2, int zs) {
+  constexpr int DIM = C == 3 ? 2 : 3;
+  constexpr int NC = C == 3 ? 6 : 21;
+  int b[3];
+  b[0] = e;
+  if (DIM == 2) {
+    b[1] = (int)m;
+  } else {
+    b[1] = (int)(m / nh);
+    b[2] = (int)(m - (int64_t)b[1] * nh);
+  }
+  double g[NC];
+  gamma_hat_moments<DIM>(G, b, g);
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    double2 *z = side ? z2 : z0;
+    if (z == nullptr) continue;
+    double xr[C], xi[C], yr[C], yi[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      double2 v = z[c * zs];
+      const bool flip = side && c >= C - (C == 3 ? 1 : 2);
+      xr[c] = flip ? -v.x : v.x;
+      xi[c] = flip ? -v.y : v.y;
+    }
+    sym_matvec<C>(g, xr, yr);
+    sym_matvec<C>(g, xi, yi);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const bool flip = side && c >= C - (C == 3 ? 1 : 2);
+      z[c * zs] = flip ? make_double2(-yr[c], -yi[c])
+                       : make_double2(yr[c], yi[c]);
+    }
+  }
+}
+
 // Register-resident transform along a strided axis of the spectrum for the
 // register FFT lengths.  NSEQ = T * C complex sequences per block (T points
 // of the contiguous inner index, all C components), TPP = L / V threads
@@ -142,13 +189,14 @@ __device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
 #define JX_GAMMA_PASS_MIN_WAVES 3
 #endif
 
-template <int L, int C, int NSEQ, int MODE>
-__global__ void __launch_bounds__(NSEQ * reg_tpp<L>(),
-                                  MODE == 0 && reg_v<L>() == 16
+template <class P, int C, int NSEQ, int MODE>
+__global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
+                                  MODE == 0 && reg_v<P>() == 16
                                       ? JX_GAMMA_PASS_MIN_WAVES
                                       : 1)
     k_axis_reg(const double *in, double *out, AxisArgs a, const double2 *W,
                GreenDev G, const SolveState *st, BatchRef br) {
+  constexpr int L = P::L;
   const Slot *sl;
   const int64_t blk = batch_block(br, sl);
   if (sl) {
@@ -161,8 +209,8 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<L>(),
     st = sl->st;
   }
   if (solve_done(st)) return;
-  constexpr int TPP = reg_tpp<L>();
-  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<P>();
+  constexpr int V = reg_v<P>();
   constexpr int T = NSEQ / C;
   constexpr int LS = reg_lds_stride(L);
   extern __shared__ double2 smem[];
@@ -179,7 +227,7 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<L>(),
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     if (valid) {
-      v[i] = *(const double2 *)(src + (int64_t)reg_in_pos<L>(t, i) * a.stride_ax);
+      v[i] = *(const double2 *)(src + (int64_t)reg_in_pos<P>(t, i) * a.stride_ax);
     } else {
       v[i] = make_double2(0., 0.);
     }
@@ -190,26 +238,30 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<L>(),
   __shared__ double2 twl[TQ ? L / 4 : 1];
   const double2 *Wt = stage_twiddles<L, NSEQ * TPP, TQ>(W, twl);
   double2 *seq = smem + s * LS;
-  reg_fft<L, (MODE > 0 ? 1 : -1), TQ>(v, t, seq, Wt);
+  reg_fft<P, (MODE > 0 ? 1 : -1), TQ>(v, t, seq, Wt);
   if constexpr (MODE == 0 && (C == 3 || C == 6)) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+    for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];
     __syncthreads();
-    for (int w = tid; w < tc * L; w += NSEQ * TPP) {
-      int p = w / L, e = w - p * L;
-      gamma_point<C>(G, e, m0 + p, a.nh, smem + (p * C) * LS + lp(e), LS);
+    // frequencies 0 ... L/2 of each point, each with its mirror L - e
+    constexpr int LH = L / 2 + 1;
+    for (int w = tid; w < tc * LH; w += NSEQ * TPP) {
+      int p = w / LH, e = w - p * LH;
+      double2 *col = smem + (p * C) * LS;
+      double2 *z2 = (e == 0 || 2 * e == L) ? nullptr : col + lp(L - e);
+      gamma_point_pair<C>(G, e, m0 + p, a.nh, col + lp(e), z2, LS);
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+    for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
     __syncthreads();
-    reg_fft<L, 1, TQ>(v, t, seq, Wt);
+    reg_fft<P, 1, TQ>(v, t, seq, Wt);
   }
   if (valid) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      *(double2 *)(dst + (int64_t)reg_out_pos<L>(t, i) * a.stride_ax) = v[i];
+      *(double2 *)(dst + (int64_t)reg_out_pos<P>(t, i) * a.stride_ax) = v[i];
     }
   }
 }
@@ -236,34 +288,35 @@ static void launch_axis_c(int dir, dim3 grid, size_t lds, hipStream_t s,
   }
 }
 
-template <int L, int C, int NSEQ>
+template <class P, int C, int NSEQ>
 static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
                                const double *in, double *out,
                                const AxisArgs &a, const double2 *W,
                                const GreenDev &G, const SolveState *st,
                                const BatchRef &br) {
-  dim3 block(NSEQ * reg_tpp<L>());
+  dim3 block(NSEQ * reg_tpp<P>());
   if (dir < 0) {
-    set_lds_attr(k_axis_reg<L, C, NSEQ, -1>, lds);
-    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, -1>), grid, block, lds, s, in,
+    set_lds_attr(k_axis_reg<P, C, NSEQ, -1>, lds);
+    hipLaunchKernelGGL((k_axis_reg<P, C, NSEQ, -1>), grid, block, lds, s, in,
                        out, a, W, G, st, br);
   } else if (dir > 0) {
-    set_lds_attr(k_axis_reg<L, C, NSEQ, 1>, lds);
-    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, 1>), grid, block, lds, s, in,
+    set_lds_attr(k_axis_reg<P, C, NSEQ, 1>, lds);
+    hipLaunchKernelGGL((k_axis_reg<P, C, NSEQ, 1>), grid, block, lds, s, in,
                        out, a, W, G, st, br);
   } else {
-    set_lds_attr(k_axis_reg<L, C, NSEQ, 0>, lds);
-    hipLaunchKernelGGL((k_axis_reg<L, C, NSEQ, 0>), grid, block, lds, s, in,
+    set_lds_attr(k_axis_reg<P, C, NSEQ, 0>, lds);
+    hipLaunchKernelGGL((k_axis_reg<P, C, NSEQ, 0>), grid, block, lds, s, in,
                        out, a, W, G, st, br);
   }
 }
 
-template <int L>
-static int launch_axis_reg_l(const Grid &g, int ncomp, int axis, int dir,
+template <class P>
+static int launch_axis_reg_p(const Grid &g, int ncomp, int axis, int dir,
                              const double *in, double *out,
                              const GreenDev *green, const SolveState *st,
-                             const FftLen &P, hipStream_t s, const Batch *b) {
-  constexpr int NSEQ = reg_nseq<L>();
+                             const FftLen &FL, hipStream_t s, const Batch *b) {
+  constexpr int L = P::L;
+  constexpr int NSEQ = reg_nseq<P>();
   static_assert(NSEQ % 6 == 0, "sequences per block must hold whole points");
   AxisArgs a;
   a.L = L;
@@ -288,31 +341,26 @@ static int launch_axis_reg_l(const Grid &g, int ncomp, int axis, int dir,
   dim3 grid((unsigned)(nbe * batch_count(b)));
   GreenDev G = green ? *green : GreenDev();
   switch (ncomp) {
-    case 1: launch_axis_reg_lc<L, 1, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st, br); break;
-    case 3: launch_axis_reg_lc<L, 3, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st, br); break;
-    case 6: launch_axis_reg_lc<L, 6, NSEQ>(dir, grid, lds, s, in, out, a, P.W, G, st, br); break;
+    case 1: launch_axis_reg_lc<P, 1, NSEQ>(dir, grid, lds, s, in, out, a, FL.W, G, st, br); break;
+    case 3: launch_axis_reg_lc<P, 3, NSEQ>(dir, grid, lds, s, in, out, a, FL.W, G, st, br); break;
+    case 6: launch_axis_reg_lc<P, 6, NSEQ>(dir, grid, lds, s, in, out, a, FL.W, G, st, br); break;
     default: return fail(JX_EINVAL, "launch_axis: ncomp must be 1, 3 or 6");
   }
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
 
+// The Gamma-hat pass and the plain axis passes choose their plan variants
+// separately.
 static int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
                            const double *in, double *out,
                            const GreenDev *green, const SolveState *st,
-                           const FftLen &P, hipStream_t s, const Batch *b) {
-  switch (P.L) {
-    case 16: return launch_axis_reg_l<16>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 32: return launch_axis_reg_l<32>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 64: return launch_axis_reg_l<64>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 128: return launch_axis_reg_l<128>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 256: return launch_axis_reg_l<256>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 512: return launch_axis_reg_l<512>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 80: return launch_axis_reg_l<80>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 96: return launch_axis_reg_l<96>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    case 160: return launch_axis_reg_l<160>(g, ncomp, axis, dir, in, out, green, st, P, s, b);
-    default: return fail(JX_EUNSUPPORTED, "no register FFT for this length");
-  }
+                           const FftLen &FL, hipStream_t s, const Batch *b) {
+  const int k = g.plan[dir == 0 ? PASS_GAMMA : PASS_AXIS];
+  return with_plan(FL.L, k, [&](auto plan) -> int {
+    return launch_axis_reg_p<decltype(plan)>(g, ncomp, axis, dir, in, out,
+                                             green, st, FL, s, b);
+  });
 }
 
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,

@@ -77,15 +77,19 @@ struct jx_batch {
 namespace {
 
 using Key = std::tuple<int, int64_t, int64_t, int64_t, int, uint64_t, bool,
-                       bool>;
+                       bool, int>;
 
+// Entries share launches when they have the same grid, kind, h (the axis
+// tables), rhs use, schedule and plan variants (the rounding).
 Key entry_key(const Entry &e) {
   const jx_green *G = e.S->green;
   uint64_t hb = 0;
   std::memcpy(&hb, &G->h, sizeof(hb));
   bool fused = e.S->fused && rows_fusable(G->g, G->sym);
+  int plans = 0;
+  for (int p = 0; p < kPlanPasses; ++p) plans |= G->g.plan[p] << p;
   return Key(G->g.dim, G->g.n[0], G->g.n[1], G->g.n[2], G->dev.kind, hb,
-             e.rhs != nullptr, fused);
+             e.rhs != nullptr, fused, plans);
 }
 
 void build_groups(jx_batch *B) {
@@ -259,7 +263,7 @@ std::vector<int> candidate_widths(const Group &gr) {
   const int n = (int)gr.members.size();
   std::vector<int> c;
   if (!gr.batched || gr.user_width || n <= 1) return c;
-  for (int w : {n, 6, 3, 1})
+  for (int w : {n, 12, 6, 3, 2, 1})
     if (w <= n && std::find(c.begin(), c.end(), w) == c.end()) c.push_back(w);
   return c;
 }
@@ -359,9 +363,11 @@ int run(jx_batch *B, hipStream_t caller) {
     std::vector<hipEvent_t> ev(2 * ng);
     for (auto &e : ev) JX_HIP(hipEventCreate(&e));
     for (size_t r = 0; r < rounds && !rc; ++r) {
-      std::vector<int> running(ng, 0);
+      // iterations done by the group's entries before the replay
+      std::vector<long long> before(ng, 0);
       for (int g = 0; g < ng; ++g)
-        for (int i : B->groups[g].members) running[g] += entry_running(B, i);
+        for (int i : B->groups[g].members)
+          before[g] += B->st_host[B->ents[i].slot].iter;
       for (int g = 0; g < ng && !rc; ++g) {
         Group &gr = B->groups[g];
         int w = r < cand[g].size() ? cand[g][r] : gr.width;
@@ -373,10 +379,14 @@ int run(jx_batch *B, hipStream_t caller) {
       ++B->replays;
       for (int g = 0; g < ng && !rc; ++g) {
         if (r >= cand[g].size()) continue;
+        long long done = -before[g];
+        for (int i : B->groups[g].members)
+          done += B->st_host[B->ents[i].slot].iter;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, ev[2 * g], ev[2 * g + 1]);
-        // time per running entry and iteration
-        cost[g].push_back(running[g] ? ms / (double)running[g] : 1e30);
+        // time per entry-iteration actually run (stopped entries' blocks
+        // exit at once and count for nothing)
+        cost[g].push_back(done > 0 ? ms / (double)done : 1e30);
       }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -386,7 +396,8 @@ int run(jx_batch *B, hipStream_t caller) {
       size_t best = 0;
       for (size_t i = 1; i < cost[g].size(); ++i)
         if (cost[g][i] < cost[g][best]) best = i;
-      B->groups[g].width = cand[g][best];
+      // no valid measurement: keep the whole group per launch
+      B->groups[g].width = cost[g][best] < 1e30 ? cand[g][best] : cand[g][0];
     }
   }
 

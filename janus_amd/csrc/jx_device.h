@@ -191,33 +191,43 @@ static inline void set_lds_attr(K kernel, size_t bytes) {
                               (int)bytes);
 }
 
-// Pairs per block of the register last-axis kernels and sequences per block
-// of the register axis kernels: 192 threads for every length.
-template <int L>
-constexpr int reg_np() {
-  return 192 / reg_tpp<L>();
+// Threads per block of the register kernels: 192, or six transforms when
+// those need more (a block of the axis kernels holds whole points of up to
+// six components).  Pairs per block of the last-axis kernels and sequences
+// per block of the axis kernels follow.
+template <class P>
+constexpr int reg_block_threads() {
+  return 6 * reg_tpp<P>() > 192 ? 6 * reg_tpp<P>() : 192;
 }
-template <int L>
+template <class P>
+constexpr int reg_np() {
+  return reg_block_threads<P>() / reg_tpp<P>();
+}
+template <class P>
 constexpr int reg_nseq() {
-  return 192 / reg_tpp<L>();
+  return reg_block_threads<P>() / reg_tpp<P>();
 }
 
-// Rows per block of the register last-axis kernels (0 = not available).
-static inline int reg_rows_per_block(int L, int C) {
+// Rows per block of the register last-axis kernels with plan variant k
+// (0 = no register FFT for L).
+static inline int reg_rows_per_block(int L, int C, int k) {
   if (!reg_fft_length(L) || (C != 1 && C != 3 && C != 6)) return 0;
   int np = 0;
-  switch (L) {
-    case 16: np = reg_np<16>(); break;
-    case 32: np = reg_np<32>(); break;
-    case 64: np = reg_np<64>(); break;
-    case 128: np = reg_np<128>(); break;
-    case 256: np = reg_np<256>(); break;
-    case 512: np = reg_np<512>(); break;
-    case 80: np = reg_np<80>(); break;
-    case 96: np = reg_np<96>(); break;
-    case 160: np = reg_np<160>(); break;
-  }
+  with_plan(L, k, [&](auto plan) -> int {
+    np = reg_np<decltype(plan)>();
+    return 0;
+  });
   return 2 * np / C;
+}
+
+// Most rows-kernel blocks over the plan variants of the last axis.
+static inline int64_t reg_rows_max_blocks(const Grid &g, int C) {
+  int64_t nb = 0;
+  for (int k = 0; k < reg_variants((int)g.L); ++k) {
+    int rb = reg_rows_per_block((int)g.L, C, k);
+    if (rb) nb = std::max<int64_t>(nb, (g.rows + rb - 1) / rb);
+  }
+  return nb;
 }
 
 // Rows per block of the generic last-axis kernels and their LDS bytes.
@@ -234,25 +244,17 @@ static inline size_t rows_lds_bytes(int L, int C, int RB) {
   return 2 * (size_t)nseq_max * lds_stride(L) * sizeof(double2);
 }
 
-#define JX_DISPATCH_LC(FN, ...)                                            \
-  switch (L) {                                                             \
-    case 16: JX_DISPATCH_C(16, FN, __VA_ARGS__);                           \
-    case 32: JX_DISPATCH_C(32, FN, __VA_ARGS__);                           \
-    case 64: JX_DISPATCH_C(64, FN, __VA_ARGS__);                           \
-    case 128: JX_DISPATCH_C(128, FN, __VA_ARGS__);                         \
-    case 256: JX_DISPATCH_C(256, FN, __VA_ARGS__);                         \
-    case 512: JX_DISPATCH_C(512, FN, __VA_ARGS__);                         \
-    case 80: JX_DISPATCH_C(80, FN, __VA_ARGS__);                           \
-    case 96: JX_DISPATCH_C(96, FN, __VA_ARGS__);                           \
-    case 160: JX_DISPATCH_C(160, FN, __VA_ARGS__);                         \
-    default: return fail(JX_EUNSUPPORTED, "no register FFT for length");   \
-  }
-#define JX_DISPATCH_C(LL, FN, ...)                                         \
-  switch (C) {                                                             \
-    case 1: return FN<LL, 1>(__VA_ARGS__);                                 \
-    case 3: return FN<LL, 3>(__VA_ARGS__);                                 \
-    case 6: return FN<LL, 6>(__VA_ARGS__);                                 \
-    default: return fail(JX_EINVAL, "ncomp must be 1, 3 or 6");            \
-  }
+// Runs FN<P, C>(args) for the plan P of variant K of length LEN and the
+// component count C in scope.
+#define JX_WITH_PLAN_C(LEN, K, FN, ...)                                    \
+  return with_plan((LEN), (K), [&](auto plan_) -> int {                    \
+    using P_ = decltype(plan_);                                            \
+    switch (C) {                                                           \
+      case 1: return FN<P_, 1>(__VA_ARGS__);                               \
+      case 3: return FN<P_, 3>(__VA_ARGS__);                               \
+      case 6: return FN<P_, 6>(__VA_ARGS__);                               \
+      default: return fail(JX_EINVAL, "ncomp must be 1, 3 or 6");          \
+    }                                                                      \
+  })
 
 }  // namespace jx

@@ -5,10 +5,13 @@
 // thread runs V / R butterflies j = t + TPP q (q < V / R) of the Stockham
 // recurrence (inputs src[j + r L/R], twiddles w^(r k), k = j mod Ns,
 // outputs dst[(j / Ns) Ns R + k + r Ns]); consecutive stages exchange data
-// through LDS.  Plans (V; radices):
-//   16 (16; 16)   32 (16; 16 2)   64 (16; 16 4)   128 (16; 16 8)
-//   256 (16; 16 16)   512 (16; 16 16 2)
+// through LDS.  Plans (V; radices), variant 0 and, where listed, 1:
+//   16 (16; 16)   32 (16; 16 2)   64 (16; 16 4) | (8; 8 8)
+//   128 (16; 16 8) | (8; 8 4 4)   256 (16; 16 16) | (8; 8 8 4)
+//   512 (16; 16 16 2) | (8; 8 8 8)
 //   80 (20; 5 4 4)   96 (12; 6 4 4)   160 (20; 10 4 4)
+// Variant 1 holds half the values per thread: fewer registers and more
+// waves per SIMD for one more LDS exchange; each pass picks its variant.
 // The first stage reads x[t + TPP (i / R0) + (i mod R0) L / R0] and the last
 // one leaves X[t + TPP q + r L / R_last] (i = q R_last + r) in registers, so
 // loads and stores are plain strided patterns.  Stage twiddles come from the
@@ -20,22 +23,32 @@
 
 namespace jx {
 
-template <int L>
+// A plan: length L, V values per thread, n stages of radices r0, r1, r2.
+template <int L_, int V_, int N_, int R0_, int R1_, int R2_>
+struct Plan {
+  static constexpr int L = L_, V = V_, n = N_, r0 = R0_, r1 = R1_, r2 = R2_;
+};
+
+template <int L, int K>
 struct RegPlan;
-#define JX_REG_PLAN(LL, VV, N, R0, R1, R2)                                  \
+#define JX_REG_PLAN(LL, KK, VV, N, R0, R1, R2)                              \
   template <>                                                              \
-  struct RegPlan<LL> {                                                     \
-    static constexpr int V = VV, n = N, r0 = R0, r1 = R1, r2 = R2;         \
+  struct RegPlan<LL, KK> {                                                 \
+    using type = Plan<LL, VV, N, R0, R1, R2>;                              \
   };
-JX_REG_PLAN(16, 16, 1, 16, 1, 1)
-JX_REG_PLAN(32, 16, 2, 16, 2, 1)
-JX_REG_PLAN(64, 16, 2, 16, 4, 1)
-JX_REG_PLAN(128, 16, 2, 16, 8, 1)
-JX_REG_PLAN(256, 16, 2, 16, 16, 1)
-JX_REG_PLAN(512, 16, 3, 16, 16, 2)
-JX_REG_PLAN(80, 20, 3, 5, 4, 4)
-JX_REG_PLAN(96, 12, 3, 6, 4, 4)
-JX_REG_PLAN(160, 20, 3, 10, 4, 4)
+JX_REG_PLAN(16, 0, 16, 1, 16, 1, 1)
+JX_REG_PLAN(32, 0, 16, 2, 16, 2, 1)
+JX_REG_PLAN(64, 0, 16, 2, 16, 4, 1)
+JX_REG_PLAN(64, 1, 8, 2, 8, 8, 1)
+JX_REG_PLAN(128, 0, 16, 2, 16, 8, 1)
+JX_REG_PLAN(128, 1, 8, 3, 8, 4, 4)
+JX_REG_PLAN(256, 0, 16, 2, 16, 16, 1)
+JX_REG_PLAN(256, 1, 8, 3, 8, 8, 4)
+JX_REG_PLAN(512, 0, 16, 3, 16, 16, 2)
+JX_REG_PLAN(512, 1, 8, 3, 8, 8, 8)
+JX_REG_PLAN(80, 0, 20, 3, 5, 4, 4)
+JX_REG_PLAN(96, 0, 12, 3, 6, 4, 4)
+JX_REG_PLAN(160, 0, 20, 3, 10, 4, 4)
 #undef JX_REG_PLAN
 
 // Lengths the register kernels are instantiated for.
@@ -44,11 +57,16 @@ constexpr bool reg_fft_length(int L) {
          L == 80 || L == 96 || L == 160;
 }
 
+// Plan variants compiled for a length.
+constexpr int reg_variants(int L) {
+  return (L == 64 || L == 128 || L == 256 || L == 512) ? 2 : 1;
+}
+
 // Values held per thread, threads per transform.
-template <int L>
-constexpr int reg_v() { return RegPlan<L>::V; }
-template <int L>
-constexpr int reg_tpp() { return L / RegPlan<L>::V; }
+template <class P>
+constexpr int reg_v() { return P::V; }
+template <class P>
+constexpr int reg_tpp() { return P::L / P::V; }
 
 // (a + i b) * (c + i d) with compile-time constants
 __device__ __forceinline__ double2 cmulc(double2 a, double c, double d) {
@@ -215,10 +233,11 @@ constexpr int reg_twiddle_entries() {
 }
 
 // One stage on the registers: twiddle, then V / R butterflies of radix R.
-template <int L, int R, int NSP, int DIR, bool TQ>
+template <class P, int R, int NSP, int DIR, bool TQ>
 __device__ __forceinline__ void reg_stage(double2 *v, int t, const double2 *W) {
-  constexpr int V = reg_v<L>();
-  constexpr int TPP = reg_tpp<L>();
+  constexpr int L = P::L;
+  constexpr int V = reg_v<P>();
+  constexpr int TPP = reg_tpp<P>();
   constexpr int tws = L / (NSP * R);
 #pragma unroll
   for (int q = 0; q < V / R; ++q) {
@@ -241,11 +260,11 @@ __device__ __forceinline__ int lp(int p) { return p + (p >> 4); }
 constexpr int reg_lds_stride(int L) { return L + L / 16 + 1; }
 
 // Write the outputs of a stage (radix R, NSP) to LDS sequence `seq`.
-template <int L, int R, int NSP>
+template <class P, int R, int NSP>
 __device__ __forceinline__ void reg_store_stage(const double2 *v, int t,
                                                 double2 *seq) {
-  constexpr int V = reg_v<L>();
-  constexpr int TPP = reg_tpp<L>();
+  constexpr int V = reg_v<P>();
+  constexpr int TPP = reg_tpp<P>();
 #pragma unroll
   for (int q = 0; q < V / R; ++q) {
     int j = t + TPP * q;
@@ -256,11 +275,12 @@ __device__ __forceinline__ void reg_store_stage(const double2 *v, int t,
 }
 
 // Read the inputs of a stage of radix R from LDS sequence `seq`.
-template <int L, int R>
+template <class P, int R>
 __device__ __forceinline__ void reg_load_stage(double2 *v, int t,
                                                const double2 *seq) {
-  constexpr int V = reg_v<L>();
-  constexpr int TPP = reg_tpp<L>();
+  constexpr int L = P::L;
+  constexpr int V = reg_v<P>();
+  constexpr int TPP = reg_tpp<P>();
 #pragma unroll
   for (int q = 0; q < V / R; ++q) {
     int j = t + TPP * q;
@@ -270,20 +290,21 @@ __device__ __forceinline__ void reg_load_stage(double2 *v, int t,
 }
 
 // Position of register slot i after the last stage.
-template <int L>
+template <class P>
 __device__ __forceinline__ int reg_out_pos(int t, int i) {
-  using P = RegPlan<L>;
-  constexpr int TPP = reg_tpp<L>();
+  constexpr int L = P::L;
+  constexpr int TPP = reg_tpp<P>();
   constexpr int RL = P::n == 1 ? P::r0 : (P::n == 2 ? P::r1 : P::r2);
   int q = i / RL, r = i - q * RL;
   return t + TPP * q + r * (L / RL);
 }
 
 // Position of register slot i before the first stage.
-template <int L>
+template <class P>
 __device__ __forceinline__ int reg_in_pos(int t, int i) {
-  constexpr int TPP = reg_tpp<L>();
-  constexpr int R0 = RegPlan<L>::r0;
+  constexpr int L = P::L;
+  constexpr int TPP = reg_tpp<P>();
+  constexpr int R0 = P::r0;
   int q = i / R0, r = i - q * R0;
   return t + TPP * q + r * (L / R0);
 }
@@ -292,23 +313,48 @@ __device__ __forceinline__ int reg_in_pos(int t, int i) {
 // v[i] = X[reg_out_pos(t, i)].  `seq` is this sequence's LDS scratch; the
 // caller's block must reach the barriers inside (every thread of the block
 // calls this function the same number of times).
-template <int L, int DIR, bool TQ>
+template <class P, int DIR, bool TQ>
 __device__ __forceinline__ void reg_fft(double2 *v, int t, double2 *seq,
                                         const double2 *W) {
-  using P = RegPlan<L>;
-  reg_stage<L, P::r0, 1, DIR, TQ>(v, t, W);
+  reg_stage<P, P::r0, 1, DIR, TQ>(v, t, W);
   if constexpr (P::n >= 2) {
-    reg_store_stage<L, P::r0, 1>(v, t, seq);
+    reg_store_stage<P, P::r0, 1>(v, t, seq);
     __syncthreads();
-    reg_load_stage<L, P::r1>(v, t, seq);
-    reg_stage<L, P::r1, P::r0, DIR, TQ>(v, t, W);
+    reg_load_stage<P, P::r1>(v, t, seq);
+    reg_stage<P, P::r1, P::r0, DIR, TQ>(v, t, W);
   }
   if constexpr (P::n >= 3) {
     __syncthreads();
-    reg_store_stage<L, P::r1, P::r0>(v, t, seq);
+    reg_store_stage<P, P::r1, P::r0>(v, t, seq);
     __syncthreads();
-    reg_load_stage<L, P::r2>(v, t, seq);
-    reg_stage<L, P::r2, P::r0 * P::r1, DIR, TQ>(v, t, W);
+    reg_load_stage<P, P::r2>(v, t, seq);
+    reg_stage<P, P::r2, P::r0 * P::r1, DIR, TQ>(v, t, W);
+  }
+}
+
+// Calls f(P{}) with the plan type of variant k of length L (variant 0 when
+// L has no variant k); returns JX_EUNSUPPORTED for other lengths.
+template <int LL, typename F>
+int with_variant(int k, F &&f) {
+  if constexpr (reg_variants(LL) > 1) {
+    if (k == 1) return f(typename RegPlan<LL, 1>::type());
+  }
+  return f(typename RegPlan<LL, 0>::type());
+}
+
+template <typename F>
+int with_plan(int L, int k, F &&f) {
+  switch (L) {
+    case 16: return with_variant<16>(k, f);
+    case 32: return with_variant<32>(k, f);
+    case 64: return with_variant<64>(k, f);
+    case 128: return with_variant<128>(k, f);
+    case 256: return with_variant<256>(k, f);
+    case 512: return with_variant<512>(k, f);
+    case 80: return with_variant<80>(k, f);
+    case 96: return with_variant<96>(k, f);
+    case 160: return with_variant<160>(k, f);
+    default: return fail(JX_EUNSUPPORTED, "no register FFT for this length");
   }
 }
 

@@ -93,7 +93,18 @@ struct Batch {
   int n = 0;
 };
 
-// Geometry of a grid with ncomp components per point.
+// Passes that choose a register FFT plan variant (jx_fft_reg.h).
+enum PlanPass {
+  PASS_ROWS_FWD = 0,  // A: last-axis r2c (+ local operator)
+  PASS_AXIS = 1,      // B, D: axis-1 transforms
+  PASS_GAMMA = 2,     // C: axis-0 transforms with Gamma-hat
+  PASS_ROWS_INV = 3,  // E: last-axis c2r (+ update)
+  PASS_FUSED = 4,     // E + A fused
+  kPlanPasses = 5
+};
+
+// Geometry of a grid with ncomp components per point, and the register FFT
+// plan variant each pass uses on it.
 struct Grid {
   int dim = 0;
   int64_t n[kMaxDim] = {1, 1, 1};
@@ -102,6 +113,7 @@ struct Grid {
   int64_t L = 0;      // n_last
   int64_t nh = 0;     // n_last / 2 + 1
   int64_t Nh = 0;     // rows * nh spectral points
+  int plan[kPlanPasses] = {0, 0, 0, 0, 0};
 };
 
 Grid make_grid(int dim, const int64_t *shape);
@@ -141,6 +153,8 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
                         const RowsInverseLs &ls, const SolveState *st,
                         int *nblocks_out, hipStream_t s,
                         const Batch *b = nullptr);
+// Residual partial sums written by the update pass: the most blocks any
+// plan variant of the last axis uses (the size of a partials buffer).
 int rows_inverse_nblocks(const Grid &g, int ncomp);
 // Fused pass E of iteration k + pass A of iteration k + 1 (register path).
 bool rows_fusable(const Grid &g, int ncomp);

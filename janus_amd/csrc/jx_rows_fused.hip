@@ -18,13 +18,14 @@ namespace jx {
 #define JX_FUSED_MIN_WAVES 3
 #endif
 
-template <int L, int C, int NP, int MODE>
-__global__ void __launch_bounds__(NP * reg_tpp<L>(),
-                                  reg_v<L>() == 16 ? JX_FUSED_MIN_WAVES : 1)
+template <class P, int C, int NP, int MODE>
+__global__ void __launch_bounds__(NP * reg_tpp<P>(),
+                                  reg_v<P>() == 16 ? JX_FUSED_MIN_WAVES : 1)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
                               double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st,
                               BatchRef br) {
+  constexpr int L = P::L;
   const Slot *sl;
   const int64_t blk = batch_block(br, sl);
   if (sl) {
@@ -36,8 +37,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
     st = sl->st;
   }
   if (solve_done(st)) return;
-  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
-  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<P>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   extern __shared__ double2 smem[];
@@ -109,9 +110,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
   double2 *seq = smem + p * LS;
   double2 v[V];
 #pragma unroll
-  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
   __syncthreads();
-  reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
+  reg_fft<P, 1, JX_ROWS_TQ>(v, t, seq, Wt);
 
   // ---- update; v becomes the new iterate (part 0 in .x, part 1 in .y)
   double acc = 0.;
@@ -125,7 +126,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
       const double e0 = st->E[c0], e1 = st->E[c0 + 1];
 #pragma unroll
       for (int i = 0; i < V; ++i) {
-        int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
+        int64_t off = (int64_t)reg_out_pos<P>(t, i) * 3;
         double2 nv = make_double2(ls_new(e0, v[i].x, scale),
                                   ls_new(e1, v[i].y, scale));
         if constexpr (MODE == 2) {
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
       for (int i = 0; i < V; ++i) {
         double nv = 0.;
         if (r < rb) {
-          int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
+          int64_t off = (int64_t)reg_out_pos<P>(t, i) * C;
           double raw = part ? v[i].y : v[i].x;
           nv = ls_new(st->E[c], raw, scale);
           if constexpr (MODE == 2) nv = ls_new_rhs(st->E[c], hr[off], raw, scale);
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
   // ---- the new iterate into LDS, local operator voxel by voxel
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];
   __syncthreads();
   for (int w = tid; w < rb * L; w += NT) {
     int r = w / L, j = w - r * L;
@@ -195,12 +196,12 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
 
   // ---- forward transform of the polarization rows (see k_rows_forward_reg)
 #pragma unroll
-  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
   __syncthreads();
-  reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
+  reg_fft<P, -1, JX_ROWS_TQ>(v, t, seq, Wt);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];
   __syncthreads();
   if constexpr (C == 6) {
     for (int w = tid; w < rb * NH * 3; w += NT) {
@@ -243,30 +244,31 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>(),
 }
 // ============================================================ launchers
 
-template <int L, int C>
+template <class P, int C>
 static int launch_rows_update_forward_reg_lc(const Grid &g, double *spec,
                                              double *y, const LsArgs &la,
                                              const LocalDev &loc,
                                              const SolveState *st,
                                              const double2 *W, hipStream_t s,
                                              const Batch *b) {
+  constexpr int L = P::L;
   if constexpr (C == 3 || C == 6) {
-    constexpr int NP = reg_np<L>();
+    constexpr int NP = reg_np<P>();
     constexpr int RB = 2 * NP / C;
     size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
     int rc = check_lds(lds, "rows_update_forward");
     if (rc) return rc;
     const int64_t nbe = (g.rows + RB - 1) / RB;
     const BatchRef br = batch_ref(b, nbe);
-    dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<L>());
+    dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<P>());
     double scale = 1.0 / (double)g.N;
     if (la.rhs) {
-      set_lds_attr(k_rows_update_forward_reg<L, C, NP, 2>, lds);
-      hipLaunchKernelGGL((k_rows_update_forward_reg<L, C, NP, 2>), grid, block,
+      set_lds_attr(k_rows_update_forward_reg<P, C, NP, 2>, lds);
+      hipLaunchKernelGGL((k_rows_update_forward_reg<P, C, NP, 2>), grid, block,
                          lds, s, spec, y, g.rows, scale, W, la, loc, st, br);
     } else {
-      set_lds_attr(k_rows_update_forward_reg<L, C, NP, 1>, lds);
-      hipLaunchKernelGGL((k_rows_update_forward_reg<L, C, NP, 1>), grid, block,
+      set_lds_attr(k_rows_update_forward_reg<P, C, NP, 1>, lds);
+      hipLaunchKernelGGL((k_rows_update_forward_reg<P, C, NP, 1>), grid, block,
                          lds, s, spec, y, g.rows, scale, W, la, loc, st, br);
     }
     JX_HIP(hipGetLastError());
@@ -281,12 +283,13 @@ static int rows_update_forward_reg(const Grid &g, int C, double *spec,
                                    const LocalDev &loc, const SolveState *st,
                                    const double2 *W, hipStream_t s,
                                    const Batch *b) {
-  const int L = (int)g.L;
-  JX_DISPATCH_LC(launch_rows_update_forward_reg_lc, g, spec, y, la, loc, st, W, s, b)
+  JX_WITH_PLAN_C((int)g.L, g.plan[PASS_FUSED],
+                 launch_rows_update_forward_reg_lc, g, spec, y, la, loc, st, W,
+                 s, b);
 }
 
 bool rows_fusable(const Grid &g, int ncomp) {
-  return (ncomp == 3 || ncomp == 6) && reg_rows_per_block((int)g.L, ncomp) > 0;
+  return (ncomp == 3 || ncomp == 6) && reg_fft_length((int)g.L);
 }
 
 int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,
@@ -302,7 +305,7 @@ int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,
   LsArgs la;
   la.rhs = ls.rhs;
   la.partials = ls.partials;
-  int RB = reg_rows_per_block((int)g.L, ncomp);
+  int RB = reg_rows_per_block((int)g.L, ncomp, g.plan[PASS_FUSED]);
   if (nblocks_out) *nblocks_out = (int)((g.rows + RB - 1) / RB);
   return rows_update_forward_reg(g, ncomp, spec, y, la, loc, st, P.W, s, b);
 }

@@ -84,11 +84,12 @@ __global__ void __launch_bounds__(kThreads)
 // ones (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / V
 // threads per pair.  Batched: the entry's iterate, spectrum, local operator
 // and state come from its slot.
-template <int L, int C, int NP, bool LOCAL>
-__global__ void __launch_bounds__(NP * reg_tpp<L>())
+template <class P, int C, int NP, bool LOCAL>
+__global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
                        int64_t nrows, const double2 *W, LocalDev loc,
                        const SolveState *st, BatchRef br) {
+  constexpr int L = P::L;
   const Slot *sl;
   const int64_t blk = batch_block(br, sl);
   if (sl) {
@@ -98,8 +99,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
     st = sl->st;
   }
   if (solve_done(st)) return;
-  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
-  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<P>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   static_assert((2 * NP) % C == 0, "whole rows per block");
@@ -145,12 +146,12 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
   double2 *seq = smem + p * LS;
   double2 v[V];
 #pragma unroll
-  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
   __syncthreads();
-  reg_fft<L, -1, JX_ROWS_TQ>(v, t, seq, Wt);
+  reg_fft<P, -1, JX_ROWS_TQ>(v, t, seq, Wt);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<L>(t, i))] = v[i];
+  for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];
   __syncthreads();
   // separate the pairs: Xa = (Z[k] + conj Z[-k]) / 2, Xb = (Z[k] - conj Z[-k]) / 2i
   if constexpr (C == 6) {
@@ -197,31 +198,32 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
 
 // ============================================================ launchers
 
-template <int L, int C>
+template <class P, int C>
 static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
                                       double *spec, const LocalDev &loc,
                                       const SolveState *st, const double2 *W,
                                       hipStream_t s, const Batch *b) {
-  constexpr int NP = reg_np<L>();
+  constexpr int L = P::L;
+  constexpr int NP = reg_np<P>();
   constexpr int RB = 2 * NP / C;
   size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
   int rc = check_lds(lds, "rows_forward");
   if (rc) return rc;
   const int64_t nbe = (g.rows + RB - 1) / RB;
   const BatchRef br = batch_ref(b, nbe);
-  dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<L>());
+  dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<P>());
   // a batch always applies the local operator of its slots
   if (b || loc.mode != LOCAL_NONE) {
     if constexpr (C == 3 || C == 6) {
-      set_lds_attr(k_rows_forward_reg<L, C, NP, true>, lds);
-      hipLaunchKernelGGL((k_rows_forward_reg<L, C, NP, true>), grid, block, lds,
+      set_lds_attr(k_rows_forward_reg<P, C, NP, true>, lds);
+      hipLaunchKernelGGL((k_rows_forward_reg<P, C, NP, true>), grid, block, lds,
                          s, x, spec, g.rows, W, loc, st, br);
     } else {
       return fail(JX_EINVAL, "local operator needs sym components");
     }
   } else {
-    set_lds_attr(k_rows_forward_reg<L, C, NP, false>, lds);
-    hipLaunchKernelGGL((k_rows_forward_reg<L, C, NP, false>), grid, block, lds,
+    set_lds_attr(k_rows_forward_reg<P, C, NP, false>, lds);
+    hipLaunchKernelGGL((k_rows_forward_reg<P, C, NP, false>), grid, block, lds,
                        s, x, spec, g.rows, W, loc, st, br);
   }
   JX_HIP(hipGetLastError());
@@ -231,8 +233,8 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
 static int rows_forward_reg(const Grid &g, int C, const double *x, double *spec,
                             const LocalDev &loc, const SolveState *st,
                             const double2 *W, hipStream_t s, const Batch *b) {
-  const int L = (int)g.L;
-  JX_DISPATCH_LC(launch_rows_forward_reg_lc, g, x, spec, loc, st, W, s, b)
+  JX_WITH_PLAN_C((int)g.L, g.plan[PASS_ROWS_FWD], launch_rows_forward_reg_lc,
+                 g, x, spec, loc, st, W, s, b);
 }
 
 int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
@@ -241,7 +243,7 @@ int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
   FftLen P;
   int rc = get_fft_len((int)g.L, &P);
   if (rc) return rc;
-  if (reg_rows_per_block((int)g.L, ncomp))
+  if (reg_fft_length((int)g.L))
     return rows_forward_reg(g, ncomp, x, spec, loc, st, P.W, s, b);
   if (b) return fail(JX_EUNSUPPORTED, "rows_forward: batch needs a register FFT length");
   RowsArgs a;

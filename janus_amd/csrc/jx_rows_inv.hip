@@ -97,11 +97,12 @@ __global__ void __launch_bounds__(kThreads)
 // 1: y <- E - c2r(spec) with residual partials; 2: y <- (E + rhs) -
 // c2r(spec) with residual partials.  Batched: the entry's spectrum,
 // iterate, rhs, partials and state come from its slot.
-template <int L, int C, int NP, int MODE>
-__global__ void __launch_bounds__(NP * reg_tpp<L>())
+template <class P, int C, int NP, int MODE>
+__global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_inverse_reg(const double *__restrict__ spec, double *y,
                        int64_t nrows, double scale, const double2 *W,
                        LsArgs ls, const SolveState *st, BatchRef br) {
+  constexpr int L = P::L;
   const Slot *sl;
   const int64_t blk = batch_block(br, sl);
   if (sl) {
@@ -112,8 +113,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
     st = sl->st;
   }
   if (solve_done(st)) return;
-  constexpr int TPP = reg_tpp<L>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
-  constexpr int V = reg_v<L>();
+  constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
+  constexpr int V = reg_v<P>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   constexpr bool LS_UPDATE = MODE > 0;
@@ -186,9 +187,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
   double2 *seq = smem + p * LS;
   double2 v[V];
 #pragma unroll
-  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<L>(t, i))];
+  for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
   __syncthreads();
-  reg_fft<L, 1, JX_ROWS_TQ>(v, t, seq, Wt);
+  reg_fft<P, 1, JX_ROWS_TQ>(v, t, seq, Wt);
   double acc = 0.;
   if constexpr (C == 6) {
     // pair p = components (c0, c0 + 1) of row r: 16-byte accesses
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
       const double e1 = LS_UPDATE ? st->E[c0 + 1] : 0.;
 #pragma unroll
       for (int i = 0; i < V; ++i) {
-        int64_t off = (int64_t)reg_out_pos<L>(t, i) * 3;
+        int64_t off = (int64_t)reg_out_pos<P>(t, i) * 3;
         double2 val = make_double2(scale * v[i].x, scale * v[i].y);
         if constexpr (LS_UPDATE) {
           double2 nv = make_double2(ls_new(e0, v[i].x, scale),
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
         const double *hr = MODE == 2 ? ls.rhs + (row0 + r) * L * C + c : nullptr;
 #pragma unroll
         for (int i = 0; i < V; ++i) {
-          int64_t off = (int64_t)reg_out_pos<L>(t, i) * C;
+          int64_t off = (int64_t)reg_out_pos<P>(t, i) * C;
           double raw = part ? v[i].y : v[i].x;
           double val = scale * raw;
           if constexpr (LS_UPDATE) {
@@ -256,37 +257,38 @@ __global__ void __launch_bounds__(NP * reg_tpp<L>())
 
 // ============================================================ launchers
 
-template <int L, int C>
+template <class P, int C>
 static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
                                       double *y, const LsArgs &la, bool upd,
                                       const SolveState *st, const double2 *W,
                                       hipStream_t s, const Batch *b) {
-  constexpr int NP = reg_np<L>();
+  constexpr int L = P::L;
+  constexpr int NP = reg_np<P>();
   constexpr int RB = 2 * NP / C;
   size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
   int rc = check_lds(lds, "rows_inverse");
   if (rc) return rc;
   const int64_t nbe = (g.rows + RB - 1) / RB;
   const BatchRef br = batch_ref(b, nbe);
-  dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<L>());
+  dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<P>());
   double scale = 1.0 / (double)g.N;
   if (upd) {
     if constexpr (C == 3 || C == 6) {
       if (la.rhs) {
-        set_lds_attr(k_rows_inverse_reg<L, C, NP, 2>, lds);
-        hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 2>), grid, block,
+        set_lds_attr(k_rows_inverse_reg<P, C, NP, 2>, lds);
+        hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 2>), grid, block,
                            lds, s, spec, y, g.rows, scale, W, la, st, br);
       } else {
-        set_lds_attr(k_rows_inverse_reg<L, C, NP, 1>, lds);
-        hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 1>), grid, block,
+        set_lds_attr(k_rows_inverse_reg<P, C, NP, 1>, lds);
+        hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 1>), grid, block,
                            lds, s, spec, y, g.rows, scale, W, la, st, br);
       }
     } else {
       return fail(JX_EINVAL, "fixed point needs sym components");
     }
   } else {
-    set_lds_attr(k_rows_inverse_reg<L, C, NP, 0>, lds);
-    hipLaunchKernelGGL((k_rows_inverse_reg<L, C, NP, 0>), grid, block, lds,
+    set_lds_attr(k_rows_inverse_reg<P, C, NP, 0>, lds);
+    hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 0>), grid, block, lds,
                        s, spec, y, g.rows, scale, W, la, st, br);
   }
   JX_HIP(hipGetLastError());
@@ -296,13 +298,13 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
 static int rows_inverse_reg(const Grid &g, int C, const double *spec, double *y,
                             const LsArgs &la, bool upd, const SolveState *st,
                             const double2 *W, hipStream_t s, const Batch *b) {
-  const int L = (int)g.L;
-  JX_DISPATCH_LC(launch_rows_inverse_reg_lc, g, spec, y, la, upd, st, W, s, b)
+  JX_WITH_PLAN_C((int)g.L, g.plan[PASS_ROWS_INV], launch_rows_inverse_reg_lc,
+                 g, spec, y, la, upd, st, W, s, b);
 }
 
 int rows_inverse_nblocks(const Grid &g, int ncomp) {
-  int RB = reg_rows_per_block((int)g.L, ncomp);
-  if (!RB) RB = rows_per_block((int)g.L, ncomp);
+  if (reg_fft_length((int)g.L)) return (int)reg_rows_max_blocks(g, ncomp);
+  int RB = rows_per_block((int)g.L, ncomp);
   return (int)((g.rows + RB - 1) / RB);
 }
 
@@ -312,7 +314,7 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
   FftLen P;
   int rc = get_fft_len((int)g.L, &P);
   if (rc) return rc;
-  if (int RBr = reg_rows_per_block((int)g.L, ncomp)) {
+  if (int RBr = reg_rows_per_block((int)g.L, ncomp, g.plan[PASS_ROWS_INV])) {
     LsArgs la;
     la.rhs = ls.rhs;
     la.partials = ls.partials;

@@ -86,6 +86,27 @@ class _DiscreteGreenOperator:
     def handle(self):
         return self._handle
 
+    # register FFT plan variant of each pass (no janus counterpart): 0 last-
+    # axis r2c, 1 axis-1 transforms, 2 Gamma-hat pass, 3 last-axis c2r,
+    # 4 fused update pass; variant 0 (16 values per thread) or 1 (8)
+    PLAN_PASSES = ('rows_r2c', 'axis', 'gamma', 'rows_c2r', 'fused')
+
+    def set_plan(self, pass_, variant):
+        """Plan variant of one pass (name or index); results change by
+        rounding only."""
+        p = self.PLAN_PASSES.index(pass_) if isinstance(pass_, str) else pass_
+        _lib.check(_lib.lib().jx_green_set_plan(self._handle, int(p),
+                                                int(variant)))
+
+    def get_plan(self, pass_):
+        p = self.PLAN_PASSES.index(pass_) if isinstance(pass_, str) else pass_
+        return _lib.lib().jx_green_get_plan(self._handle, int(p))
+
+    def plans(self):
+        """{pass name: variant}."""
+        return {name: self.get_plan(i)
+                for i, name in enumerate(self.PLAN_PASSES)}
+
     def __del__(self):
         hdl = getattr(self, '_handle', None)
         if hdl is not None and _lib._lib is not None:

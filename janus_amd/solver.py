@@ -75,6 +75,20 @@ class LSSolver:
     def fused(self, value):
         _lib.check(_lib.lib().jx_solver_set_fused(self._handle, int(bool(value))))
 
+    def autotune(self, iters=5):
+        """Time every pass of this grid with each register FFT plan variant
+        (``iters`` launches each, on scratch fields), keep the fastest in the
+        Green operator and choose the fused or five-pass schedule.
+
+        Returns {pass name: [ms of variant 0, ms of variant 1]} (0 when a
+        variant does not exist or the pass does not run on this grid)."""
+        ms = np.zeros(10)
+        _lib.check(_lib.lib().jx_solver_autotune(
+            self._handle, int(iters), ms.ctypes.data, _lib.current_stream()))
+        names = self.green.PLAN_PASSES
+        return {names[p]: [float(ms[2 * p]), float(ms[2 * p + 1])]
+                for p in range(len(names))}
+
     def solve(self, E, tol=1e-10, max_iter=200, eps=None, rhs=None,
               res_scale=None):
         """Run the fixed point.

@@ -60,6 +60,37 @@ def test_apply_vs_oracle(kind, n, h):
     np.testing.assert_allclose(actual, expected, rtol=0, atol=1e-14 * scale)
 
 
+@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+@pytest.mark.parametrize('n', [(64, 128, 256), (8, 16, 512), (512, 12),
+                               (128, 64), (256, 10, 64)])
+@pytest.mark.parametrize('variant', [0, 1])
+def test_apply_plan_variants_vs_oracle(kind, n, variant):
+    """Every pass on plan variant 0 or 1 (8 values per thread for lengths
+    64 ... 512): the same operator up to rounding."""
+    rng = np.random.default_rng(zlib.crc32(repr((kind, n, 'v')).encode()))
+    dim = len(n)
+    sym = 3 if dim == 2 else 6
+    x = rng.standard_normal(n + (sym,))
+    green = FACTORIES[kind](material.create(MU, NU, dim).green_operator(), n,
+                            1.0, transform=fft.create_real(n))
+    for p in green.PLAN_PASSES:
+        green.set_plan(p, variant)
+    assert set(green.plans().values()) == {variant}
+    actual = np.asarray(green.apply(x))
+    expected = jo.green_apply(x, kind, MU, NU, 1.0)
+    scale = np.abs(expected).max()
+    np.testing.assert_allclose(actual, expected, rtol=0, atol=1e-14 * scale)
+
+
+def test_set_plan_rejects_bad_arguments():
+    green = filtered(material.create(MU, NU, 3).green_operator(), (8, 8, 8),
+                     1.0, transform=fft.create_real((8, 8, 8)))
+    with pytest.raises(ValueError):
+        green.set_plan(5, 0)
+    with pytest.raises(ValueError):
+        green.set_plan('gamma', 2)
+
+
 def test_apply_device_tensors():
     import torch
     n = (10, 12, 14)

@@ -170,3 +170,55 @@ def test_stream_copy_helper():
                                          _lib.current_stream()))
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize('variant', [0, 1])
+def test_plan_variants_match_oracle(variant):
+    """Fused and five-pass schedules on plan variant 0 or 1 (16 or 8 values
+    per thread) against the oracle, iterate for iterate."""
+    n = (32, 64, 128)
+    phase = ms.spheres(32, 4.0, 0.2, seed=5)
+    phase = np.tile(phase, (1, 2, 4))
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(10.0, 0.25, 3)
+    mat0 = material.create(5.5, 0.3, 3)
+    E = np.array([0.3, -0.1, 0.2, 0.0, 0.5, 1.0])
+    eps_ref, res_ref = None, None
+    for fused in (True, False):
+        solver, table = build(phase, [mat_m, mat_i], mat0, 'filtered')
+        assert solver.shape[:3] == n
+        for p in solver.green.PLAN_PASSES:
+            solver.green.set_plan(p, variant)
+        solver.fused = fused
+        r = solver.solve(E, tol=1e-10, max_iter=1000)
+        if eps_ref is None:
+            eps_ref, res_ref = jo.basic_scheme(
+                table[phase, 0], table[phase, 1], E, 'filtered', mat0.g,
+                mat0.nu, tol=1e-10, max_iter=1000)
+        assert r.iterations == len(res_ref)
+        np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-9,
+                                   atol=1e-15)
+        eps = r.eps.cpu().numpy()
+        assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12
+
+
+def test_autotune_keeps_results():
+    n = 64
+    phase = ms.spheres(n, 4.0, 0.2, seed=6)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(20.0, 0.25, 3)
+    mat0 = material.create(10.5, 0.3, 3)
+    solver, table = build(phase, [mat_m, mat_i], mat0, 'filtered')
+    times = solver.autotune(iters=3)
+    assert set(times) == set(solver.green.PLAN_PASSES)
+    for name in ('axis', 'gamma', 'fused'):
+        assert times[name][0] > 0 and times[name][1] > 0
+    E = np.zeros(6)
+    E[5] = 1.0
+    r = solver.solve(E, tol=1e-10, max_iter=1000)
+    eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1], E,
+                                       'filtered', mat0.g, mat0.nu, tol=1e-10,
+                                       max_iter=1000)
+    assert r.iterations == len(res_ref)
+    eps = r.eps.cpu().numpy()
+    assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12

@@ -195,6 +195,33 @@ def test_gamma_hat_brute_force(kind, n):
         np.testing.assert_allclose(a, expected, rtol=1e-13, atol=1e-15)
 
 
+@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+@pytest.mark.parametrize('n', [(16, 16), (15, 8), (16, 12, 10), (9, 7, 6)])
+def test_gamma_hat_axis0_mirror(kind, n):
+    """Gamma-hat(n0 - b0, b1, ...) = S Gamma-hat(b0, b1, ...) S with S = -1
+    on the Mandel components holding axis 0 once (xy in 2D; zx, xy in 3D),
+    for every b0 except 0 and n0 / 2: the relation the GPU Gamma-hat pass
+    uses to evaluate one of each mirror pair."""
+    S = np.diag([1., 1., -1.]) if len(n) == 2 else \
+        np.diag([1., 1., 1., 1., -1., -1.])
+    rest = list(itertools.product(*[range(ni) for ni in n[1:]]))
+    b0 = np.array([b for b in range(1, n[0]) if 2 * b != n[0]])
+    b = np.array([(i,) + r for i in b0 for r in rest])
+    bm = b.copy()
+    bm[:, 0] = n[0] - b[:, 0]
+    g = jo.gamma_hat(kind, b, n, 1.0, 0.75, 0.3)
+    gm = jo.gamma_hat(kind, bm, n, 1.0, 0.75, 0.3)
+    np.testing.assert_allclose(S @ g @ S, gm, rtol=0,
+                               atol=1e-15 * np.abs(gm).max())
+    if n[0] % 2 == 0:
+        # at b0 = n0 / 2 the truncated operator is not mirror-symmetric
+        # (green.pyx:432-445 maps it to +pi / h only)
+        half = np.array([(n[0] // 2,) + r for r in rest])
+        gh = jo.gamma_hat(kind, half, n, 1.0, 0.75, 0.3)
+        if kind == 'truncated':
+            assert np.abs(S @ gh @ S - gh).max() > 1e-3
+
+
 def test_h_cancels():
     """k = 2 pi b / (h n): the truncated and filtered operators do not
     depend on h (green.pyx:140-141, 740-800)."""
