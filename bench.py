@@ -462,6 +462,7 @@ def workload_batched(args):
     ms_run, replays, captures = batch.stats()
     _log('batched: %.3f s (%d replays, %d captures)' % (t_b, replays,
                                                          captures))
+    _log('tuning (width, ms/entry-it) per group: %s' % batch.tuning())
     per_grid = {}
     if args.per_group:
         # each grid size alone: sequential solves vs its own batch
@@ -490,6 +491,8 @@ def workload_batched(args):
             _sync()
             t_g = time.perf_counter() - t0
             its_g = sum(seq[k][0] for k in idx)
+            _log('grid %d tuning (width, ms/entry-it): %s'
+                 % (n, bg.tuning()))
             per_grid[n] = {'entries': len(idx), 'iterations': its_g,
                            'seconds_sequential': round(t_s, 4),
                            'seconds_batched': round(t_g, 4),

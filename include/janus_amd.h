@@ -191,6 +191,11 @@ int jx_batch_add(jx_batch *batch, jx_solver *solver, double *eps,
 int jx_batch_ngroups(jx_batch *batch, int *ngroups);
 int jx_batch_group(jx_batch *batch, int group, int *dim, int64_t *shape,
                    int *size, int *width, int *batched);
+/* Autotuning of the last run for a group: *n candidate widths and their
+ * measured ms per entry-iteration (1e30 when no entry ran), at most cap of
+ * them copied. */
+int jx_batch_tuning(jx_batch *batch, int group, int cap, int *n, int *widths,
+                    double *ms);
 /* Entries per launch of a group (0 = the whole group). */
 int jx_batch_set_width(jx_batch *batch, int group, int width);
 /* iters_per_launch >= 1 (default 8); graph = 1 (default) replays a

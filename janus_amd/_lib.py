@@ -76,6 +76,8 @@ SIGNATURES = [
       _VP]),
     ('jx_batch_ngroups', ctypes.c_int, [_VP, _VP]),
     ('jx_batch_group', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]),
+    ('jx_batch_tuning', ctypes.c_int,
+     [_VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP]),
     ('jx_batch_set_width', ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     ('jx_batch_set_schedule', ctypes.c_int,
      [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

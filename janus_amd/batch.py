@@ -108,6 +108,20 @@ class BatchSolver:
                         bool(batched.value)))
         return out
 
+    def tuning(self):
+        """Per group, the widths the autotuner timed in the last run and
+        their ms per entry-iteration: [[(width, ms), ...], ...]."""
+        lib = _lib.lib()
+        out = []
+        for g in range(len(self.groups())):
+            n = ctypes.c_int(0)
+            w = (ctypes.c_int * 16)()
+            ms = (ctypes.c_double * 16)()
+            _lib.check(lib.jx_batch_tuning(self._handle, g, 16,
+                                           ctypes.byref(n), w, ms))
+            out.append([(w[i], ms[i]) for i in range(min(n.value, 16))])
+        return out
+
     def run(self):
         """Run every entry to its stop; returns [SolveResult] in entry
         order."""

@@ -6,14 +6,17 @@
 // schedule and rhs use, form a group.  A group is cut into chunks of `width`
 // entries; every pass of a chunk is one launch of the batched register
 // kernels (the block index selects the entry's slot), so small grids fill
-// the GPU.  A replay runs `iters_per_launch` iterations of every chunk that
-// still has a running entry, chunk after chunk, so that a chunk's working
-// set stays in cache across its passes and iterations.  The first replay
-// runs eagerly; with autotuning the next replays time the candidate widths
-// of each group (hipEvents around the group's launches) and keep the
-// fastest; the others are captured once in a hipGraph and replayed, and
-// captured again only when a chunk has finished.  The host reads the solve
-// states once per replay.  Each entry runs exactly the kernels and
+// the GPU.  Chunks are cut from the running entries only: the device table
+// of slots holds the running entries group after group and is rebuilt when
+// entries stop, so that converged entries do not keep launching blocks.  A
+// replay runs `iters_per_launch` iterations of every chunk, chunk after
+// chunk, so that a chunk's working set stays in cache across its passes and
+// iterations.  The first replay runs eagerly; with autotuning the next
+// replays time the candidate widths of each group (hipEvents around the
+// group's launches) and keep the fastest; the others are captured in a
+// hipGraph and replayed, and the table and graph are rebuilt once a tenth
+// of the running entries have stopped.  The host reads the solve states
+// once per replay.  Each entry runs exactly the kernels and
 // arithmetic of jx_solver_solve, and its converged blocks skip their work,
 // so the iterates and iteration counts are those of the sequential solves.
 
@@ -40,17 +43,22 @@ struct Entry {
   double tol = 0, res_scale = 0;
   int max_iter = 0;
   double *spec = nullptr, *partials = nullptr, *res_hist = nullptr;
-  int slot = -1;  // position in the slot table of the last run
+  int slot = -1;  // index of its solve state in the last run
   int iters = 0, status = 0;
 };
 
 struct Group {
-  std::vector<int> members;  // entry indices, in slot order
-  int off = 0;               // first slot
+  std::vector<int> members;  // entry indices, in order of addition
+  int off = 0;               // first solve state
+  std::vector<int> act;      // entries in the device table (running)
+  int act_off = 0;           // their first position in the table
   bool batched = false;      // register kernels take the whole chunk
   bool fused = false;
   int width = 0;             // entries per launch (0 = all)
   bool user_width = false;
+  // autotuning of the last run: candidate widths, ms per entry-iteration
+  std::vector<int> tried;
+  std::vector<double> cost;
 };
 
 }  // namespace
@@ -63,6 +71,8 @@ struct jx_batch {
   bool graph = true;
   bool autotune = false;
   // device state of the last run
+  std::vector<Slot> slots;    // host copy of every entry's slot
+  int built = 0;              // entries in the device table
   Slot *tab = nullptr;
   SolveState *st = nullptr;
   SolveState *st_host = nullptr;
@@ -120,20 +130,20 @@ void build_groups(jx_batch *B) {
   }
 }
 
-int chunk_width(const Group &gr, int width) {
-  int n = (int)gr.members.size();
+// Entries per launch over n running entries of a group.
+int chunk_width(const Group &gr, int width, int n) {
   if (!gr.batched) return 1;
-  return (width <= 0 || width > n) ? n : width;
+  return (width <= 0 || width > n) ? std::max(n, 1) : width;
 }
 
-// One iteration of the chunk of group gr starting at member c0 with n
-// entries.
+// One iteration of the chunk of group gr holding its table entries
+// c0 ... c0 + n - 1.
 int enqueue_chunk_iteration(jx_batch *B, const Group &gr, int c0, int n,
                             bool first, hipStream_t s) {
   int rc;
   if (!gr.batched) {
     for (int m = c0; m < c0 + n; ++m) {
-      Entry &e = B->ents[gr.members[m]];
+      Entry &e = B->ents[gr.act[m]];
       RowsInverseLs ls;
       ls.enabled = true;
       ls.rhs = e.rhs;
@@ -146,13 +156,13 @@ int enqueue_chunk_iteration(jx_batch *B, const Group &gr, int c0, int n,
     }
     return JX_OK;
   }
-  const Entry &e0 = B->ents[gr.members[c0]];
+  const Entry &e0 = B->ents[gr.act[c0]];
   const jx_green *G = e0.S->green;
   const Grid &g = G->g;
   const int C = G->sym;
   const LocalDev &loc = e0.S->local->dev;
   Batch b;
-  b.tab = B->tab + gr.off + c0;
+  b.tab = B->tab + gr.act_off + c0;
   b.n = n;
   RowsInverseLs ls;
   ls.enabled = true;
@@ -182,15 +192,22 @@ bool entry_running(const jx_batch *B, int i) {
   return B->st_host[B->ents[i].slot].done == 0;
 }
 
-// iters iterations of every chunk of group gr that has a running entry.
+int count_running(const jx_batch *B) {
+  int n = 0;
+  for (int i = 0; i < (int)B->ents.size(); ++i) n += entry_running(B, i);
+  return n;
+}
+
+// iters iterations of every chunk of group gr's table entries that has a
+// running entry.
 int enqueue_group(jx_batch *B, const Group &gr, int width, int iters,
                   bool first, hipStream_t s) {
-  const int n = (int)gr.members.size();
-  const int w = chunk_width(gr, width);
+  const int n = (int)gr.act.size();
+  const int w = chunk_width(gr, width, n);
   for (int c0 = 0; c0 < n; c0 += w) {
     int cn = std::min(w, n - c0);
     bool running = false;
-    for (int m = c0; m < c0 + cn; ++m) running |= entry_running(B, gr.members[m]);
+    for (int m = c0; m < c0 + cn; ++m) running |= entry_running(B, gr.act[m]);
     if (!running) continue;
     for (int k = 0; k < iters; ++k) {
       int rc = enqueue_chunk_iteration(B, gr, c0, cn, first && k == 0, s);
@@ -200,24 +217,24 @@ int enqueue_group(jx_batch *B, const Group &gr, int width, int iters,
   return JX_OK;
 }
 
-// Chunks with a running entry, as (group, first member) pairs: a new graph
-// is captured when this set changes.
-std::vector<std::pair<int, int>> running_chunks(const jx_batch *B) {
-  std::vector<std::pair<int, int>> out;
-  for (int g = 0; g < (int)B->groups.size(); ++g) {
-    const Group &gr = B->groups[g];
-    const int n = (int)gr.members.size();
-    const int w = chunk_width(gr, gr.width);
-    for (int c0 = 0; c0 < n; c0 += w) {
-      int cn = std::min(w, n - c0);
-      for (int m = c0; m < c0 + cn; ++m)
-        if (entry_running(B, gr.members[m])) {
-          out.emplace_back(g, c0);
-          break;
-        }
-    }
+// Device table of the running entries, group after group.
+int upload_table(jx_batch *B, hipStream_t s) {
+  std::vector<Slot> tab;
+  for (Group &gr : B->groups) {
+    gr.act.clear();
+    gr.act_off = (int)tab.size();
+    for (int i : gr.members)
+      if (entry_running(B, i)) {
+        gr.act.push_back(i);
+        tab.push_back(B->slots[i]);
+      }
   }
-  return out;
+  B->built = (int)tab.size();
+  if (!tab.empty())
+    JX_HIP(hipMemcpyAsync(B->tab, tab.data(), sizeof(Slot) * tab.size(),
+                          hipMemcpyHostToDevice, s));
+  JX_HIP(hipStreamSynchronize(s));
+  return JX_OK;
 }
 
 int read_states(jx_batch *B, hipStream_t s) {
@@ -303,12 +320,12 @@ int run(jx_batch *B, hipStream_t caller) {
   hipStream_t s = B->stream;
   JX_HIP(hipStreamSynchronize(caller));
 
-  std::vector<Slot> tab(n);
+  B->slots.assign(n, Slot());
   for (Group &gr : B->groups)
     for (int m = 0; m < (int)gr.members.size(); ++m) {
       Entry &e = B->ents[gr.members[m]];
       e.slot = gr.off + m;
-      Slot &sl = tab[e.slot];
+      Slot &sl = B->slots[gr.members[m]];
       sl.eps = e.eps;
       sl.spec = e.spec;
       sl.rhs = e.rhs;
@@ -330,16 +347,14 @@ int run(jx_batch *B, hipStream_t caller) {
       st.pad = 0;
       for (int c = 0; c < 6; ++c) st.E[c] = e.E[c];
     }
-  JX_HIP(hipMemcpyAsync(B->tab, tab.data(), sizeof(Slot) * n,
-                        hipMemcpyHostToDevice, s));
   JX_HIP(hipMemcpyAsync(B->st, B->st_host, sizeof(SolveState) * n,
                         hipMemcpyHostToDevice, s));
-  JX_HIP(hipStreamSynchronize(s));
+  int rc = upload_table(B, s);
+  if (rc) return rc;
   B->replays = 0;
   B->captures = 0;
   auto t0 = std::chrono::steady_clock::now();
   const int K = B->iters_per_launch;
-  int rc = JX_OK;
 
   // replay 0: eager, carries pass A of the fused groups
   for (Group &gr : B->groups) {
@@ -363,6 +378,7 @@ int run(jx_batch *B, hipStream_t caller) {
     std::vector<hipEvent_t> ev(2 * ng);
     for (auto &e : ev) JX_HIP(hipEventCreate(&e));
     for (size_t r = 0; r < rounds && !rc; ++r) {
+      if (count_running(B) < B->built && (rc = upload_table(B, s))) break;
       // iterations done by the group's entries before the replay
       std::vector<long long> before(ng, 0);
       for (int g = 0; g < ng; ++g)
@@ -392,6 +408,8 @@ int run(jx_batch *B, hipStream_t caller) {
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
     for (int g = 0; g < ng; ++g) {
+      B->groups[g].tried = cand[g];
+      B->groups[g].cost = cost[g];
       if (cost[g].empty()) continue;
       size_t best = 0;
       for (size_t i = 1; i < cost[g].size(); ++i)
@@ -401,9 +419,16 @@ int run(jx_batch *B, hipStream_t caller) {
     }
   }
 
-  // steady state: captured replays until every entry has stopped
-  std::vector<std::pair<int, int>> live = running_chunks(B);
-  while (!live.empty()) {
+  // steady state: replays until every entry has stopped; the table (and
+  // the graph) is rebuilt once a tenth of its entries have stopped, or at
+  // every change without a graph
+  for (int running = count_running(B); running > 0;
+       running = count_running(B)) {
+    if (running < B->built &&
+        (!B->graph || 10 * running <= 9 * B->built || !B->exec)) {
+      drop_graph(B);
+      if ((rc = upload_table(B, s))) return rc;
+    }
     if (B->graph) {
       if (!B->exec && (rc = capture(B, s))) return rc;
       JX_HIP(hipGraphLaunch(B->exec, s));
@@ -413,9 +438,6 @@ int run(jx_batch *B, hipStream_t caller) {
     }
     if ((rc = read_states(B, s))) return rc;
     ++B->replays;
-    std::vector<std::pair<int, int>> now = running_chunks(B);
-    if (now != live) drop_graph(B);
-    live.swap(now);
   }
   drop_graph(B);
   B->ms = std::chrono::duration<double, std::milli>(
@@ -494,8 +516,11 @@ int jx_batch_ngroups(jx_batch *B, int *ngroups) {
   // keep the widths chosen by the last run
   if (keep.size() == B->groups.size())
     for (size_t g = 0; g < keep.size(); ++g)
-      if (keep[g].members == B->groups[g].members)
+      if (keep[g].members == B->groups[g].members) {
         B->groups[g].width = keep[g].width;
+        B->groups[g].tried = keep[g].tried;
+        B->groups[g].cost = keep[g].cost;
+      }
   *ngroups = (int)B->groups.size();
   return JX_OK;
 }
@@ -511,8 +536,22 @@ int jx_batch_group(jx_batch *B, int g, int *dim, int64_t *shape, int *size,
   if (shape)
     for (int d = 0; d < 3; ++d) shape[d] = d < G.dim ? G.n[d] : 1;
   if (size) *size = (int)gr.members.size();
-  if (width) *width = chunk_width(gr, gr.width);
+  if (width) *width = chunk_width(gr, gr.width, (int)gr.members.size());
   if (batched) *batched = gr.batched ? 1 : 0;
+  return JX_OK;
+}
+
+int jx_batch_tuning(jx_batch *B, int g, int cap, int *n, int *widths,
+                    double *ms) {
+  if (!B || !n) return fail(JX_EINVAL, "jx_batch_tuning: NULL argument");
+  if (g < 0 || g >= (int)B->groups.size())
+    return fail(JX_EINVAL, "jx_batch_tuning: group out of range");
+  const Group &gr = B->groups[g];
+  *n = (int)gr.cost.size();
+  for (int i = 0; i < *n && i < cap; ++i) {
+    if (widths) widths[i] = gr.tried[i];
+    if (ms) ms[i] = gr.cost[i];
+  }
   return JX_OK;
 }
 
