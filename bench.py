@@ -154,27 +154,75 @@ def traffic_file(n, kind):
                         'ls_iteration_%d_%s_traffic.md' % (n, kind))
 
 
-def read_traffic(n, kind, kernel):
-    """HBM bytes per launch of a kernel from the committed counter summary
-    (lines '<tag> <kernel> <bytes>'), or None."""
+def read_traffic(n, kind, kernel, v):
+    """HBM bytes per launch of a pass run with the v-values-per-thread plan,
+    from the committed counter summary (lines '<tag> <pass> <v> <bytes>'),
+    or None."""
     path = traffic_file(n, kind)
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.strip().strip('`').split()
-        if len(f) == 3 and f[0] == TRAFFIC_TAG and f[1] == kernel:
-            return int(f[2])
+        if len(f) == 4 and f[0] == TRAFFIC_TAG and f[1] == kernel and \
+                f[2] == str(v):
+            return int(f[3])
     return None
 
 
-# kernel symbol prefix -> bench pass name (256^3, sym = 6)
-KERNEL_PASS = [('k_rows_update_forward_reg', 'EA_rows_c2r_update_local_r2c'),
-               ('k_rows_forward_reg', 'A_rows_r2c_local'),
-               ('k_rows_inverse_reg', 'E_rows_c2r_update'),
-               ('k_axis_reg<256, 6, 12, -1>', 'B_axis1_fwd'),
-               ('k_axis_reg<256, 6, 12, 1>', 'D_axis1_inv'),
-               ('k_axis_reg<256, 6, 12, 0>', 'C_axis0_gamma'),
-               ('k_stream_copy', 'stream_copy')]
+def plan_v(name):
+    """Values per thread of the plan in a kernel symbol (Plan<L, V, ...>)."""
+    if 'Plan<' not in name:
+        return None
+    return int(name.split('Plan<')[1].split(',')[1])
+
+
+# bench pass -> plan pass of the Green operator
+PASS_PLAN = {'A_rows_r2c_local': 'rows_r2c', 'B_axis1_fwd': 'axis',
+             'C_axis0_gamma': 'gamma', 'D_axis1_inv': 'axis',
+             'E_rows_c2r_update': 'rows_c2r',
+             'EA_rows_c2r_update_local_r2c': 'fused'}
+
+
+def kernel_pass(name):
+    """Bench pass name of a kernel symbol (None for other kernels)."""
+    if 'k_stream_copy' in name:
+        return 'stream_copy'
+    if 'k_rows_update_forward_reg' in name:
+        return 'EA_rows_c2r_update_local_r2c'
+    if 'k_rows_forward_reg' in name:
+        return 'A_rows_r2c_local'
+    if 'k_rows_inverse_reg' in name:
+        return 'E_rows_c2r_update'
+    if 'k_axis_reg' in name:
+        tail = name.split('(')[0].replace(' ', '')
+        if tail.endswith(',-1>'):
+            return 'B_axis1_fwd'
+        if tail.endswith(',1>'):
+            return 'D_axis1_inv'
+        if tail.endswith(',0>'):
+            return 'C_axis0_gamma'
+    return None
+
+
+def stats_summary(stats_csv, cmd):
+    """Markdown table of a rocprofv3 --stats kernel summary (jx kernels)."""
+    import csv
+    out = ['# Kernel durations, 256^3 filtered LS iteration', '',
+           'Command:', '', '    ' + cmd, '',
+           'rocprofv3 kernel statistics (jx kernels; the autotuner also '
+           'launches the variants it does not keep):', '',
+           '| kernel | pass | calls | average us | min us | max us |',
+           '|---|---|---|---|---|---|']
+    for r in csv.DictReader(open(stats_csv)):
+        name = r['Name']
+        if 'jx::' not in name:
+            continue
+        short = name.split('(')[0].replace('void ', '')
+        out.append('| %s | %s | %s | %.1f | %.1f | %.1f |'
+                   % (short, kernel_pass(name) or '', r['Calls'],
+                      float(r['AverageNs']) / 1e3, float(r['MinNs']) / 1e3,
+                      float(r['MaxNs']) / 1e3))
+    return '\n'.join(out) + '\n'
 
 
 def traffic_summary(fetch_csv, write_csv, cmd):
@@ -215,10 +263,10 @@ def traffic_summary(fetch_csv, write_csv, cmd):
         out.append('| %s | %.1f | %.1f | %.1f | %.1f | %.1f |'
                    % (short, f / 1e6, w / 1e6, fr / 1e6, wr / 1e6,
                       (fr + wr) / 1e6))
-        for pre, pname in KERNEL_PASS:
-            if pre in name:
-                tags.append('    %s %s %d' % (TRAFFIC_TAG, pname, int(fr + wr)))
-                break
+        pname = kernel_pass(name)
+        if pname:
+            tags.append('    %s %s %s %d' % (TRAFFIC_TAG, pname, plan_v(name),
+                                            int(fr + wr)))
     out += ['', 'Traffic per launch (bytes) read by bench.py:', ''] + tags
     return '\n'.join(out) + '\n'
 
@@ -822,6 +870,9 @@ def main():
     ap.add_argument('--adjoint-reps', type=int, default=2)
     ap.add_argument('--fd-voxels', type=int, default=20)
     ap.add_argument('--large-iters', type=int, default=20)
+    ap.add_argument('--stats-summary', nargs=2, metavar=('STATS_CSV', 'COMMAND'),
+                    help='print the kernel table of a rocprofv3 --stats run '
+                         '(no GPU work)')
     ap.add_argument('--traffic-summary', nargs=3,
                     metavar=('FETCH_CSV', 'WRITE_CSV', 'COMMAND'),
                     help='print the counter summary of two rocprofv3 --pmc '
@@ -829,6 +880,9 @@ def main():
     args = ap.parse_args()
     if args.traffic_summary:
         print(traffic_summary(*args.traffic_summary), end='')
+        return
+    if args.stats_summary:
+        print(stats_summary(*args.stats_summary), end='')
         return
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only; the multi-GPU '
@@ -947,7 +1001,10 @@ def main():
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4),
-            'traffic': read_traffic(n, args.kind, names[dom]),
+            'traffic': read_traffic(
+                n, args.kind, names[dom],
+                16 if solver.green.get_plan(PASS_PLAN[names[dom]]) == 0
+                else 8),
         },
         'cpu_baseline': cpu,
     }

@@ -24,6 +24,8 @@ __global__ void __launch_bounds__(kFinalizeThreads)
   // the summation order is fixed, so the result is deterministic
   double acc[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   const int stride = kFinalizeThreads * 8;
+  // four strides per unrolled step: 32 loads in flight per thread
+#pragma unroll 4
   for (int i0 = 0; i0 < nb; i0 += stride) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
