@@ -132,6 +132,23 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
                     const double *rhs, double tol, int max_iter,
                     double res_scale, double *res_hist, int *iters,
                     int *status, void *stream);
+/* Conjugate gradients on the same operators (no janus counterpart in the
+ * library; janus's examples/square_inclusion.py:40-47, 71-86 drives PETSc CG
+ * on the polarization form).  Solves (I + Gamma0 dC) x = -Gamma0[dC : E]
+ * for the fluctuation x = eps - E in the inner product sum a : C0 : b, in
+ * which the operator is self-adjoint (truncated operator; close to it for
+ * the filtered one) and positive definite for any isotropic C0 when C is:
+ * it converges where the basic scheme diverges (C0 = the matrix).  The
+ * residual after iteration k is ||b - A x_k|| / (sqrt(N) ||E||) (res_scale
+ * overrides the 1 / (sqrt(N) ||E||)), the basic scheme's own residual
+ * definition (square_basic.py:86, 91); the solve stops after the first
+ * iteration below tol.  eps: device field, output only (the iteration
+ * starts from E).  Other arguments as jx_solver_solve.  Allocates three
+ * fields of the grid on first use.  Synchronizes the stream. */
+int jx_solver_solve_cg(jx_solver *solver, double *eps, const double *E,
+                       double tol, int max_iter, double res_scale,
+                       double *res_hist, int *iters, int *status,
+                       void *stream);
 /* Schedule choice (no janus counterpart).  fused = 1 (default): the last
  * pass of an iteration also applies the local operator to the new iterate
  * and transforms it for the next iteration, so the iterate is written once

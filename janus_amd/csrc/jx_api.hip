@@ -511,6 +511,9 @@ int jx_solver_destroy(jx_solver *S) {
   if (S->st_host) (void)hipHostFree(S->st_host);
   if (S->partials) (void)hipFree(S->partials);
   if (S->res_dev) (void)hipFree(S->res_dev);
+  if (S->cg_buf) (void)hipFree(S->cg_buf);
+  if (S->cg_partials) (void)hipFree(S->cg_partials);
+  if (S->cg_s) (void)hipFree(S->cg_s);
   delete S;
   return JX_OK;
 }

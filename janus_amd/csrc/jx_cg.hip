@@ -1,0 +1,325 @@
+// Conjugate gradients on the Lippmann-Schwinger operator (no janus
+// counterpart in the library; janus drives Krylov solvers from its
+// examples, examples/square_inclusion.py:40-47, 71-86 with PETSc CG on the
+// polarization form).
+//
+// Strain form: the fluctuation x = eps - E solves (I + G dC) x = -G[dC : E]
+// with G = Gamma0, in the inner product <a, b> = sum_voxels a : C0 : b.
+// G C0 is the C0-orthogonal projector onto compatible zero-mean fields for
+// the truncated operator (exactly on odd grids), so the operator is
+// self-adjoint and positive definite there whenever C = C0 + dC is, for any
+// isotropic C0 -- including C0 = C_matrix, where the basic scheme diverges.
+// For the filtered operator it is close to self-adjoint and CG converges in
+// practice.  The residual after iteration k is ||r_k|| / (sqrt(N) ||E||),
+// which for the basic scheme is its own residual ||eps_{k+1} - eps_k||.
+//
+// One iteration: q = A p through the five passes of the Green operator
+// (pass A applies dC to p) and four vector kernels with deterministic
+// per-block partial sums; alpha, beta and the residual are computed on the
+// device by a one-block kernel, and every kernel returns at once when the
+// device state says the solve is done.
+
+#include "jx_device.h"
+#include "jx_handles.h"
+
+namespace jx {
+
+namespace {
+
+enum CgMode { CG_FILL, CG_INIT, CG_QP, CG_XR, CG_P, CG_FINAL };
+enum CgPhase { PH_INIT, PH_ALPHA, PH_BETA };
+
+constexpr int kCgBlocks = 1024;  // blocks of the vector kernels (partials)
+constexpr int kScalarThreads = 1024;
+
+// a : C0 : b of one voxel, C0 = dim k0 J + 2 g0 K in Mandel notation:
+// wd (a . b) + wt tr(a) tr(b) with wd = 2 g0, wt = k0 - 2 g0 / dim.
+template <int C>
+__device__ __forceinline__ double c0_dot(const double *a, const double *b,
+                                         double wd, double wt) {
+  constexpr int D = C == 3 ? 2 : 3;
+  double dot = 0., ta = 0., tb = 0.;
+#pragma unroll
+  for (int c = 0; c < C; ++c) dot += a[c] * b[c];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    ta += a[c];
+    tb += b[c];
+  }
+  return wd * dot + wt * ta * tb;
+}
+
+template <int C, int MODE>
+__global__ void __launch_bounds__(kThreads)
+    k_cg_vec(double *x, double *r, double *p, double *q, int64_t N, double wd,
+             double wt, const CgScalars *cs, const SolveState *st,
+             double *partials) {
+#pragma clang fp contract(off)
+  // the fill and the final sum run whatever the state
+  if (MODE != CG_FILL && MODE != CG_FINAL && solve_done(st)) return;
+  __shared__ double red[kThreads / 64];
+  double acc0 = 0., acc1 = 0.;
+  const double alpha = MODE == CG_XR ? cs->alpha : 0.;
+  const double beta = MODE == CG_P ? cs->beta : 0.;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = v * C;
+    if constexpr (MODE == CG_FILL) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) x[o + c] = st->E[c];
+    } else if constexpr (MODE == CG_FINAL) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) x[o + c] = st->E[c] + x[o + c];
+    } else if constexpr (MODE == CG_INIT) {
+      // r = -G[dC : E] (in q), p = r, x = 0
+      double rv[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        rv[c] = -q[o + c];
+        r[o + c] = rv[c];
+        p[o + c] = rv[c];
+        x[o + c] = 0.;
+      }
+      acc0 += c0_dot<C>(rv, rv, wd, wt);
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc1 += rv[c] * rv[c];
+    } else if constexpr (MODE == CG_QP) {
+      // q = p + G[dC : p]
+      double pv[C], qv[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        pv[c] = p[o + c];
+        qv[c] = pv[c] + q[o + c];
+        q[o + c] = qv[c];
+      }
+      acc0 += c0_dot<C>(pv, qv, wd, wt);
+    } else if constexpr (MODE == CG_XR) {
+      double rv[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        double ap = alpha * p[o + c];
+        double aq = alpha * q[o + c];
+        x[o + c] = x[o + c] + ap;
+        rv[c] = r[o + c] - aq;
+        r[o + c] = rv[c];
+      }
+      acc0 += c0_dot<C>(rv, rv, wd, wt);
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc1 += rv[c] * rv[c];
+    } else {  // CG_P
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        double bp = beta * p[o + c];
+        p[o + c] = r[o + c] + bp;
+      }
+    }
+  }
+  if constexpr (MODE == CG_INIT || MODE == CG_QP || MODE == CG_XR) {
+    double s0 = block_sum(acc0, red);
+    __syncthreads();
+    double s1 = block_sum(acc1, red);
+    if (threadIdx.x == 0) {
+      partials[2 * blockIdx.x] = s0;
+      partials[2 * blockIdx.x + 1] = s1;
+    }
+  }
+}
+
+// Fixed-order sum of column `col` of the partials (nb rows of 2); valid in
+// thread 0.
+__device__ double sum_partials(const double *partials, int nb, int col,
+                               double *red) {
+  double acc = 0.;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[2 * i + col];
+  return block_sum(acc, red);
+}
+
+template <int PHASE>
+__global__ void __launch_bounds__(kScalarThreads)
+    k_cg_scalar(const double *partials, int nb, CgScalars *cs, SolveState *st,
+                double *res_hist) {
+  if (solve_done(st)) return;
+  __shared__ double red[kScalarThreads / 64];
+  double s0 = sum_partials(partials, nb, 0, red);
+  __syncthreads();
+  double s1 = sum_partials(partials, nb, 1, red);
+  if (threadIdx.x != 0) return;
+  if constexpr (PHASE == PH_INIT) {
+    cs->rr = s0;
+    // homogeneous medium: the initial iterate E is the solution
+    if (s0 == 0.) st->done = JX_SOLVE_CONVERGED;
+  } else if constexpr (PHASE == PH_ALPHA) {
+    cs->alpha = s0 != 0. ? cs->rr / s0 : 0.;
+  } else {
+    double res = sqrt(s1) * st->res_scale;
+    int it = st->iter;
+    res_hist[it] = res;
+    st->iter = it + 1;
+    if (!isfinite(res))
+      st->done = JX_SOLVE_NONFINITE;
+    else if (res < st->tol)
+      st->done = JX_SOLVE_CONVERGED;
+    else if (it + 1 >= st->max_iter)
+      st->done = JX_SOLVE_MAXITER;
+    cs->beta = cs->rr != 0. ? s0 / cs->rr : 0.;
+    cs->rr = s0;
+  }
+}
+
+template <int C, int MODE>
+int vec(double *x, double *r, double *p, double *q, const Grid &g, double wd,
+        double wt, const CgScalars *cs, const SolveState *st,
+        double *partials, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_vec<C, MODE>), dim3(kCgBlocks), dim3(kThreads), 0,
+                     s, x, r, p, q, g.N, wd, wt, cs, st, partials);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+template <int PHASE>
+int scalar(const double *partials, CgScalars *cs, SolveState *st,
+           double *res_hist, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_scalar<PHASE>), dim3(1), dim3(kScalarThreads), 0, s,
+                     partials, kCgBlocks, cs, st, res_hist);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+// y = G[dC : x] (five passes, the last one plain: scaled c2r).
+int matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
+           hipStream_t s) {
+  jx_green *G = S->green;
+  const Grid &g = G->g;
+  const int C = G->sym;
+  int rc = launch_rows_forward(g, C, x, G->spec, S->local->dev, st, s);
+  if (rc) return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, st, s)))
+    return rc;
+  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, st, s)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, st, s)))
+    return rc;
+  RowsInverseLs plain;
+  return launch_rows_inverse(g, C, G->spec, y, plain, st, nullptr, s);
+}
+
+template <int C>
+int cg_iteration(jx_solver *S, double *x, double wd, double wt, hipStream_t s) {
+  const Grid &g = S->green->g;
+  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
+  int rc;
+  if ((rc = matvec(S, p, q, S->st, s))) return rc;
+  if ((rc = vec<C, CG_QP>(x, r, p, q, g, wd, wt, S->cg_s, S->st,
+                          S->cg_partials, s)))
+    return rc;
+  if ((rc = scalar<PH_ALPHA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  if ((rc = vec<C, CG_XR>(x, r, p, q, g, wd, wt, S->cg_s, S->st,
+                          S->cg_partials, s)))
+    return rc;
+  if ((rc = scalar<PH_BETA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  return vec<C, CG_P>(x, r, p, q, g, wd, wt, S->cg_s, S->st, S->cg_partials,
+                      s);
+}
+
+template <int C>
+int cg_solve(jx_solver *S, double *eps, int max_iter, double wd, double wt,
+             hipStream_t s) {
+  const Grid &g = S->green->g;
+  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
+  int rc;
+  // q = G[dC : E], then r = p = -q, x = 0 (x is eps)
+  if ((rc = vec<C, CG_FILL>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
+                            S->cg_partials, s)))
+    return rc;
+  if ((rc = matvec(S, eps, q, S->st, s))) return rc;
+  if ((rc = vec<C, CG_INIT>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
+                            S->cg_partials, s)))
+    return rc;
+  if ((rc = scalar<PH_INIT>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  int enqueued = 0, chunk = 4;
+  while (true) {
+    int n = std::min(chunk, max_iter - enqueued);
+    for (int i = 0; i < n; ++i)
+      if ((rc = cg_iteration<C>(S, eps, wd, wt, s))) return rc;
+    enqueued += n;
+    JX_HIP(hipMemcpyAsync(S->st_host, S->st, sizeof(SolveState),
+                          hipMemcpyDeviceToHost, s));
+    JX_HIP(hipStreamSynchronize(s));
+    if (S->st_host->done || enqueued >= max_iter) break;
+    chunk = std::min(chunk * 2, 64);
+  }
+  // eps = E + x
+  return vec<C, CG_FINAL>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
+                          S->cg_partials, s);
+}
+
+}  // namespace
+
+}  // namespace jx
+
+using namespace jx;
+
+extern "C" {
+
+int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E, double tol,
+                       int max_iter, double res_scale, double *res_hist,
+                       int *iters, int *status, void *stream) {
+  if (!S || !eps || !E) return fail(JX_EINVAL, "jx_solver_solve_cg: NULL argument");
+  if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  jx_green *G = S->green;
+  const Grid &g = G->g;
+  const int C = G->sym;
+  const int dim = g.dim;
+  if (res_scale <= 0.) {
+    double nE = 0.;
+    for (int c = 0; c < C; ++c) nE += E[c] * E[c];
+    nE = std::sqrt(nE);
+    if (nE == 0.) return fail(JX_EINVAL, "res_scale must be given when E = 0");
+    res_scale = 1.0 / std::sqrt((double)g.N) / nE;  // square_basic.py:86
+  }
+  // buffers: r, p, q fields, partial sums, scalars, residual history
+  if (!S->cg_buf) {
+    JX_HIP(hipMalloc((void **)&S->cg_buf, sizeof(double) * 3 * C * g.N));
+    JX_HIP(hipMalloc((void **)&S->cg_partials, sizeof(double) * 2 * kCgBlocks));
+    JX_HIP(hipMalloc((void **)&S->cg_s, sizeof(CgScalars)));
+  }
+  if (S->res_cap < max_iter) {
+    if (S->res_dev) (void)hipFree(S->res_dev);
+    S->res_dev = nullptr;
+    S->res_cap = 0;
+    JX_HIP(hipMalloc((void **)&S->res_dev, sizeof(double) * max_iter));
+    S->res_cap = max_iter;
+  }
+  SolveState init = {};
+  init.tol = tol;
+  init.res_scale = res_scale;
+  init.max_iter = max_iter;
+  for (int c = 0; c < 6; ++c) init.E[c] = c < C ? E[c] : 0.;
+  *S->st_host = init;
+  JX_HIP(hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState),
+                        hipMemcpyHostToDevice, s));
+  // C0 inner product weights: 2 g0 (a . b) + (k0 - 2 g0 / dim) tr a tr b
+  const double g0 = G->g0, nu0 = G->nu0;
+  const double k0 = dim == 2 ? g0 / (1. - 2. * nu0)
+                             : 2. / 3. * (1. + nu0) / (1. - 2. * nu0) * g0;
+  const double wd = 2. * g0, wt = k0 - 2. * g0 / dim;
+  int rc = C == 3 ? cg_solve<3>(S, eps, max_iter, wd, wt, s)
+                  : cg_solve<6>(S, eps, max_iter, wd, wt, s);
+  if (rc) return rc;
+  JX_HIP(hipStreamSynchronize(s));
+  int it = S->st_host->iter;
+  if (iters) *iters = it;
+  if (status) *status = S->st_host->done ? S->st_host->done : JX_SOLVE_MAXITER;
+  if (res_hist && it > 0)
+    JX_HIP(hipMemcpy(res_hist, S->res_dev, sizeof(double) * it,
+                     hipMemcpyDeviceToHost));
+  return JX_OK;
+}
+
+}  // extern "C"

@@ -30,6 +30,15 @@ struct jx_local {
   double *partials = nullptr;  // device, 1024 blocks x 2 sym columns + 2 sym
 };
 
+namespace jx {
+// Scalars of a conjugate-gradient solve, kept on the device.
+struct CgScalars {
+  double rr;     // <r, r> in the C0 inner product
+  double alpha;  // step of the current iteration
+  double beta;   // direction update of the current iteration
+};
+}  // namespace jx
+
 struct jx_solver {
   jx_green *green = nullptr;
   jx_local *local = nullptr;
@@ -40,6 +49,11 @@ struct jx_solver {
   int nblocks = 0;
   double *res_dev = nullptr;
   int res_cap = 0;
+  // conjugate gradients (allocated on first use): r, p, q fields, block
+  // partial sums, scalars
+  double *cg_buf = nullptr;
+  double *cg_partials = nullptr;
+  jx::CgScalars *cg_s = nullptr;
 };
 
 namespace jx {

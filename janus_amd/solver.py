@@ -135,6 +135,41 @@ class LSSolver:
         n = iters.value
         return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
 
+    def solve_cg(self, E, tol=1e-10, max_iter=200, eps=None, res_scale=None):
+        """Conjugate gradients on the same operators: the fluctuation
+        eps - E solves (I + Gamma0 dC) x = -Gamma0[dC : E] in the inner
+        product sum a : C0 : b.  Converges for any isotropic C0 with C
+        positive definite, including C0 = the matrix phase (the basic scheme
+        diverges there).  Residual and stopping rule as :meth:`solve`.
+
+        Args:
+            E: macroscopic strain (sym,), Mandel notation.
+            eps: optional float64 GPU tensor of shape ``grid + (sym,)``
+                receiving the solution (its content is not used).
+        Returns:
+            SolveResult(eps, residuals, iterations, status).
+        """
+        t = _device.torch()
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        if E.shape != (self.sym,):
+            raise ValueError('E must have shape ({0},)'.format(self.sym))
+        if eps is None:
+            eps = t.empty(self.shape, dtype=t.float64, device='cuda')
+        elif not (_device.is_device_tensor(eps) and eps.is_contiguous() and
+                  tuple(eps.shape) == self.shape and eps.dtype == t.float64):
+            raise ValueError('eps must be a contiguous float64 GPU tensor of '
+                             'shape {0}'.format(self.shape))
+        hist = np.zeros(max_iter, dtype=np.float64)
+        iters = ctypes.c_int(0)
+        status = ctypes.c_int(0)
+        _lib.check(_lib.lib().jx_solver_solve_cg(
+            self._handle, eps.data_ptr(), E.ctypes.data, float(tol),
+            int(max_iter), float(res_scale or 0.0), hist.ctypes.data,
+            ctypes.byref(iters), ctypes.byref(status),
+            _lib.current_stream()))
+        n = iters.value
+        return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
+
     def mean_stress(self, eps, E, mat0):
         """<sigma> = C0 : E + <(C - C0) : eps> (square_basic.rst:212-221)."""
         tau, _ = self.local.means(eps)

@@ -25,6 +25,10 @@ rfft / irfft                 fft/serial/_serial_fft.pyx:152-177, 296-325     num
                                                                              oracle's FFT *is* numpy's pocketfft
 adjoint_gradient             (no janus counterpart) implicit                 central finite differences of basic_scheme
                              differentiation of the fixed point
+cg_scheme                    examples/square_inclusion.py:40-47, 71-86       converged solution solves the Lippmann-
+                             (Krylov on the same operators; strain form      Schwinger equation and equals the basic
+                             in the C0 inner product, no janus counterpart)  scheme's; C0-conjugacy of the directions;
+                                                                             dense direct solve on a 4x4 grid
 ===========================  ==============================================  ==========================================
 """
 
@@ -420,3 +424,71 @@ def _field_rhs_scheme(sph, dev, rhs, kind, g0, nu0, tol, max_iter, h):
         if r < tol:
             break
     return mu, np.array(res)
+
+
+# --------------------------------------------------------------------------
+# Conjugate gradients (janus drives Krylov solvers from its examples:
+# examples/square_inclusion.py:40-47, 71-86 runs PETSc CG on the polarization
+# form; this is CG on the strain form, which also holds for C0 = C_matrix)
+# --------------------------------------------------------------------------
+
+def c0_inner(a, b, g0, nu0):
+    """Sum over voxels of a : C0 : b in Mandel notation, C0 = dim k0 J +
+    2 g0 K: 2 g0 a.b + (k0 - 2 g0 / dim) tr(a) tr(b)."""
+    dim = 2 if a.shape[-1] == 3 else 3
+    k0 = bulk_modulus(g0, nu0, dim)
+    tra = a[..., :dim].sum(axis=-1)
+    trb = b[..., :dim].sum(axis=-1)
+    return 2. * g0 * np.sum(a * b) + (k0 - 2. * g0 / dim) * np.sum(tra * trb)
+
+
+def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
+              directions=None):
+    """Conjugate gradients on (I + Gamma0 dC) eps~ = -Gamma0[dC : E] for the
+    fluctuation eps~ (eps = E + eps~), in the inner product
+    <a, b> = sum a : C0 : b.
+
+    Gamma0 C0 is the C0-orthogonal projector onto compatible zero-mean
+    fields, so I + Gamma0 dC is self-adjoint there and positive definite
+    whenever C = C0 + dC is, whatever the isotropic C0 (the basic scheme
+    needs C0 near the middle of the phases).  The residual after iteration k
+    is ||r_k|| / (sqrt(N) ||E||) with r_k = b - A eps~_k: for the basic
+    scheme this quantity is its own residual ||eps_{k+1} - eps_k|| (same
+    normalization, square_basic.py:86, 91).  Stops after the first
+    iteration whose residual is < tol.  ``directions``: optional list
+    receiving the search directions (for tests).  Returns (eps, residuals).
+    """
+    E = np.asarray(E, dtype=np.float64)
+    shape = np.shape(sph)
+    N = int(np.prod(shape))
+    nE = np.linalg.norm(E)
+    normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
+
+    def op(v):
+        return v + green_apply(iso4_apply(sph, dev, v), kind, g0, nu0, h)
+
+    Ef = np.empty(shape + (E.size,), dtype=np.float64)
+    Ef[...] = E
+    r = -green_apply(iso4_apply(sph, dev, Ef), kind, g0, nu0, h)
+    x = np.zeros_like(r)
+    p = r.copy()
+    rr = c0_inner(r, r, g0, nu0)
+    res = []
+    if rr == 0.:
+        return Ef, np.zeros(1)
+    for _ in range(max_iter):
+        if directions is not None:
+            directions.append(p.copy())
+        q = op(p)
+        alpha = rr / c0_inner(p, q, g0, nu0)
+        x = x + alpha * p
+        r = r - alpha * q
+        res_k = normalization * np.linalg.norm(r)
+        res.append(res_k)
+        if res_k < tol:
+            break
+        rr_new = c0_inner(r, r, g0, nu0)
+        beta = rr_new / rr
+        p = r + beta * p
+        rr = rr_new
+    return Ef + x, np.array(res)

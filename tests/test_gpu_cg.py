@@ -1,0 +1,125 @@
+"""Conjugate gradients on the GPU (jx_solver_solve_cg) vs the oracle's
+cg_scheme, iterate for iterate, and on the configurations where the basic
+scheme diverges (C0 = the matrix phase)."""
+
+import numpy as np
+import pytest
+
+import janus_amd.fft.serial as fft
+import janus_amd.material.elastic.linear.isotropic as material
+from janus_amd import microstructure as ms
+from janus_amd.green import filtered, truncated
+from janus_amd.operators import BlockDiagonalOperator2D, \
+    BlockDiagonalOperator3D
+from janus_amd.solver import LSSolver, delta_stiffness
+from oracle import janus_oracle as jo
+
+pytestmark = pytest.mark.gpu
+
+FACTORIES = {'truncated': truncated, 'filtered': filtered}
+
+
+def build(phase, mats, mat0, kind):
+    dim = phase.ndim
+    table = np.array([delta_stiffness(m, mat0) for m in mats])
+    cls = BlockDiagonalOperator2D if dim == 2 else BlockDiagonalOperator3D
+    local = cls.from_phases(phase, table)
+    n = phase.shape
+    green = FACTORIES[kind](mat0.green_operator(), n, 1.0, fft.create_real(n))
+    return LSSolver(green, local), table
+
+
+CASES = [
+    # dim, n, kind, mu_i, g0 (1.0 = the matrix: the basic scheme diverges)
+    (2, 32, 'truncated', 10.0, 1.0),
+    (2, 33, 'filtered', 20.0, 1.0),
+    (2, 31, 'truncated', 10.0, 5.5),
+    (3, 16, 'filtered', 50.0, 1.0),
+    (3, 15, 'truncated', 5.0, 3.0),
+    (3, 32, 'filtered', 10.0, 0.9),
+]
+
+
+@pytest.mark.parametrize('dim, n, kind, mu_i, g0', CASES)
+def test_cg_matches_oracle(dim, n, kind, mu_i, g0):
+    if dim == 2:
+        phase = ms.square_inclusion(n, 0.5)
+    else:
+        phase = ms.spheres(n, 3.0, 0.2, seed=3)
+    mat_m = material.create(1.0, 0.3, dim)
+    mat_i = material.create(mu_i, 0.25, dim)
+    mat0 = material.create(g0, 0.3, dim)
+    solver, table = build(phase, [mat_m, mat_i], mat0, kind)
+    rng = np.random.default_rng(n)
+    E = rng.standard_normal(solver.sym)
+    r = solver.solve_cg(E, tol=1e-10, max_iter=500)
+    assert r.status == 'converged'
+    eps_ref, res_ref = jo.cg_scheme(table[phase, 0], table[phase, 1], E, kind,
+                                    g0, 0.3, tol=1e-10, max_iter=500)
+    assert r.iterations == len(res_ref)
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-8, atol=1e-15)
+    eps = r.eps.cpu().numpy()
+    err = np.abs(eps - eps_ref).max() / np.abs(eps_ref).max()
+    assert err < 1e-12, err
+
+
+def test_cg_converges_where_basic_scheme_diverges():
+    """2D 256^2 square inclusion, contrast 10, truncated, C0 = matrix: the
+    basic scheme diverges; CG converges to a field that satisfies that C0's
+    Lippmann-Schwinger equation.  (The discrete solution depends on C0, so
+    solutions are compared at equal C0: CG and the basic scheme at the
+    midpoint C0.)"""
+    import torch
+    n = 256
+    phase = ms.square_inclusion(n, 0.5)
+    mat_m = material.create(1.0, 0.3, 2)
+    mat_i = material.create(10.0, 0.3, 2)
+    E = np.array([0., 0., 1.])
+    s_mat, _ = build(phase, [mat_m, mat_i], material.create(1.0, 0.3, 2),
+                     'truncated')
+    div = s_mat.solve(E, tol=1e-10, max_iter=50)
+    assert div.status != 'converged' and div.residuals[-1] > 1e3
+    cg = s_mat.solve_cg(E, tol=1e-10, max_iter=500)
+    assert cg.status == 'converged'
+    # eps = E - Gamma0[dC : eps] for C0 = matrix
+    tau = s_mat.local.apply(cg.eps)
+    g = s_mat.green.apply(torch.as_tensor(tau).contiguous())
+    ls = cg.eps - (torch.from_numpy(E).cuda() - torch.as_tensor(g))
+    assert float(ls.abs().max()) < 1e-8
+    s_mid, _ = build(phase, [mat_m, mat_i], material.create(5.5, 0.3, 2),
+                     'truncated')
+    basic = s_mid.solve(E, tol=1e-12, max_iter=2000)
+    cg_mid = s_mid.solve_cg(E, tol=1e-12, max_iter=500)
+    assert basic.status == 'converged' and cg_mid.status == 'converged'
+    assert cg.iterations < basic.iterations / 2
+    assert float((cg_mid.eps - basic.eps).abs().max()) < 1e-9
+
+
+def test_cg_homogeneous_medium_returns_E():
+    n = 16
+    phase = np.zeros((n, n, n), dtype=np.uint8)
+    m = material.create(2.0, 0.3, 3)
+    solver, _ = build(phase, [m, m], m, 'filtered')
+    E = np.arange(1., 7.)
+    r = solver.solve_cg(E, tol=1e-10)
+    assert r.status == 'converged' and r.iterations == 0
+    np.testing.assert_array_equal(r.eps.cpu().numpy().reshape(-1, 6),
+                                  np.tile(E, (n ** 3, 1)))
+
+
+def test_cg_plan_variants_agree():
+    phase = ms.spheres(64, 5.0, 0.2, seed=9)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(30.0, 0.25, 3)
+    mat0 = material.create(1.0, 0.3, 3)
+    E = np.zeros(6)
+    E[5] = 1.0
+    out = []
+    for v in (0, 1):
+        solver, _ = build(phase, [mat_m, mat_i], mat0, 'filtered')
+        for p in solver.green.PLAN_PASSES:
+            solver.green.set_plan(p, v)
+        r = solver.solve_cg(E, tol=1e-10, max_iter=500)
+        assert r.status == 'converged'
+        out.append(r.eps.cpu().numpy())
+    assert np.abs(out[0] - out[1]).max() < 1e-9

@@ -317,3 +317,104 @@ def test_adjoint_gradient_finite_differences():
         dm[i, j] -= h
         fd = (loss(sph, dp)[0] - loss(sph, dm)[0]) / (2 * h)
         assert abs(fd - ddev[i, j]) < 1e-7 * max(1., abs(fd))
+
+
+# ---------------------------------------------------------------- CG
+
+def _cg_setup(n, g0, kind='truncated'):
+    from janus_amd import microstructure as ms
+    phase = ms.square_inclusion(n, 0.5)
+    t = np.array([jo.delta_c_coefficients(g, 0.3, g0, 0.3, 2)
+                  for g in (1.0, 10.0)])
+    return t[phase, 0], t[phase, 1]
+
+
+@pytest.mark.parametrize('g0', [1.0, 0.9, 5.5, 12.0])
+def test_cg_solves_lippmann_schwinger(g0):
+    """The CG solution satisfies eps = E - Gamma0[dC : eps] for every C0,
+    including C0 = C_matrix where the basic scheme diverges."""
+    sph, dev = _cg_setup(16, g0)
+    E = np.array([0.1, -0.3, 1.0])
+    eps, res = jo.cg_scheme(sph, dev, E, 'truncated', g0, 0.3, tol=1e-12,
+                            max_iter=500)
+    assert res[-1] < 1e-12 and len(res) < 60
+    ls = eps - (E - jo.green_apply(jo.iso4_apply(sph, dev, eps), 'truncated',
+                                   g0, 0.3))
+    assert np.abs(ls).max() < 1e-10
+    if g0 == 5.5:
+        eb, rb = jo.basic_scheme(sph, dev, E, 'truncated', g0, 0.3,
+                                 tol=1e-13, max_iter=3000)
+        assert np.abs(eb - eps).max() < 1e-10
+
+
+@pytest.mark.parametrize('g0', [1.0, 5.5])
+def test_cg_solves_lippmann_schwinger_filtered(g0):
+    sph, dev = _cg_setup(16, g0)
+    E = np.array([0.1, -0.3, 1.0])
+    eps, res = jo.cg_scheme(sph, dev, E, 'filtered', g0, 0.3, tol=1e-12,
+                            max_iter=500)
+    assert res[-1] < 1e-12 and len(res) < 60
+    ls = eps - (E - jo.green_apply(jo.iso4_apply(sph, dev, eps), 'filtered',
+                                   g0, 0.3))
+    assert np.abs(ls).max() < 1e-10
+
+
+def test_cg_directions_are_conjugate():
+    """<p_i, A p_j>_C0 = 0 for i != j over the first directions, truncated
+    operator on an odd grid, where Gamma0 C0 is exactly the C0-orthogonal
+    projector onto compatible fields.  (On even grids janus's truncated
+    operator maps b and -b at an axis-0 Nyquist index to (pi, k1) and
+    (pi, -k1), green.pyx:432-445; the filtered operator is a weighted sum of
+    projectors, not a projector.  A is then only close to C0-self-adjoint:
+    CG still converges, test_cg_solves_lippmann_schwinger_filtered.)"""
+    g0 = 1.0
+    sph, dev = _cg_setup(11, g0)
+    dirs = []
+    jo.cg_scheme(sph, dev, np.array([0., 0., 1.]), 'truncated', g0, 0.3,
+                 max_iter=6, directions=dirs)
+
+    def op(v):
+        return v + jo.green_apply(jo.iso4_apply(sph, dev, v), 'truncated',
+                                  g0, 0.3)
+    Ap = [op(p) for p in dirs]
+    for i in range(len(dirs)):
+        for j in range(i):
+            a = jo.c0_inner(dirs[i], Ap[j], g0, 0.3)
+            scale = np.sqrt(jo.c0_inner(dirs[i], Ap[i], g0, 0.3) *
+                            jo.c0_inner(dirs[j], Ap[j], g0, 0.3))
+            assert abs(a) < 1e-9 * scale
+
+
+def test_cg_matches_dense_direct_solve():
+    """4 x 4 grid: the operator I + Gamma0 dC assembled column by column
+    and solved directly."""
+    g0 = 1.0
+    sph, dev = _cg_setup(4, g0)
+    E = np.array([0.2, 0.5, -1.0])
+    n = sph.size * 3
+    cols = []
+    for i in range(n):
+        v = np.zeros(n)
+        v[i] = 1.0
+        v = v.reshape(sph.shape + (3,))
+        cols.append((v + jo.green_apply(jo.iso4_apply(sph, dev, v),
+                                        'truncated', g0, 0.3)).ravel())
+    A = np.array(cols).T
+    rhs = np.empty(sph.shape + (3,))
+    rhs[...] = E
+    direct = np.linalg.solve(A, rhs.ravel()).reshape(rhs.shape)
+    eps, _ = jo.cg_scheme(sph, dev, E, 'truncated', g0, 0.3, tol=1e-14,
+                          max_iter=200)
+    np.testing.assert_allclose(eps, direct, rtol=0, atol=1e-12)
+
+
+def test_c0_inner_matches_matrix():
+    rng = np.random.default_rng(3)
+    for dim, sym in ((2, 3), (3, 6)):
+        g0, nu0 = 2.5, 0.27
+        a = rng.standard_normal((5, sym))
+        b = rng.standard_normal((5, sym))
+        k0 = jo.bulk_modulus(g0, nu0, dim)
+        C0 = jo.iso4_matrix(dim * k0, 2 * g0, dim)
+        assert np.isclose(jo.c0_inner(a, b, g0, nu0),
+                          np.einsum('vi,ij,vj->', a, C0, b), rtol=1e-13)
