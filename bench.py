@@ -356,6 +356,30 @@ def workload_plain2d(args):
                 'strain_rel_err': rel, 'residual_rel_err': res_rel}}
         _log('%s: %s after %d iterations, residual %.3e'
              % (label, r.status, r.iterations, r.residuals[-1]))
+        if g0 == 1.0:
+            # the stated configuration by conjugate gradients
+            solver.solve_cg(E, tol=1e-10, max_iter=3)  # warm-up
+            _sync()
+            t0 = time.perf_counter()
+            rc = solver.solve_cg(E, tol=1e-10, max_iter=500)
+            _sync()
+            t = time.perf_counter() - t0
+            eps_cg, res_cg = jo.cg_scheme(table[phase, 0], table[phase, 1], E,
+                                          'truncated', g0, 0.3, tol=1e-10,
+                                          max_iter=500)
+            tau, _ = solver.local.means(rc.eps)
+            out['C0=matrix, CG'] = {
+                'status': rc.status, 'iterations': rc.iterations,
+                'residual_last': float(rc.residuals[-1]),
+                'seconds': round(t, 4),
+                'C_1212': float(g0 + 0.5 * tau[2] / E[2]),
+                'oracle_cg': {
+                    'iterations': len(res_cg),
+                    'strain_rel_err': float(
+                        np.abs(rc.eps.cpu().numpy() - eps_cg).max() /
+                        np.abs(eps_cg).max())}}
+            _log('C0=matrix, CG: %s after %d iterations'
+                 % (rc.status, rc.iterations))
     _line('2D 256^2 basic scheme (square inclusion, contrast 10)',
           out['C0=midpoint']['it_per_s'], 'iterations/s',
           {'workload': '2D 256^2 centred square vf 0.25, mu 1/10 nu 0.3, '
@@ -381,7 +405,24 @@ def workload_stiffness(args):
     mat_i = material.create(50.0, 0.25, 3)
     mat_div = material.create(1.0, 0.3, 3)
     s_div, _ = _two_phase_solver(phase, mat_m, mat_i, mat_div, 'filtered')
+    if not args.no_autotune:
+        s_div.autotune(iters=3)
     r_div = s_div.solve(_unit(6, 5), tol=1e-10, max_iter=30)
+    # the stated configuration (C0 = matrix) by conjugate gradients
+    s_div.solve_cg(_unit(6, 0), tol=1e-10, max_iter=3)  # warm-up
+    C_cg = np.zeros((6, 6))
+    its_cg = []
+    _sync()
+    t0 = time.perf_counter()
+    for j in range(6):
+        rc = s_div.solve_cg(_unit(6, j), tol=1e-10, max_iter=2000)
+        its_cg.append(rc.iterations)
+        tau, _ = s_div.local.means(rc.eps)
+        from janus_amd.solver import reference_stiffness
+        C_cg[:, j] = reference_stiffness(mat_div).dot(_unit(6, j)) + tau
+    _sync()
+    t_cg = time.perf_counter() - t0
+    _log('CG, C0 = matrix, 6 loads: %.3f s, iterations %s' % (t_cg, its_cg))
     del s_div
     mat0 = material.create(25.5, 0.3, 3)
     solver, table = _two_phase_solver(phase, mat_m, mat_i, mat0, 'filtered')
@@ -431,6 +472,14 @@ def workload_stiffness(args):
               'C0_matrix_divergence': {
                   'status': r_div.status,
                   'residuals': [float(x) for x in r_div.residuals[::5]]},
+              'C0_matrix_cg': {
+                  'C_eff': np.round(C_cg, 10).tolist(),
+                  'iterations': its_cg,
+                  'seconds': round(t_cg, 4),
+                  'symmetry_rel': float(np.abs(C_cg - C_cg.T).max() /
+                                        np.abs(C_cg).max()),
+                  'rel_diff_to_midpoint_basic_C_eff': float(
+                      np.abs(C_cg - C_seq).max() / np.abs(C_seq).max())},
               'oracle_%d_iterations_strain_rel_err' % k: rel},
           higher=False)
 
