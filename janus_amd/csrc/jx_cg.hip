@@ -14,7 +14,7 @@
 // which for the basic scheme is its own residual ||eps_{k+1} - eps_k||.
 //
 // One iteration: q = A p through the five passes of the Green operator
-// (pass A applies dC to p) and four vector kernels with deterministic
+// (pass A applies dC to p) and three vector kernels with deterministic
 // per-block partial sums; alpha, beta and the residual are computed on the
 // device by a one-block kernel, and every kernel returns at once when the
 // device state says the solve is done.
@@ -29,7 +29,7 @@ namespace {
 enum CgMode { CG_FILL, CG_INIT, CG_QP, CG_XR, CG_P, CG_FINAL };
 enum CgPhase { PH_INIT, PH_ALPHA, PH_BETA };
 
-constexpr int kCgBlocks = 1024;  // blocks of the vector kernels (partials)
+constexpr int kCgBlocks = 4096;  // blocks of the vector kernels (partials)
 constexpr int kScalarThreads = 1024;
 
 // a : C0 : b of one voxel, C0 = dim k0 J + 2 g0 K in Mandel notation:
@@ -49,6 +49,37 @@ __device__ __forceinline__ double c0_dot(const double *a, const double *b,
   return wd * dot + wt * ta * tb;
 }
 
+// The C components of voxel v: 16-byte accesses for C = 6 (48-byte voxels).
+template <int C>
+__device__ __forceinline__ void load_voxel(const double *a, int64_t v,
+                                           double *x) {
+  if constexpr (C == 6) {
+    const double2 *a2 = (const double2 *)(a + v * 6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 t = a2[i];
+      x[2 * i] = t.x;
+      x[2 * i + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) x[c] = a[v * C + c];
+  }
+}
+
+template <int C>
+__device__ __forceinline__ void store_voxel(double *a, int64_t v,
+                                            const double *x) {
+  if constexpr (C == 6) {
+    double2 *a2 = (double2 *)(a + v * 6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a2[i] = make_double2(x[2 * i], x[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) a[v * C + c] = x[c];
+  }
+}
+
 template <int C, int MODE>
 __global__ void __launch_bounds__(kThreads)
     k_cg_vec(double *x, double *r, double *p, double *q, int64_t N, double wd,
@@ -63,55 +94,68 @@ __global__ void __launch_bounds__(kThreads)
   const double beta = MODE == CG_P ? cs->beta : 0.;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
        v += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t o = v * C;
+    double e[C];
     if constexpr (MODE == CG_FILL) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) x[o + c] = st->E[c];
+      for (int c = 0; c < C; ++c) e[c] = st->E[c];
+      store_voxel<C>(x, v, e);
     } else if constexpr (MODE == CG_FINAL) {
+      load_voxel<C>(x, v, e);
 #pragma unroll
-      for (int c = 0; c < C; ++c) x[o + c] = st->E[c] + x[o + c];
+      for (int c = 0; c < C; ++c) e[c] = st->E[c] + e[c];
+      store_voxel<C>(x, v, e);
     } else if constexpr (MODE == CG_INIT) {
       // r = -G[dC : E] (in q), p = r, x = 0
-      double rv[C];
+      double rv[C], z[C];
+      load_voxel<C>(q, v, rv);
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        rv[c] = -q[o + c];
-        r[o + c] = rv[c];
-        p[o + c] = rv[c];
-        x[o + c] = 0.;
+        rv[c] = -rv[c];
+        z[c] = 0.;
       }
+      store_voxel<C>(r, v, rv);
+      store_voxel<C>(p, v, rv);
+      store_voxel<C>(x, v, z);
       acc0 += c0_dot<C>(rv, rv, wd, wt);
 #pragma unroll
       for (int c = 0; c < C; ++c) acc1 += rv[c] * rv[c];
     } else if constexpr (MODE == CG_QP) {
       // q = p + G[dC : p]
       double pv[C], qv[C];
+      load_voxel<C>(p, v, pv);
+      load_voxel<C>(q, v, qv);
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        pv[c] = p[o + c];
-        qv[c] = pv[c] + q[o + c];
-        q[o + c] = qv[c];
-      }
+      for (int c = 0; c < C; ++c) qv[c] = pv[c] + qv[c];
+      store_voxel<C>(q, v, qv);
       acc0 += c0_dot<C>(pv, qv, wd, wt);
     } else if constexpr (MODE == CG_XR) {
-      double rv[C];
+      double xv[C], rv[C], pv[C], qv[C];
+      load_voxel<C>(x, v, xv);
+      load_voxel<C>(r, v, rv);
+      load_voxel<C>(p, v, pv);
+      load_voxel<C>(q, v, qv);
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        double ap = alpha * p[o + c];
-        double aq = alpha * q[o + c];
-        x[o + c] = x[o + c] + ap;
-        rv[c] = r[o + c] - aq;
-        r[o + c] = rv[c];
+        double ap = alpha * pv[c];
+        double aq = alpha * qv[c];
+        xv[c] = xv[c] + ap;
+        rv[c] = rv[c] - aq;
       }
+      store_voxel<C>(x, v, xv);
+      store_voxel<C>(r, v, rv);
       acc0 += c0_dot<C>(rv, rv, wd, wt);
 #pragma unroll
       for (int c = 0; c < C; ++c) acc1 += rv[c] * rv[c];
     } else {  // CG_P
+      double rv[C], pv[C];
+      load_voxel<C>(r, v, rv);
+      load_voxel<C>(p, v, pv);
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        double bp = beta * p[o + c];
-        p[o + c] = r[o + c] + bp;
+        double bp = beta * pv[c];
+        pv[c] = rv[c] + bp;
       }
+      store_voxel<C>(p, v, pv);
     }
   }
   if constexpr (MODE == CG_INIT || MODE == CG_QP || MODE == CG_XR) {
