@@ -661,36 +661,45 @@ def workload_adjoint(args):
         z = torch.zeros((n,) * 3, dtype=torch.float64, device='cuda')
         LSSolver(green, BlockDiagonalOperator3D.from_fields(z, z)).autotune(3)
 
-    def loss_fn(rho, tol, max_iter, info=None):
+    def loss_fn(rho, tol, max_iter, info=None, method='basic'):
         g = 1.0 + rho ** 3 * 19.0
         nu = torch.full_like(rho, 0.3)
         sph, dev = iso_delta(g, nu, mat0, 3)
         eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
-                       info=info)
+                       info=info, method=method)
         sigma = mean_stress(sph, dev, eps, E, mat0)
         return (0.5 * sigma[5] / E[5] - 3.0) ** 2
 
-    # warm-up
-    with torch.no_grad():
-        loss_fn(torch.tensor(rho_np, device='cuda'), 0., 3)
-    ratios = []
-    for rep in range(args.adjoint_reps):
-        rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
-        info = {}
-        _sync()
-        t0 = time.perf_counter()
-        loss = loss_fn(rho, 1e-10, 5000, info)
-        _sync()
-        t1 = time.perf_counter()
-        loss.backward()
-        _sync()
-        t2 = time.perf_counter()
-        ratios.append((t1 - t0, t2 - t1))
-    t_fwd, t_bwd = min(r[0] for r in ratios), min(r[1] for r in ratios)
-    fwd, adj = info['forward'], info['adjoint']
-    _log('forward %.3f s (%d it), backward %.3f s (adjoint %d it)'
-         % (t_fwd, fwd.iterations, t_bwd, adj.iterations))
-    grad = rho.grad.cpu().numpy()
+    runs = {}
+    grads = {}
+    for method in ('basic', 'cg'):
+        with torch.no_grad():  # warm-up
+            loss_fn(torch.tensor(rho_np, device='cuda'), 0., 3, method=method)
+        ratios = []
+        for rep in range(args.adjoint_reps):
+            rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
+            info = {}
+            _sync()
+            t0 = time.perf_counter()
+            loss = loss_fn(rho, 1e-10, 5000, info, method)
+            _sync()
+            t1 = time.perf_counter()
+            loss.backward()
+            _sync()
+            t2 = time.perf_counter()
+            ratios.append((t1 - t0, t2 - t1))
+        t_fwd, t_bwd = min(r[0] for r in ratios), min(r[1] for r in ratios)
+        fwd, adj = info['forward'], info['adjoint']
+        _log('%s: forward %.3f s (%d it), backward %.3f s (adjoint %d it)'
+             % (method, t_fwd, fwd.iterations, t_bwd, adj.iterations))
+        grads[method] = rho.grad.cpu().numpy()
+        runs[method] = {'seconds_forward': round(t_fwd, 4),
+                        'seconds_backward': round(t_bwd, 4),
+                        'ratio': round(t_bwd / t_fwd, 4),
+                        'forward_iterations': fwd.iterations,
+                        'adjoint_iterations': adj.iterations,
+                        'loss': float(loss.item())}
+    grad = grads['basic']
     fd = None
     if args.fd_voxels > 0:
         info2 = {}
@@ -719,16 +728,17 @@ def workload_adjoint(args):
               'median_rel_err': float(np.median(errs)),
               'grad_1e10_vs_1e14_max_rel': float(
                   np.abs(grad - grad14).max() / np.abs(grad14).max())}
-    _line('adjoint gradient time / forward solve time at %d^3 (1e-10)' % n,
-          round(t_bwd / t_fwd, 4), 'ratio', {
+    _line('adjoint gradient time / forward solve time at %d^3 (1e-10), '
+          'conjugate gradients' % n,
+          runs['cg']['ratio'], 'ratio', {
               'workload': '%d^3 SIMP rho smoothed noise sigma 4 seed 2014, '
                           'mu=1+19 rho^3 nu 0.3, filtered Gamma0, mu0 10.5, '
                           'E=e_xy, L=(C_1212-3)^2, tol 1e-10' % n,
-              'seconds_forward': round(t_fwd, 4),
-              'seconds_backward': round(t_bwd, 4),
-              'forward_iterations': fwd.iterations,
-              'adjoint_iterations': adj.iterations,
-              'loss': float(loss.item()),
+              'cg': runs['cg'],
+              'basic_scheme': runs['basic'],
+              'grad_cg_vs_basic_max_rel': float(
+                  np.abs(grads['cg'] - grads['basic']).max() /
+                  np.abs(grads['basic']).max()),
               'finite_differences': fd},
           higher=False)
 
@@ -827,7 +837,7 @@ def workload_large(args):
         nu = torch.full_like(g, 0.3)
         sph, dev = iso_delta(g, nu, mat0, 3)
         eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
-                       info=info)
+                       info=info, method=args.method)
         return mean_stress(sph, dev, eps, E, mat0)[5]
 
     mu_t = torch.tensor(mus, dtype=torch.float64, device='cuda',
@@ -868,6 +878,7 @@ def workload_large(args):
               'GBps_B_alg': round(B * its / 1e9, 1),
               'roofline_frac_B_alg': round(B * its / 1e9 / HBM_PEAK_GBS, 4),
               'adjoint': {
+                  'method': args.method,
                   'loss': 'C_eff[5,5] under E=e_xy',
                   'value': float(L.item()),
                   'dL_dmu_phase': grad.tolist(),
@@ -919,6 +930,9 @@ def main():
     ap.add_argument('--adjoint-reps', type=int, default=2)
     ap.add_argument('--fd-voxels', type=int, default=20)
     ap.add_argument('--large-iters', type=int, default=20)
+    ap.add_argument('--method', default='cg', choices=['basic', 'cg'],
+                    help='solver of the large workload\'s forward and '
+                         'adjoint solves')
     ap.add_argument('--stats-summary', nargs=2, metavar=('STATS_CSV', 'COMMAND'),
                     help='print the kernel table of a rocprofv3 --stats run '
                          '(no GPU work)')

@@ -143,12 +143,16 @@ int jx_solver_solve(jx_solver *solver, double *eps, const double *E,
  * overrides the 1 / (sqrt(N) ||E||)), the basic scheme's own residual
  * definition (square_basic.py:86, 91); the solve stops after the first
  * iteration below tol.  eps: device field, output only (the iteration
- * starts from E).  Other arguments as jx_solver_solve.  Allocates three
- * fields of the grid on first use.  Synchronizes the stream. */
+ * starts from E).  rhs: optional device field added to E, as in
+ * jx_solver_solve (eps solves (I + Gamma0 dC) eps = E + rhs); it must be
+ * compatible and zero-mean (a Gamma0 output, as in the adjoint solve).  It
+ * enters the initial residual only.  Other arguments as jx_solver_solve.
+ * Allocates three fields of the grid on first use.  Synchronizes the
+ * stream. */
 int jx_solver_solve_cg(jx_solver *solver, double *eps, const double *E,
-                       double tol, int max_iter, double res_scale,
-                       double *res_hist, int *iters, int *status,
-                       void *stream);
+                       const double *rhs, double tol, int max_iter,
+                       double res_scale, double *res_hist, int *iters,
+                       int *status, void *stream);
 /* Schedule choice (no janus counterpart).  fused = 1 (default): the last
  * pass of an iteration also applies the local operator to the new iterate
  * and transforms it for the next iteration, so the iterate is written once

@@ -67,8 +67,8 @@ SIGNATURES = [
     ('jx_solver_get_fused', ctypes.c_int, [_VP]),
     ('jx_solver_profile', ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]),
     ('jx_solver_solve_cg', ctypes.c_int,
-     [_VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double, _VP,
-      _VP, _VP, _VP]),
+     [_VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+      _VP, _VP, _VP, _VP]),
     ('jx_green_set_plan', ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     ('jx_green_get_plan', ctypes.c_int, [_VP, ctypes.c_int]),
     ('jx_solver_autotune', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),

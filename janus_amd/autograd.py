@@ -14,6 +14,9 @@ L(eps) with g = dL/deps:
 
 Only the converged eps is kept from the forward pass.  The backward costs
 one Green-operator application, one fixed-point solve and one contraction.
+Both solves use the same method: the basic scheme, or conjugate gradients
+(``method='cg'``), where the right-hand side G g enters the initial
+residual only, so an adjoint iteration costs what a forward one does.
 """
 
 import ctypes
@@ -46,21 +49,27 @@ def contract(eps, mu):
     return dsph, ddev
 
 
-def adjoint_solve(green, local, g, tol, max_iter, solver=None):
+def adjoint_solve(green, local, g, tol, max_iter, solver=None,
+                  method='basic'):
     """mu solving mu = G g - G[local : mu]; returns (mu, SolveResult).
 
     ``solver``: an LSSolver on (green, local) to reuse (e.g. the forward
-    solve's)."""
+    solve's); ``method``: 'basic' (fixed point) or 'cg'."""
     t = _torch()
     gg = green.apply(g.contiguous())
     norm = float(t.linalg.vector_norm(gg))
     if norm == 0.0:
         return t.zeros_like(gg), None
-    mu = gg.clone()
     if solver is None:
         solver = LSSolver(green, local)
-    r = solver.solve(np.zeros(gg.shape[-1]), tol=tol, max_iter=max_iter,
-                     eps=mu, rhs=gg, res_scale=1.0 / norm)
+    zero = np.zeros(gg.shape[-1])
+    if method == 'cg':
+        r = solver.solve_cg(zero, tol=tol, max_iter=max_iter, rhs=gg,
+                            res_scale=1.0 / norm)
+        return r.eps, r
+    mu = gg.clone()
+    r = solver.solve(zero, tol=tol, max_iter=max_iter, eps=mu, rhs=gg,
+                     res_scale=1.0 / norm)
     return mu, r
 
 
@@ -75,13 +84,17 @@ def _make_function():
 
     class LSSolveFunction(t.autograd.Function):
         @staticmethod
-        def forward(ctx, sph, dev, E, green, tol, max_iter, info):
+        def forward(ctx, sph, dev, E, green, tol, max_iter, info, method):
             sph = sph.detach().contiguous()
             dev = dev.detach().contiguous()
             local = _local_from_fields(sph, dev)
             solver = LSSolver(green, local)
-            r = solver.solve(E.detach().cpu().numpy(), tol=tol,
-                             max_iter=max_iter)
+            if method == 'cg':
+                r = solver.solve_cg(E.detach().cpu().numpy(), tol=tol,
+                                    max_iter=max_iter)
+            else:
+                r = solver.solve(E.detach().cpu().numpy(), tol=tol,
+                                 max_iter=max_iter)
             if info is not None:
                 info['forward'] = r
             ctx.save_for_backward(sph, dev, r.eps)
@@ -92,6 +105,7 @@ def _make_function():
             ctx.tol = tol
             ctx.max_iter = max_iter
             ctx.info = info
+            ctx.method = method
             return r.eps
 
         @staticmethod
@@ -99,20 +113,21 @@ def _make_function():
             sph, dev, eps = ctx.saved_tensors
             local = ctx.local
             mu, r = adjoint_solve(ctx.green, local, g, ctx.tol, ctx.max_iter,
-                                  solver=ctx.solver)
+                                  solver=ctx.solver, method=ctx.method)
             if ctx.info is not None:
                 ctx.info['adjoint'] = r
             dsph, ddev = contract(eps, mu)
             lam = g.contiguous() - local.apply(mu)
             dims = tuple(range(lam.dim() - 1))
             dE = lam.sum(dim=dims)
-            return dsph, ddev, dE, None, None, None, None
+            return dsph, ddev, dE, None, None, None, None, None
 
     return LSSolveFunction
 
 
-def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None):
-    """Differentiable basic-scheme solve.
+def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None,
+             method='basic'):
+    """Differentiable Lippmann-Schwinger solve.
 
     Args:
         sph, dev: float64 GPU tensors of the grid shape, the per-voxel
@@ -121,12 +136,17 @@ def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None):
         green: a janus_amd.green operator with a transform (reference C0).
         info: optional dict receiving the SolveResult of the forward
             ('forward') and adjoint ('adjoint') solves.
+        method: 'basic' (the fixed point) or 'cg' (conjugate gradients,
+            LSSolver.solve_cg) for both the forward and the adjoint solve.
     Returns:
         eps, the converged strain field (grid + (sym,)).
     """
+    if method not in ('basic', 'cg'):
+        raise ValueError("method must be 'basic' or 'cg'")
     if _LSSolveFunction.fn is None:
         _LSSolveFunction.fn = _make_function()
-    return _LSSolveFunction.fn.apply(sph, dev, E, green, tol, max_iter, info)
+    return _LSSolveFunction.fn.apply(sph, dev, E, green, tol, max_iter, info,
+                                     method)
 
 
 def iso_delta(g, nu, mat0, dim):

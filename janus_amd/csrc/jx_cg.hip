@@ -84,7 +84,7 @@ template <int C, int MODE>
 __global__ void __launch_bounds__(kThreads)
     k_cg_vec(double *x, double *r, double *p, double *q, int64_t N, double wd,
              double wt, const CgScalars *cs, const SolveState *st,
-             double *partials) {
+             double *partials, const double *rhs) {
 #pragma clang fp contract(off)
   // the fill and the final sum run whatever the state
   if (MODE != CG_FILL && MODE != CG_FINAL && solve_done(st)) return;
@@ -105,12 +105,15 @@ __global__ void __launch_bounds__(kThreads)
       for (int c = 0; c < C; ++c) e[c] = st->E[c] + e[c];
       store_voxel<C>(x, v, e);
     } else if constexpr (MODE == CG_INIT) {
-      // r = -G[dC : E] (in q), p = r, x = 0
+      // r = rhs - G[dC : E] (G[dC : E] in q, rhs in r or absent), p = r,
+      // x = 0
       double rv[C], z[C];
       load_voxel<C>(q, v, rv);
+      double hv[C];
+      if (rhs) load_voxel<C>(rhs, v, hv);
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        rv[c] = -rv[c];
+        rv[c] = rhs ? hv[c] - rv[c] : -rv[c];
         z[c] = 0.;
       }
       store_voxel<C>(r, v, rv);
@@ -213,9 +216,9 @@ __global__ void __launch_bounds__(kScalarThreads)
 template <int C, int MODE>
 int vec(double *x, double *r, double *p, double *q, const Grid &g, double wd,
         double wt, const CgScalars *cs, const SolveState *st,
-        double *partials, hipStream_t s) {
+        double *partials, hipStream_t s, const double *rhs = nullptr) {
   hipLaunchKernelGGL((k_cg_vec<C, MODE>), dim3(kCgBlocks), dim3(kThreads), 0,
-                     s, x, r, p, q, g.N, wd, wt, cs, st, partials);
+                     s, x, r, p, q, g.N, wd, wt, cs, st, partials, rhs);
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
@@ -270,18 +273,18 @@ int cg_iteration(jx_solver *S, double *x, double wd, double wt, hipStream_t s) {
 }
 
 template <int C>
-int cg_solve(jx_solver *S, double *eps, int max_iter, double wd, double wt,
-             hipStream_t s) {
+int cg_solve(jx_solver *S, double *eps, const double *rhs, int max_iter,
+             double wd, double wt, hipStream_t s) {
   const Grid &g = S->green->g;
   double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
   int rc;
-  // q = G[dC : E], then r = p = -q, x = 0 (x is eps)
+  // q = G[dC : E], then r = p = rhs - q, x = 0 (x is eps)
   if ((rc = vec<C, CG_FILL>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
                             S->cg_partials, s)))
     return rc;
   if ((rc = matvec(S, eps, q, S->st, s))) return rc;
   if ((rc = vec<C, CG_INIT>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
-                            S->cg_partials, s)))
+                            S->cg_partials, s, rhs)))
     return rc;
   if ((rc = scalar<PH_INIT>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
     return rc;
@@ -310,9 +313,10 @@ using namespace jx;
 
 extern "C" {
 
-int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E, double tol,
-                       int max_iter, double res_scale, double *res_hist,
-                       int *iters, int *status, void *stream) {
+int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E,
+                       const double *rhs, double tol, int max_iter,
+                       double res_scale, double *res_hist, int *iters,
+                       int *status, void *stream) {
   if (!S || !eps || !E) return fail(JX_EINVAL, "jx_solver_solve_cg: NULL argument");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   hipStream_t s = (hipStream_t)stream;
@@ -353,8 +357,8 @@ int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E, double tol,
   const double k0 = dim == 2 ? g0 / (1. - 2. * nu0)
                              : 2. / 3. * (1. + nu0) / (1. - 2. * nu0) * g0;
   const double wd = 2. * g0, wt = k0 - 2. * g0 / dim;
-  int rc = C == 3 ? cg_solve<3>(S, eps, max_iter, wd, wt, s)
-                  : cg_solve<6>(S, eps, max_iter, wd, wt, s);
+  int rc = C == 3 ? cg_solve<3>(S, eps, rhs, max_iter, wd, wt, s)
+                  : cg_solve<6>(S, eps, rhs, max_iter, wd, wt, s);
   if (rc) return rc;
   JX_HIP(hipStreamSynchronize(s));
   int it = S->st_host->iter;

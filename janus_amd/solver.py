@@ -135,7 +135,8 @@ class LSSolver:
         n = iters.value
         return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
 
-    def solve_cg(self, E, tol=1e-10, max_iter=200, eps=None, res_scale=None):
+    def solve_cg(self, E, tol=1e-10, max_iter=200, eps=None, rhs=None,
+                 res_scale=None):
         """Conjugate gradients on the same operators: the fluctuation
         eps - E solves (I + Gamma0 dC) x = -Gamma0[dC : E] in the inner
         product sum a : C0 : b.  Converges for any isotropic C0 with C
@@ -146,6 +147,8 @@ class LSSolver:
             E: macroscopic strain (sym,), Mandel notation.
             eps: optional float64 GPU tensor of shape ``grid + (sym,)``
                 receiving the solution (its content is not used).
+            rhs: optional GPU field added to E (compatible and zero-mean,
+                e.g. a Gamma0 output, as in the adjoint solve).
         Returns:
             SolveResult(eps, residuals, iterations, status).
         """
@@ -159,11 +162,17 @@ class LSSolver:
                   tuple(eps.shape) == self.shape and eps.dtype == t.float64):
             raise ValueError('eps must be a contiguous float64 GPU tensor of '
                              'shape {0}'.format(self.shape))
+        if rhs is not None and not (_device.is_device_tensor(rhs) and
+                                    rhs.is_contiguous() and
+                                    tuple(rhs.shape) == self.shape):
+            raise ValueError('rhs must be a contiguous GPU field of shape '
+                             '{0}'.format(self.shape))
         hist = np.zeros(max_iter, dtype=np.float64)
         iters = ctypes.c_int(0)
         status = ctypes.c_int(0)
         _lib.check(_lib.lib().jx_solver_solve_cg(
-            self._handle, eps.data_ptr(), E.ctypes.data, float(tol),
+            self._handle, eps.data_ptr(), E.ctypes.data, _lib.ptr(rhs),
+            float(tol),
             int(max_iter), float(res_scale or 0.0), hist.ctypes.data,
             ctypes.byref(iters), ctypes.byref(status),
             _lib.current_stream()))

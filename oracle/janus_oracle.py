@@ -443,7 +443,7 @@ def c0_inner(a, b, g0, nu0):
 
 
 def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
-              directions=None):
+              directions=None, rhs=None, res_scale=None):
     """Conjugate gradients on (I + Gamma0 dC) eps~ = -Gamma0[dC : E] for the
     fluctuation eps~ (eps = E + eps~), in the inner product
     <a, b> = sum a : C0 : b.
@@ -456,13 +456,18 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
     scheme this quantity is its own residual ||eps_{k+1} - eps_k|| (same
     normalization, square_basic.py:86, 91).  Stops after the first
     iteration whose residual is < tol.  ``directions``: optional list
-    receiving the search directions (for tests).  Returns (eps, residuals).
+    receiving the search directions (for tests).  ``rhs``: optional field
+    added to E, as in basic_scheme: eps solves (I + Gamma0 dC) eps = E + rhs
+    (rhs compatible and zero-mean, e.g. a Gamma0 output); ``res_scale``
+    replaces 1 / (sqrt(N) ||E||).  Returns (eps, residuals).
     """
     E = np.asarray(E, dtype=np.float64)
     shape = np.shape(sph)
     N = int(np.prod(shape))
     nE = np.linalg.norm(E)
     normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
+    if res_scale is not None:
+        normalization = res_scale
 
     def op(v):
         return v + green_apply(iso4_apply(sph, dev, v), kind, g0, nu0, h)
@@ -470,6 +475,8 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
     Ef = np.empty(shape + (E.size,), dtype=np.float64)
     Ef[...] = E
     r = -green_apply(iso4_apply(sph, dev, Ef), kind, g0, nu0, h)
+    if rhs is not None:
+        r = rhs + r
     x = np.zeros_like(r)
     p = r.copy()
     rr = c0_inner(r, r, g0, nu0)

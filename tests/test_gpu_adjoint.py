@@ -21,10 +21,11 @@ from oracle import janus_oracle as jo
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize('method', ['basic', 'cg'])
 @pytest.mark.parametrize('dim, n, kind', [(2, 24, 'filtered'),
                                           (3, 12, 'truncated'),
                                           (3, 10, 'filtered')])
-def test_gradient_vs_oracle(dim, n, kind):
+def test_gradient_vs_oracle(dim, n, kind, method):
     import torch
     rng = np.random.default_rng(dim * 100 + n)
     g0, nu0 = 5.0, 0.3
@@ -40,7 +41,8 @@ def test_gradient_vs_oracle(dim, n, kind):
     sph_t = torch.tensor(sph, device='cuda', requires_grad=True)
     dev_t = torch.tensor(dev, device='cuda', requires_grad=True)
     E_t = torch.tensor(E, device='cuda', requires_grad=True)
-    eps = ls_solve(sph_t, dev_t, E_t, green, tol=1e-14, max_iter=3000)
+    eps = ls_solve(sph_t, dev_t, E_t, green, tol=1e-14, max_iter=3000,
+                   method=method)
     loss = (torch.from_numpy(w).cuda() * eps).sum()
     loss.backward()
     eps_ref, _ = jo.basic_scheme(sph, dev, E, kind, g0, nu0, tol=1e-14,
@@ -57,18 +59,21 @@ def test_gradient_vs_oracle(dim, n, kind):
                                atol=1e-9 * np.abs(dE).max())
 
 
-def simp_loss(rho, green, mat0, E, tol, max_iter=4000, info=None):
+def simp_loss(rho, green, mat0, E, tol, max_iter=4000, info=None,
+              method='basic'):
     import torch
     g = 1.0 + rho ** 3 * (20.0 - 1.0)
     nu = torch.full_like(rho, 0.3)
     sph, dev = iso_delta(g, nu, mat0, 3)
-    eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter, info=info)
+    eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter, info=info,
+                   method=method)
     sigma = mean_stress(sph, dev, eps, E, mat0)
     c1212 = 0.5 * sigma[5] / E[5]
     return (c1212 - 3.0) ** 2
 
 
-def test_simp_gradient_finite_differences():
+@pytest.mark.parametrize('method', ['basic', 'cg'])
+def test_simp_gradient_finite_differences(method):
     import torch
     n = 24
     rho_np = ms.smoothed_density(n, 2.0, seed=4)
@@ -79,7 +84,7 @@ def test_simp_gradient_finite_differences():
     E[5] = 1.0
     rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
     info = {}
-    loss = simp_loss(rho, green, mat0, E, 1e-14, info=info)
+    loss = simp_loss(rho, green, mat0, E, 1e-14, info=info, method=method)
     loss.backward()
     assert info['forward'].status == 'converged'
     assert info['adjoint'].status == 'converged'
@@ -98,9 +103,9 @@ def test_simp_gradient_finite_differences():
         rm[v] -= h
         with torch.no_grad():
             lp = simp_loss(torch.tensor(rp, device='cuda'), green, mat0, E,
-                           0.0, iters).item()
+                           0.0, iters, method=method).item()
             lm = simp_loss(torch.tensor(rm, device='cuda'), green, mat0, E,
-                           0.0, iters).item()
+                           0.0, iters, method=method).item()
         fd = (lp - lm) / (2 * h)
         errs.append(abs(fd - grad[v]))
     scale = np.abs(grad).max()

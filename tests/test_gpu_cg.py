@@ -123,3 +123,30 @@ def test_cg_plan_variants_agree():
         assert r.status == 'converged'
         out.append(r.eps.cpu().numpy())
     assert np.abs(out[0] - out[1]).max() < 1e-9
+
+
+@pytest.mark.parametrize('with_E', [False, True])
+def test_cg_rhs_field_matches_oracle(with_E):
+    """eps solves (I + Gamma0 dC) eps = E + rhs for a compatible rhs (a
+    Gamma0 output, as in the adjoint solve), iterate for iterate."""
+    import torch
+    phase = ms.spheres(16, 3.0, 0.25, seed=12)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(8.0, 0.25, 3)
+    mat0 = material.create(1.0, 0.3, 3)
+    solver, table = build(phase, [mat_m, mat_i], mat0, 'filtered')
+    rng = np.random.default_rng(7)
+    w = rng.standard_normal(phase.shape + (6,))
+    rhs = np.asarray(solver.green.apply(w))
+    E = rng.standard_normal(6) if with_E else np.zeros(6)
+    scale = 1.0 / np.linalg.norm(rhs)
+    r = solver.solve_cg(E, tol=1e-10, max_iter=500,
+                        rhs=torch.from_numpy(rhs).cuda(), res_scale=scale)
+    eps_ref, res_ref = jo.cg_scheme(table[phase, 0], table[phase, 1], E,
+                                    'filtered', 1.0, 0.3, tol=1e-10,
+                                    max_iter=500, rhs=rhs, res_scale=scale)
+    assert r.status == 'converged'
+    assert r.iterations == len(res_ref)
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-8, atol=1e-15)
+    eps = r.eps.cpu().numpy()
+    assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12
