@@ -832,8 +832,14 @@ def workload_large(args):
     E = torch.zeros(6, dtype=torch.float64, device='cuda')
     E[5] = 1.0
 
+    # mu(x) = sum_p mu_p [phase(x) = p]: its backward is three reductions
+    # (an indexed gather would scatter-add 134 M values into three bins)
+    masks = [(ph == p).double() for p in range(len(mus))]
+
     def loss_fn(mu_t, tol, max_iter, info=None):
-        g = mu_t[ph]
+        g = masks[0] * mu_t[0]
+        for p in range(1, len(mus)):
+            g = g + masks[p] * mu_t[p]
         nu = torch.full_like(g, 0.3)
         sph, dev = iso_delta(g, nu, mat0, 3)
         eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
