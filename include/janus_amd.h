@@ -38,6 +38,7 @@ extern "C" {
 
 #define JX_GREEN_TRUNCATED 0
 #define JX_GREEN_FILTERED 1
+#define JX_GREEN_WILLOT 2 /* janus/green.pyx:976-1036 willot2015 */
 
 #define JX_SOLVE_CONVERGED 1 /* residual < tol */
 #define JX_SOLVE_MAXITER 2   /* max_iter iterations without reaching tol */

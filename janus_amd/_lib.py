@@ -23,6 +23,7 @@ JX_EUNSUPPORTED = -4
 
 JX_GREEN_TRUNCATED = 0
 JX_GREEN_FILTERED = 1
+JX_GREEN_WILLOT = 2
 
 JX_SOLVE_CONVERGED = 1
 JX_SOLVE_MAXITER = 2

@@ -223,10 +223,10 @@ int jx_fft_c2r(jx_fft *f, const double *c, double *r, void *stream) {
 
 // ------------------------------------------------------------ Green
 static void build_axis_tables(jx_green *G) {
-  // per axis: kt, km, kp, wm, wp (5 * n_i doubles)
+  // per axis: kt, km, kp, wm, wp, sw, cw (7 * n_i doubles)
   const Grid &g = G->g;
   size_t total = 0;
-  for (int d = 0; d < g.dim; ++d) total += 5 * (size_t)g.n[d];
+  for (int d = 0; d < g.dim; ++d) total += 7 * (size_t)g.n[d];
   G->host_tables.assign(total, 0.);
   size_t off = 0;
   for (int d = 0; d < g.dim; ++d) {
@@ -235,6 +235,7 @@ static void build_axis_tables(jx_green *G) {
     double s = 2. * M_PI / (G->h * (double)n);
     double *kt = &G->host_tables[off];
     double *km = kt + n, *kp = km + n, *wm = kp + n, *wp = wm + n;
+    double *sw = wp + n, *cw = sw + n;
     for (int64_t b = 0; b < n; ++b) {
       // truncated alias rule, green.pyx:432-445
       kt[b] = (2 * b > n) ? s * (double)(b - n) : s * (double)b;
@@ -245,8 +246,12 @@ static void build_axis_tables(jx_green *G) {
       kp[b] = s * (double)b;
       w = cos(0.25 * G->h * kp[b]);
       wp[b] = w * w;
+      // Willot's modified wave vector, green.pyx:984-1023
+      double phi = 0.5 * s * G->h * (double)b;
+      sw[b] = sin(phi);
+      cw[b] = cos(phi);
     }
-    off += 5 * (size_t)n;
+    off += 7 * (size_t)n;
   }
 }
 
@@ -259,7 +264,9 @@ static void point_tables(GreenDev &D, const Grid &g, const double *base) {
     D.ax[d].kp = base + off + 2 * n;
     D.ax[d].wm = base + off + 3 * n;
     D.ax[d].wp = base + off + 4 * n;
-    off += 5 * (size_t)n;
+    D.ax[d].sw = base + off + 5 * n;
+    D.ax[d].cw = base + off + 6 * n;
+    off += 7 * (size_t)n;
   }
 }
 
@@ -269,7 +276,8 @@ int jx_green_create(int dim, const int64_t *shape, double h, int kind, double g0
   int rc = check_shape(dim, shape);
   if (rc) return rc;
   if (!(h > 0.)) return fail(JX_EINVAL, "h must be > 0");
-  if (kind != JX_GREEN_TRUNCATED && kind != JX_GREEN_FILTERED)
+  if (kind != JX_GREEN_TRUNCATED && kind != JX_GREEN_FILTERED &&
+      kind != JX_GREEN_WILLOT)
     return fail(JX_EINVAL, "unknown Green operator kind");
   if (!(g0 > 0.)) return fail(JX_EINVAL, "g0 must be > 0");
   jx_green *G = new jx_green();

@@ -90,9 +90,37 @@ JX_HD int tri_index(int sym, int i, int j) {
 // 505-529).  Filtered: sum over the 2^dim alias combinations of
 // prod_i cos^2(h k_i / 4) Gamma0(k), accumulated in janus's order -- axis 0
 // slowest, the "minus" alias first (green.pyx:607-694, 730-910).
+// Willot's modified wave vector k_i = sin(phi_i) prod_{j != i} cos(phi_j)
+// (green.pyx:984-1023), from the per-axis sin/cos tables.
+template <int DIM>
+JX_HD void willot_k(const GreenDev &G, const int *b, double *k) {
+  if (DIM == 2) {
+    double s0 = G.ax[0].sw[b[0]], c0 = G.ax[0].cw[b[0]];
+    double s1 = G.ax[1].sw[b[1]], c1 = G.ax[1].cw[b[1]];
+    k[0] = s0 * c1;
+    k[1] = c0 * s1;
+  } else {
+    double s0 = G.ax[0].sw[b[0]], c0 = G.ax[0].cw[b[0]];
+    double s1 = G.ax[1].sw[b[1]], c1 = G.ax[1].cw[b[1]];
+    double s2 = G.ax[2].sw[b[2]], c2 = G.ax[2].cw[b[2]];
+    k[0] = s0 * c1 * c2;
+    k[1] = c0 * s1 * c2;
+    k[DIM - 1] = c0 * c1 * s2;
+  }
+}
+
 template <int DIM>
 JX_HD void gamma_hat(const GreenDev &G, const int *b, double *g) {
   constexpr int NC = DIM == 2 ? 6 : 21;
+  if (G.kind == JX_GREEN_WILLOT) {
+    double k[DIM];
+    willot_k<DIM>(G, b, k);
+    if (DIM == 2)
+      gamma0_2d(k[0], k[1], G.daux1, G.daux2, G.daux3, G.daux4, g);
+    else
+      gamma0_3d(k[0], k[1], k[DIM - 1], G.daux1, G.daux2, G.daux3, G.daux4, g);
+    return;
+  }
   if (G.kind == JX_GREEN_TRUNCATED) {
     if (DIM == 2)
       gamma0_2d(G.ax[0].kt[b[0]], G.ax[1].kt[b[1]], G.daux1, G.daux2, G.daux3,
@@ -211,10 +239,14 @@ __device__ __forceinline__ void gamma_hat_moments(const GreenDev &G,
   for (int i = 0; i < NA; ++i) A[i] = 0.;
 #pragma unroll
   for (int i = 0; i < NB; ++i) B[i] = 0.;
-  if (G.kind == JX_GREEN_TRUNCATED) {
+  if (G.kind == JX_GREEN_TRUNCATED || G.kind == JX_GREEN_WILLOT) {
     double k[DIM];
+    if (G.kind == JX_GREEN_WILLOT) {
+      willot_k<DIM>(G, b, k);
+    } else {
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) k[i] = G.ax[i].kt[b[i]];
+      for (int i = 0; i < DIM; ++i) k[i] = G.ax[i].kt[b[i]];
+    }
     if constexpr (DIM == 2) moments_add2(k, 1.0, A, B);
     else moments_add3(k, 1.0, A, B);
   } else {

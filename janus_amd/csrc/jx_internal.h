@@ -39,14 +39,15 @@ struct FftLen {
 //   kt: truncated wave number  s (b - n) if 2b > n else s b   (green.pyx:432-445)
 //   km, kp: filtered aliases    s (b - n), s b                 (green.pyx:740-800)
 //   wm, wp: filtered weights    cos^2(h k / 4) of km, kp
+//   sw, cw: Willot's sin, cos of phi = s h b / 2              (green.pyx:984-1023)
 struct AxisTab {
   const double *kt = nullptr, *km = nullptr, *kp = nullptr, *wm = nullptr,
-               *wp = nullptr;
+               *wp = nullptr, *sw = nullptr, *cw = nullptr;
 };
 
 struct GreenDev {
   int dim = 0;
-  int kind = 0;  // JX_GREEN_TRUNCATED / JX_GREEN_FILTERED
+  int kind = 0;  // JX_GREEN_TRUNCATED / JX_GREEN_FILTERED / JX_GREEN_WILLOT
   double daux1 = 0, daux2 = 0, daux3 = 0, daux4 = 0;
   AxisTab ax[kMaxDim];
 };

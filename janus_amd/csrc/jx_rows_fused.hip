@@ -17,10 +17,16 @@ namespace jx {
 #ifndef JX_FUSED_MIN_WAVES
 #define JX_FUSED_MIN_WAVES 3
 #endif
+// Same for the 8-value plans (1: the compiler's choice).
+#ifndef JX_FUSED8_MIN_WAVES
+#define JX_FUSED8_MIN_WAVES 1
+#endif
 
 template <class P, int C, int NP, int MODE>
 __global__ void __launch_bounds__(NP * reg_tpp<P>(),
-                                  reg_v<P>() == 16 ? JX_FUSED_MIN_WAVES : 1)
+                                  reg_v<P>() == 16 ? JX_FUSED_MIN_WAVES
+                                  : reg_v<P>() == 8 ? JX_FUSED8_MIN_WAVES
+                                                    : 1)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
                               double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st,

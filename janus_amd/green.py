@@ -11,6 +11,10 @@ never stored.
   2 b_i > n_i else s_i b_i, s_i = 2 pi / (h n_i) (green.pyx:432-451).
 * filtered: sum over the 2^dim aliases (b_i - n_i or b_i) of
   prod_i cos^2(h k_i / 4) Gamma0(k) (green.pyx:607-694, 730-910).
+* willot2015: Gamma0 of Willot's modified wave vector
+  k_i = sin(phi_i) prod_{j != i} cos(phi_j), phi_i = s_i h b_i / 2
+  (green.pyx:976-1036): the symbol of the rotated-grid finite-difference
+  gradient, up to a common complex factor.
 
 The per-frequency API (``set_frequency``, ``to_memoryview``,
 ``apply_by_freq``; green.pyx:146-180, 312-351) is evaluated by the host code
@@ -40,6 +44,15 @@ def filtered(green, n, h, transform=None):
         return FilteredGreenOperator2D(green, n, h, transform)
     elif green.dim == 3:
         return FilteredGreenOperator3D(green, n, h, transform)
+    else:
+        raise ValueError('dim must be 2 or 3 (was {0})'.format(green.dim))
+
+
+def willot2015(green, n, h, transform=None):
+    if green.dim == 2:
+        return FiniteDifferences2D(green, n, h, transform)
+    elif green.dim == 3:
+        return FiniteDifferences3D(green, n, h, transform)
     else:
         raise ValueError('dim must be 2 or 3 (was {0})'.format(green.dim))
 
@@ -219,3 +232,11 @@ class FilteredGreenOperator2D(DiscreteGreenOperator2D):
 
 class FilteredGreenOperator3D(DiscreteGreenOperator3D):
     _kind = _lib.JX_GREEN_FILTERED
+
+
+class FiniteDifferences2D(DiscreteGreenOperator2D):
+    _kind = _lib.JX_GREEN_WILLOT
+
+
+class FiniteDifferences3D(DiscreteGreenOperator3D):
+    _kind = _lib.JX_GREEN_WILLOT

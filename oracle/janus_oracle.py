@@ -25,6 +25,9 @@ rfft / irfft                 fft/serial/_serial_fft.pyx:152-177, 296-325     num
                                                                              oracle's FFT *is* numpy's pocketfft
 adjoint_gradient             (no janus counterpart) implicit                 central finite differences of basic_scheme
                              differentiation of the fixed point
+willot_wavevector /          green.pyx:976-1036 (willot2015)                 Fourier symbol of the rotated-grid
+gamma_hat_willot                                                             finite-difference gradient (brute-force DFT
+                                                                             of the stencil); gamma0 pins
 cg_scheme                    examples/square_inclusion.py:40-47, 71-86       converged solution solves the Lippmann-
                              (Krylov on the same operators; strain form      Schwinger equation and equals the basic
                              in the C0 inner product, no janus counterpart)  scheme's; C0-conjugacy of the directions;
@@ -229,11 +232,36 @@ def gamma_hat_filtered(b, n, h, g, nu):
     return acc
 
 
+def willot_wavevector(b, n, h):
+    """Willot's modified wave vector (green.pyx:984-1023):
+    k_i = sin(phi_i) prod_{j != i} cos(phi_j), phi_i = s_i h b_i / 2, for
+    the full-grid index b (no aliasing)."""
+    b = np.asarray(b, dtype=np.float64)
+    s = np.asarray(_steps(n, h))
+    phi = 0.5 * s * h * b
+    sn, cs = np.sin(phi), np.cos(phi)
+    dim = len(n)
+    k = np.empty(b.shape)
+    for i in range(dim):
+        k[..., i] = sn[..., i]
+        for j in range(dim):
+            if j != i:
+                k[..., i] = k[..., i] * cs[..., j]
+    return k
+
+
+def gamma_hat_willot(b, n, h, g, nu):
+    """Gamma-hat(b) of janus's willot2015 operator (green.pyx:976-1036)."""
+    return gamma0(willot_wavevector(b, n, h), g, nu)
+
+
 def gamma_hat(kind, b, n, h, g, nu):
     if kind == 'truncated':
         return gamma_hat_truncated(b, n, h, g, nu)
     if kind == 'filtered':
         return gamma_hat_filtered(b, n, h, g, nu)
+    if kind == 'willot':
+        return gamma_hat_willot(b, n, h, g, nu)
     raise ValueError(kind)
 
 

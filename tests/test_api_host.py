@@ -299,3 +299,23 @@ def test_mandelvoigt():
     with pytest.raises(ValueError):
         mv.unravel_index(6)
     assert MandelVoigt(2).ravel_multi_index(1, 0) == 2
+
+
+@pytest.mark.parametrize('n', [(8, 6), (7, 5), (6, 4, 5), (5, 8, 3)])
+def test_willot_per_frequency_api(n):
+    """to_memoryview / apply_by_freq of willot2015 (green.pyx:976-1036)
+    against the oracle, whose wave vector is pinned by the finite-difference
+    stencil's symbol (test_oracle.py)."""
+    from janus_amd.green import willot2015
+    from oracle import janus_oracle as jo
+    dim = len(n)
+    greend = willot2015(material.create(MU, NU, dim).green_operator(), n, 0.5)
+    rng = np.random.default_rng(1)
+    for b in multi_indices(n):
+        expected = jo.gamma_hat('willot', np.array(b), n, 0.5, MU, NU)
+        greend.set_frequency(b)
+        np.testing.assert_allclose(greend.to_memoryview(), expected,
+                                   rtol=1e-13, atol=1e-15)
+        tau = rng.standard_normal(expected.shape[0])
+        np.testing.assert_allclose(greend.apply_by_freq(tau), expected @ tau,
+                                   rtol=1e-13, atol=1e-14)

@@ -14,7 +14,7 @@ import janus_amd.fft.serial as fft
 import janus_amd.material.elastic.linear.isotropic as material
 from janus_amd import microstructure as ms
 from janus_amd.batch import BatchSolver, effective_stiffness_batched
-from janus_amd.green import filtered, truncated
+from janus_amd.green import filtered, truncated, willot2015
 from janus_amd.operators import BlockDiagonalOperator2D, \
     BlockDiagonalOperator3D
 from janus_amd.solver import LSSolver, delta_stiffness, effective_stiffness
@@ -22,7 +22,8 @@ from oracle import janus_oracle as jo
 
 pytestmark = pytest.mark.gpu
 
-FACTORIES = {'truncated': truncated, 'filtered': filtered}
+FACTORIES = {'truncated': truncated, 'filtered': filtered,
+             'willot': willot2015}
 
 
 def make_solver(phase, contrast, kind, nu_i=0.25, fused=True):

@@ -10,7 +10,7 @@ import pytest
 import janus_amd.fft.serial as fft
 import janus_amd.material.elastic.linear.isotropic as material
 from janus_amd import microstructure as ms
-from janus_amd.green import filtered, truncated
+from janus_amd.green import filtered, truncated, willot2015
 from janus_amd.operators import BlockDiagonalOperator2D, \
     BlockDiagonalOperator3D
 from janus_amd.solver import LSSolver, delta_stiffness
@@ -18,7 +18,8 @@ from oracle import janus_oracle as jo
 
 pytestmark = pytest.mark.gpu
 
-FACTORIES = {'truncated': truncated, 'filtered': filtered}
+FACTORIES = {'truncated': truncated, 'filtered': filtered,
+             'willot': willot2015}
 
 
 def build(phase, mats, mat0, kind):
@@ -40,6 +41,9 @@ CASES = [
     (2, 80, 'filtered', (1.0, 0.3), (5.0, 0.3)),
     (2, 96, 'truncated', (1.0, 0.3), (5.0, 0.3)),
     (2, 160, 'filtered', (1.0, 0.3), (3.0, 0.3)),
+    (2, 30, 'willot', (1.0, 0.3), (10.0, 0.3)),
+    (3, 16, 'willot', (1.0, 0.3), (20.0, 0.25)),
+    (3, 13, 'willot', (1.0, 0.3), (5.0, 0.3)),
 ]
 
 

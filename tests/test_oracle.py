@@ -195,7 +195,7 @@ def test_gamma_hat_brute_force(kind, n):
         np.testing.assert_allclose(a, expected, rtol=1e-13, atol=1e-15)
 
 
-@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+@pytest.mark.parametrize('kind', ['truncated', 'filtered', 'willot'])
 @pytest.mark.parametrize('n', [(16, 16), (15, 8), (16, 12, 10), (9, 7, 6)])
 def test_gamma_hat_axis0_mirror(kind, n):
     """Gamma-hat(n0 - b0, b1, ...) = S Gamma-hat(b0, b1, ...) S with S = -1
@@ -211,8 +211,9 @@ def test_gamma_hat_axis0_mirror(kind, n):
     bm[:, 0] = n[0] - b[:, 0]
     g = jo.gamma_hat(kind, b, n, 1.0, 0.75, 0.3)
     gm = jo.gamma_hat(kind, bm, n, 1.0, 0.75, 0.3)
+    # (sin(pi - phi) and sin(phi) may differ by an ulp in Willot's vector)
     np.testing.assert_allclose(S @ g @ S, gm, rtol=0,
-                               atol=1e-15 * np.abs(gm).max())
+                               atol=4e-15 * np.abs(gm).max())
     if n[0] % 2 == 0:
         # at b0 = n0 / 2 the truncated operator is not mirror-symmetric
         # (green.pyx:432-445 maps it to +pi / h only)
@@ -418,3 +419,54 @@ def test_c0_inner_matches_matrix():
         C0 = jo.iso4_matrix(dim * k0, 2 * g0, dim)
         assert np.isclose(jo.c0_inner(a, b, g0, nu0),
                           np.einsum('vi,ij,vj->', a, C0, b), rtol=1e-13)
+
+
+# ---------------------------------------------------------------- Willot
+
+def _rotated_gradient_symbol(n, h, axis):
+    """DFT symbol of Willot's rotated-grid derivative along `axis`: the
+    forward difference along `axis` averaged over the 2^(dim-1) corners of
+    the cell in the other directions, applied to a unit impulse."""
+    dim = len(n)
+    f = np.zeros(n)
+    f[(0,) * dim] = 1.0
+    d = np.zeros(n)
+    others = [a for a in range(dim) if a != axis]
+    for corner in itertools.product((0, 1), repeat=dim - 1):
+        shift = [0] * dim
+        for a, c in zip(others, corner):
+            shift[a] = c
+        g0 = np.roll(f, [-s for s in shift], axis=tuple(range(dim)))
+        shift[axis] = 1
+        g1 = np.roll(f, [-s for s in shift], axis=tuple(range(dim)))
+        d += (g1 - g0)
+    d /= (2 ** (dim - 1)) * h
+    return np.fft.fftn(d)
+
+
+@pytest.mark.parametrize('n', [(8, 6), (7, 5), (6, 4, 5), (4, 4, 4)])
+def test_willot_wavevector_is_rotated_gradient_symbol(n):
+    """S_i(b) = (2 i / h) exp(i sum_j phi_j) k_i(b) with k janus's Willot
+    wave vector (green.pyx:984-1023), phi_j = pi b_j / n_j."""
+    h = 0.37
+    dim = len(n)
+    bs = np.array(list(itertools.product(*[range(m) for m in n])))
+    k = jo.willot_wavevector(bs, n, h)
+    phase = np.exp(1j * np.pi * (bs / np.asarray(n)).sum(axis=1))
+    for axis in range(dim):
+        S = _rotated_gradient_symbol(n, h, axis)[tuple(bs.T)]
+        np.testing.assert_allclose(S, 2j / h * phase * k[:, axis], rtol=0,
+                                   atol=1e-13 / h)
+
+
+@pytest.mark.parametrize('n', [(8, 6), (6, 4, 5)])
+def test_willot_gamma_hat_is_a_projector(n):
+    """Gamma-hat C0 Gamma-hat = Gamma-hat at every frequency (one wave
+    vector, no alias sum): the operator is exactly C0-self-adjoint."""
+    g, nu = 0.75, 0.3
+    dim = len(n)
+    k0 = jo.bulk_modulus(g, nu, dim)
+    C0 = jo.iso4_matrix(dim * k0, 2 * g, dim)
+    bs = np.array(list(itertools.product(*[range(m) for m in n])))
+    G = jo.gamma_hat('willot', bs, n, 1.0, g, nu)
+    np.testing.assert_allclose(G @ C0 @ G, G, rtol=0, atol=1e-14)
