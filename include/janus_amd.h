@@ -114,6 +114,39 @@ int jx_local_means(jx_local *local, const double *x, double *out_tau,
                    double *out_eps, void *stream);
 
 /* ---------------------------------------------------------------------
+ * Anisotropic local operators: a general sym x sym matrix per voxel, as a
+ * janus BlockDiagonalOperator2D/3D whose loc entries are any sym x sym
+ * linear operators (e.g. FourthRankCubicTensor2D, operators.pyx:411-450)
+ * or a square BlockDiagonalLinearOperator2D/3D (operators.pyx:719-890).
+ * y_i = sum_j a_ij x_j (operators.pyx:739-756); the handle works wherever
+ * a jx_local does: jx_local_apply, jx_local_means, jx_solver_*, jx_batch_*.
+ * Conjugate gradients (jx_solver_solve_cg) need symmetric matrices and
+ * fail with JX_EINVAL otherwise.
+ * ------------------------------------------------------------------- */
+/* phase: (n0, n1[, n2]) uint8 device array; mats: host array of nphase
+ * row-major sym x sym matrices, nphase in [1, 256] (copied). */
+int jx_local_create_phase_matrices(int dim, const int64_t *shape,
+                                   const uint8_t *phase, const double *mats,
+                                   int nphase, jx_local **out);
+/* mats: (n0, n1[, n2], sym, sym) device array, kept by reference. */
+int jx_local_create_matrix_field(int dim, const int64_t *shape,
+                                 const double *mats, jx_local **out);
+/* *out = 1 when every block of the operator is a symmetric matrix. */
+int jx_local_is_symmetric(jx_local *local, int *out);
+/* y = local^T : x, y_i = sum_j a_ji x_j (operators.pyx:758-775, 843-861);
+ * the isotropic modes are symmetric.  x == y allowed. */
+int jx_local_apply_transpose(jx_local *local, const double *x, double *y,
+                             void *stream);
+/* Block-diagonal linear operator with blocks of any size m x n
+ * (BlockDiagonalLinearOperator2D/3D, operators.pyx:719-890): a is the
+ * (n0, n1[, n2], m, n) device array; transpose = 0: x (.., n) -> y (.., m),
+ * y_i = sum_j a_ij x_j; transpose = 1: x (.., m) -> y (.., n),
+ * y_i = sum_j a_ji x_j.  x and y must not overlap. */
+int jx_block_linear_apply(int dim, const int64_t *shape, int m, int n,
+                          const double *a, const double *x, double *y,
+                          int transpose, void *stream);
+
+/* ---------------------------------------------------------------------
  * Basic (Moulinec-Suquet) fixed-point solver, the loop of
  * sphinx/tutorials/square_basic/square_basic.py:83-92:
  *     eps <- E + rhs - Gamma0[local : eps]

@@ -56,6 +56,16 @@ SIGNATURES = [
     ('jx_local_destroy', ctypes.c_int, [_VP]),
     ('jx_local_apply', ctypes.c_int, [_VP, _VP, _VP, _VP]),
     ('jx_local_means', ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    ('jx_local_create_phase_matrices', ctypes.c_int,
+     [ctypes.c_int, _c_int64_p, _VP, _VP, ctypes.c_int, _PP]),
+    ('jx_local_create_matrix_field', ctypes.c_int,
+     [ctypes.c_int, _c_int64_p, _VP, _PP]),
+    ('jx_local_is_symmetric', ctypes.c_int,
+     [_VP, ctypes.POINTER(ctypes.c_int)]),
+    ('jx_local_apply_transpose', ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    ('jx_block_linear_apply', ctypes.c_int,
+     [ctypes.c_int, _c_int64_p, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP,
+      ctypes.c_int, _VP]),
     ('jx_solver_create', ctypes.c_int, [_VP, _VP, _PP]),
     ('jx_solver_destroy', ctypes.c_int, [_VP]),
     ('jx_solver_solve', ctypes.c_int,

@@ -449,6 +449,69 @@ int jx_local_create_fields(int dim, const int64_t *shape, const double *sph,
   return JX_OK;
 }
 
+int jx_local_create_phase_matrices(int dim, const int64_t *shape,
+                                   const uint8_t *phase, const double *mats,
+                                   int nphase, jx_local **out) {
+  if (!out) return fail(JX_EINVAL, "out is NULL");
+  int rc = check_shape(dim, shape);
+  if (rc) return rc;
+  if (!phase || !mats) return fail(JX_EINVAL, "phase/mats is NULL");
+  if (nphase < 1 || nphase > 256) return fail(JX_EINVAL, "nphase must be in [1, 256]");
+  jx_local *l = new jx_local();
+  l->g = make_grid(dim, shape);
+  int C = dim == 2 ? 3 : 6;
+  l->sym = C;
+  size_t count = (size_t)nphase * C * C;
+  for (int p = 0; p < nphase; ++p)
+    for (int i = 0; i < C; ++i)
+      for (int j = i + 1; j < C; ++j)
+        if (mats[(size_t)p * C * C + i * C + j] != mats[(size_t)p * C * C + j * C + i])
+          l->symmetric = false;
+  hipError_t e = hipMalloc((void **)&l->table, sizeof(double) * count);
+  if (e == hipSuccess)
+    e = hipMemcpy(l->table, mats, sizeof(double) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    jx_local_destroy(l);
+    return hip_fail(e, "jx_local_create_phase_matrices");
+  }
+  l->dev.mode = LOCAL_PHASE_MATRIX;
+  l->dev.phase = phase;
+  l->dev.mat = l->table;
+  if ((rc = local_alloc_partials(l))) {
+    jx_local_destroy(l);
+    return rc;
+  }
+  *out = l;
+  return JX_OK;
+}
+
+int jx_local_create_matrix_field(int dim, const int64_t *shape, const double *mats,
+                                 jx_local **out) {
+  if (!out) return fail(JX_EINVAL, "out is NULL");
+  int rc = check_shape(dim, shape);
+  if (rc) return rc;
+  if (!mats) return fail(JX_EINVAL, "mats is NULL");
+  jx_local *l = new jx_local();
+  l->g = make_grid(dim, shape);
+  l->sym = dim == 2 ? 3 : 6;
+  l->dev.mode = LOCAL_MATRIX_FIELD;
+  l->dev.mat = mats;
+  if ((rc = local_alloc_partials(l)) ||
+      (rc = matrices_symmetric(l->sym, mats, l->g.N, (int *)l->partials,
+                               &l->symmetric))) {
+    jx_local_destroy(l);
+    return rc;
+  }
+  *out = l;
+  return JX_OK;
+}
+
+int jx_local_is_symmetric(jx_local *l, int *out) {
+  if (!l || !out) return fail(JX_EINVAL, "jx_local_is_symmetric: NULL argument");
+  *out = l->symmetric ? 1 : 0;
+  return JX_OK;
+}
+
 int jx_local_destroy(jx_local *l) {
   if (!l) return JX_OK;
   if (l->table) (void)hipFree(l->table);
@@ -460,6 +523,26 @@ int jx_local_destroy(jx_local *l) {
 int jx_local_apply(jx_local *l, const double *x, double *y, void *stream) {
   if (!l || !x || !y) return fail(JX_EINVAL, "jx_local_apply: NULL argument");
   return launch_local_apply(l->g, l->sym, l->dev, x, y, (hipStream_t)stream);
+}
+
+int jx_local_apply_transpose(jx_local *l, const double *x, double *y,
+                             void *stream) {
+  if (!l || !x || !y) return fail(JX_EINVAL, "jx_local_apply_transpose: NULL argument");
+  return launch_local_apply(l->g, l->sym, l->dev, x, y, (hipStream_t)stream, true);
+}
+
+int jx_block_linear_apply(int dim, const int64_t *shape, int m, int n,
+                          const double *a, const double *x, double *y,
+                          int transpose, void *stream) {
+  int rc = check_shape(dim, shape);
+  if (rc) return rc;
+  if (!a || !x || !y) return fail(JX_EINVAL, "jx_block_linear_apply: NULL argument");
+  if (m < 1 || n < 1) return fail(JX_EINVAL, "block sizes must be >= 1");
+  if ((const double *)y == x)
+    return fail(JX_EINVAL, "jx_block_linear_apply: x and y must not overlap");
+  int64_t N = 1;
+  for (int d = 0; d < dim; ++d) N *= shape[d];
+  return launch_block_linear(N, m, n, a, x, y, transpose != 0, (hipStream_t)stream);
 }
 
 int jx_local_means(jx_local *l, const double *x, double *out_tau, double *out_eps,

@@ -87,10 +87,11 @@ struct jx_batch {
 namespace {
 
 using Key = std::tuple<int, int64_t, int64_t, int64_t, int, uint64_t, bool,
-                       bool, int>;
+                       bool, int, int>;
 
 // Entries share launches when they have the same grid, kind, h (the axis
-// tables), rhs use, schedule and plan variants (the rounding).
+// tables), rhs use, schedule, plan variants (the rounding) and family of
+// local operator (the kernel instantiation).
 Key entry_key(const Entry &e) {
   const jx_green *G = e.S->green;
   uint64_t hb = 0;
@@ -99,7 +100,7 @@ Key entry_key(const Entry &e) {
   int plans = 0;
   for (int p = 0; p < kPlanPasses; ++p) plans |= G->g.plan[p] << p;
   return Key(G->g.dim, G->g.n[0], G->g.n[1], G->g.n[2], G->dev.kind, hb,
-             e.rhs != nullptr, fused, plans);
+             e.rhs != nullptr, fused, plans, local_family(e.S->local->dev));
 }
 
 void build_groups(jx_batch *B) {

@@ -319,6 +319,8 @@ int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E,
                        int *status, void *stream) {
   if (!S || !eps || !E) return fail(JX_EINVAL, "jx_solver_solve_cg: NULL argument");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
+  if (!S->local->symmetric)
+    return fail(JX_EINVAL, "conjugate gradients need a symmetric local operator");
   hipStream_t s = (hipStream_t)stream;
   jx_green *G = S->green;
   const Grid &g = G->g;

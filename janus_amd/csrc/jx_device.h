@@ -39,10 +39,36 @@ __device__ __forceinline__ void iso4(double tr, double dev, const double *e,
   }
 }
 
-// Phase tables hold (tr, dev) per phase; field mode computes tr per voxel.
+// t = a e (TRANSPOSE: t = a^T e) for one C x C row-major matrix; each
+// output is accumulated over j = 0, 1, ... in the order of janus's c_apply
+// and c_apply_transpose loops (operators.pyx:739-756, 758-775).
+template <int C, bool TRANSPOSE>
+__device__ __forceinline__ void mat_voxel(const double *__restrict__ a,
+                                          const double *e, double *t) {
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < C; ++j)
+      acc += (TRANSPOSE ? a[j * C + i] : a[i * C + j]) * e[j];
+    t[i] = acc;
+  }
+}
+
+// The C x C matrix of voxel vox in the matrix modes.
 template <int C>
-__device__ __forceinline__ void local_voxel(const LocalDev &loc, int64_t vox,
-                                            const double *e, double *t) {
+__device__ __forceinline__ const double *local_matrix(const LocalDev &loc,
+                                                      int64_t vox) {
+  int64_t k = loc.mode == LOCAL_PHASE_MATRIX ? (int64_t)loc.phase[vox] : vox;
+  return loc.mat + k * (C * C);
+}
+
+// Isotropic modes: phase tables hold (tr, dev) per phase; field mode
+// computes tr per voxel.
+template <int C>
+__device__ __forceinline__ void local_voxel_iso(const LocalDev &loc,
+                                                int64_t vox, const double *e,
+                                                double *t) {
   double tr, dev;
   if (loc.mode == LOCAL_PHASES) {
     int p = loc.phase[vox];
@@ -54,6 +80,26 @@ __device__ __forceinline__ void local_voxel(const LocalDev &loc, int64_t vox,
     tr = (sph - dev) / (C == 3 ? 2.0 : 3.0);
   }
   iso4<C>(tr, dev, e, t);
+}
+
+// t = local : e at voxel vox, for a local operator of family LF.
+template <int C, int LF>
+__device__ __forceinline__ void local_voxel_f(const LocalDev &loc, int64_t vox,
+                                              const double *e, double *t) {
+  if constexpr (LF == LF_MATRIX)
+    mat_voxel<C, false>(local_matrix<C>(loc, vox), e, t);
+  else
+    local_voxel_iso<C>(loc, vox, e, t);
+}
+
+// Any mode, chosen at run time (kernels that are not register bound).
+template <int C>
+__device__ __forceinline__ void local_voxel(const LocalDev &loc, int64_t vox,
+                                            const double *e, double *t) {
+  if (loc.mode >= LOCAL_PHASE_MATRIX)
+    local_voxel_f<C, LF_MATRIX>(loc, vox, e, t);
+  else
+    local_voxel_f<C, LF_ISO>(loc, vox, e, t);
 }
 
 // ------------------------------------------------------------ update

@@ -26,8 +26,9 @@ struct jx_local {
   jx::Grid g;
   int sym = 0;
   jx::LocalDev dev;
-  double *table = nullptr;     // device (tr, dev) per phase
+  double *table = nullptr;     // device (tr, dev) per phase, or matrices
   double *partials = nullptr;  // device, 1024 blocks x 2 sym columns + 2 sym
+  bool symmetric = true;       // every block is a symmetric matrix
 };
 
 namespace jx {

@@ -52,16 +52,34 @@ struct GreenDev {
   AxisTab ax[kMaxDim];
 };
 
-// Local operator modes.
-enum LocalMode { LOCAL_NONE = 0, LOCAL_PHASES = 1, LOCAL_FIELDS = 2 };
+// Local operator modes: isotropic tensors sph J + dev K from a phase map and
+// a (tr, dev) table or from per-voxel fields; general C x C matrices from a
+// phase map and a matrix table or from a per-voxel matrix field.
+enum LocalMode {
+  LOCAL_NONE = 0,
+  LOCAL_PHASES = 1,
+  LOCAL_FIELDS = 2,
+  LOCAL_PHASE_MATRIX = 3,
+  LOCAL_MATRIX_FIELD = 4
+};
 
 struct LocalDev {
   int mode = LOCAL_NONE;
   const uint8_t *phase = nullptr;
-  const double *table = nullptr;  // device, 2 * nphase (sph, dev) pairs
+  const double *table = nullptr;  // device, 2 * nphase (tr, dev) pairs
   const double *sph = nullptr;
   const double *dev = nullptr;
+  const double *mat = nullptr;    // device, C * C per phase or per voxel
 };
+
+// Families of local operators that the register-heavy kernels are compiled
+// for separately, so that the isotropic kernels keep their register budget:
+// LF_ISO covers LOCAL_PHASES and LOCAL_FIELDS, LF_MATRIX the matrix modes.
+enum LocalFamily { LF_ISO = 0, LF_MATRIX = 1 };
+
+__host__ __device__ inline int local_family(const LocalDev &loc) {
+  return loc.mode >= LOCAL_PHASE_MATRIX ? LF_MATRIX : LF_ISO;
+}
 
 // Device-side state of a running solve.
 struct SolveState {
@@ -169,7 +187,17 @@ int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,
 int launch_finalize(const double *partials, int nblocks, SolveState *st,
                     double *res_hist, hipStream_t s, const Batch *b = nullptr);
 int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
-                       const double *x, double *y, hipStream_t s);
+                       const double *x, double *y, hipStream_t s,
+                       bool transpose = false);
+// *out = every one of the count C x C device matrices a is symmetric
+// (synchronous; flag_dev: one device int of scratch).
+int matrices_symmetric(int ncomp, const double *a, int64_t count, int *flag_dev,
+                       bool *out);
+// y = a x (or a^T x) voxel by voxel for a per-voxel (m, n) matrix field of
+// any size (x, y must not overlap).
+int launch_block_linear(int64_t N, int m, int n, const double *a,
+                        const double *x, double *y, bool transpose,
+                        hipStream_t s);
 int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,
                             const double *mu, double *dsph, double *ddev,
                             hipStream_t s);

@@ -51,7 +51,9 @@ __global__ void __launch_bounds__(kFinalizeThreads)
 }
 
 // ------------------------------------------------------------ local apply
-template <int C>
+// TRANSPOSE applies the transposed local operator; the isotropic modes are
+// symmetric, so only the matrix modes differ.
+template <int C, bool TRANSPOSE>
 __global__ void __launch_bounds__(kThreads)
     k_local_apply(const double *x, double *y, int64_t N, LocalDev loc) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
@@ -59,7 +61,10 @@ __global__ void __launch_bounds__(kThreads)
     double e[C], t[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) e[c] = x[v * C + c];
-    local_voxel<C>(loc, v, e, t);
+    if (TRANSPOSE && loc.mode >= LOCAL_PHASE_MATRIX)
+      mat_voxel<C, true>(local_matrix<C>(loc, v), e, t);
+    else
+      local_voxel<C>(loc, v, e, t);
 #pragma unroll
     for (int c = 0; c < C; ++c) y[v * C + c] = t[c];
   }
@@ -169,16 +174,91 @@ int launch_finalize(const double *partials, int nblocks, SolveState *st,
 }
 
 int launch_local_apply(const Grid &g, int ncomp, const LocalDev &loc,
-                       const double *x, double *y, hipStream_t s) {
+                       const double *x, double *y, hipStream_t s,
+                       bool transpose) {
   int64_t nb = std::min<int64_t>((g.N + kThreads - 1) / kThreads, 65536);
-  if (ncomp == 3)
-    hipLaunchKernelGGL(k_local_apply<3>, dim3((unsigned)nb), dim3(kThreads), 0,
-                       s, x, y, g.N, loc);
+  dim3 grid((unsigned)nb), block(kThreads);
+  if (ncomp == 3 && !transpose)
+    hipLaunchKernelGGL((k_local_apply<3, false>), grid, block, 0, s, x, y, g.N, loc);
+  else if (ncomp == 3)
+    hipLaunchKernelGGL((k_local_apply<3, true>), grid, block, 0, s, x, y, g.N, loc);
+  else if (ncomp == 6 && !transpose)
+    hipLaunchKernelGGL((k_local_apply<6, false>), grid, block, 0, s, x, y, g.N, loc);
   else if (ncomp == 6)
-    hipLaunchKernelGGL(k_local_apply<6>, dim3((unsigned)nb), dim3(kThreads), 0,
-                       s, x, y, g.N, loc);
+    hipLaunchKernelGGL((k_local_apply<6, true>), grid, block, 0, s, x, y, g.N, loc);
   else
     return fail(JX_EINVAL, "local_apply: ncomp must be 3 or 6");
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+// Sets *flag to 1 when the matrix of some voxel (or phase) is not
+// symmetric; exact comparison.
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_asymmetric(const double *__restrict__ a, int64_t count, int *flag) {
+  int bad = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < count;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double *av = a + v * (C * C);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+#pragma unroll
+      for (int j = i + 1; j < C; ++j) bad |= av[i * C + j] != av[j * C + i];
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+int matrices_symmetric(int ncomp, const double *a, int64_t count, int *flag_dev,
+                       bool *out) {
+  JX_HIP(hipMemset(flag_dev, 0, sizeof(int)));
+  int64_t nb = std::max<int64_t>(1, std::min<int64_t>((count + kThreads - 1) / kThreads, 4096));
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_asymmetric<3>, dim3((unsigned)nb), dim3(kThreads), 0, 0,
+                       a, count, flag_dev);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_asymmetric<6>, dim3((unsigned)nb), dim3(kThreads), 0, 0,
+                       a, count, flag_dev);
+  else
+    return fail(JX_EINVAL, "matrices_symmetric: ncomp must be 3 or 6");
+  JX_HIP(hipGetLastError());
+  int flag = 0;
+  JX_HIP(hipMemcpy(&flag, flag_dev, sizeof(int), hipMemcpyDeviceToHost));
+  *out = flag == 0;
+  return JX_OK;
+}
+
+// Block-diagonal linear operator of any block size (operators.pyx:719-890):
+// one thread per voxel, the voxel's (m, n) matrix read row by row.
+__global__ void __launch_bounds__(kThreads)
+    k_block_linear(const double *__restrict__ a, const double *__restrict__ x,
+                   double *__restrict__ y, int64_t N, int m, int n,
+                   int transpose) {
+  int64_t mn = (int64_t)m * n;
+  int xs = transpose ? m : n, ys = transpose ? n : m;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double *av = a + v * mn;
+    const double *xv = x + v * xs;
+    double *yv = y + v * ys;
+    for (int i = 0; i < ys; ++i) {
+      double acc = 0.0;
+      if (transpose)
+        for (int j = 0; j < xs; ++j) acc += av[(int64_t)j * n + i] * xv[j];
+      else
+        for (int j = 0; j < xs; ++j) acc += av[(int64_t)i * n + j] * xv[j];
+      yv[i] = acc;
+    }
+  }
+}
+
+int launch_block_linear(int64_t N, int m, int n, const double *a,
+                        const double *x, double *y, bool transpose,
+                        hipStream_t s) {
+  if (N <= 0) return JX_OK;
+  int64_t nb = std::min<int64_t>((N + kThreads - 1) / kThreads, 65536);
+  hipLaunchKernelGGL(k_block_linear, dim3((unsigned)nb), dim3(kThreads), 0, s,
+                     a, x, y, N, m, n, (int)transpose);
   JX_HIP(hipGetLastError());
   return JX_OK;
 }

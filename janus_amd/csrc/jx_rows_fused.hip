@@ -22,7 +22,7 @@ namespace jx {
 #define JX_FUSED8_MIN_WAVES 1
 #endif
 
-template <class P, int C, int NP, int MODE>
+template <class P, int C, int NP, int MODE, int LF>
 __global__ void __launch_bounds__(NP * reg_tpp<P>(),
                                   reg_v<P>() == 16 ? JX_FUSED_MIN_WAVES
                                   : reg_v<P>() == 8 ? JX_FUSED8_MIN_WAVES
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
       int q = r * C + c;
       e[c] = ((const double *)&smem[(q >> 1) * LS + lp(j)])[q & 1];
     }
-    local_voxel<C>(loc, vox, e, tt);
+    local_voxel_f<C, LF>(loc, vox, e, tt);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       int q = r * C + c;
@@ -268,15 +268,20 @@ static int launch_rows_update_forward_reg_lc(const Grid &g, double *spec,
     const BatchRef br = batch_ref(b, nbe);
     dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<P>());
     double scale = 1.0 / (double)g.N;
-    if (la.rhs) {
-      set_lds_attr(k_rows_update_forward_reg<P, C, NP, 2>, lds);
-      hipLaunchKernelGGL((k_rows_update_forward_reg<P, C, NP, 2>), grid, block,
-                         lds, s, spec, y, g.rows, scale, W, la, loc, st, br);
-    } else {
-      set_lds_attr(k_rows_update_forward_reg<P, C, NP, 1>, lds);
-      hipLaunchKernelGGL((k_rows_update_forward_reg<P, C, NP, 1>), grid, block,
-                         lds, s, spec, y, g.rows, scale, W, la, loc, st, br);
-    }
+    auto go = [&](auto kernel) {
+      set_lds_attr(kernel, lds);
+      hipLaunchKernelGGL(kernel, grid, block, lds, s, spec, y, g.rows, scale, W,
+                         la, loc, st, br);
+    };
+    const bool mat = local_family(loc) == LF_MATRIX;
+    if (la.rhs && mat)
+      go(k_rows_update_forward_reg<P, C, NP, 2, LF_MATRIX>);
+    else if (la.rhs)
+      go(k_rows_update_forward_reg<P, C, NP, 2, LF_ISO>);
+    else if (mat)
+      go(k_rows_update_forward_reg<P, C, NP, 1, LF_MATRIX>);
+    else
+      go(k_rows_update_forward_reg<P, C, NP, 1, LF_ISO>);
     JX_HIP(hipGetLastError());
     return JX_OK;
   } else {

@@ -84,7 +84,8 @@ __global__ void __launch_bounds__(kThreads)
 // ones (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / V
 // threads per pair.  Batched: the entry's iterate, spectrum, local operator
 // and state come from its slot.
-template <class P, int C, int NP, bool LOCAL>
+// LOCAL: 0 no local operator, else 1 + its LocalFamily.
+template <class P, int C, int NP, int LOCAL>
 __global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
                        int64_t nrows, const double2 *W, LocalDev loc,
@@ -125,8 +126,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
 #pragma unroll
         for (int c = 0; c < C; ++c) e[c] = x[vox * C + c];
       }
-      if (LOCAL) {
-        local_voxel<C>(loc, vox, e, t);
+      if constexpr (LOCAL != 0) {
+        local_voxel_f<C, LOCAL - 1>(loc, vox, e, t);
       } else {
 #pragma unroll
         for (int c = 0; c < C; ++c) t[c] = e[c];
@@ -212,19 +213,24 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
   const int64_t nbe = (g.rows + RB - 1) / RB;
   const BatchRef br = batch_ref(b, nbe);
   dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<P>());
-  // a batch always applies the local operator of its slots
+  auto go = [&](auto kernel) {
+    set_lds_attr(kernel, lds);
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, x, spec, g.rows, W, loc, st,
+                       br);
+  };
+  // a batch always applies the local operator of its slots (all of one
+  // family: the first slot's, which the caller passes as loc)
   if (b || loc.mode != LOCAL_NONE) {
     if constexpr (C == 3 || C == 6) {
-      set_lds_attr(k_rows_forward_reg<P, C, NP, true>, lds);
-      hipLaunchKernelGGL((k_rows_forward_reg<P, C, NP, true>), grid, block, lds,
-                         s, x, spec, g.rows, W, loc, st, br);
+      if (local_family(loc) == LF_MATRIX)
+        go(k_rows_forward_reg<P, C, NP, 1 + LF_MATRIX>);
+      else
+        go(k_rows_forward_reg<P, C, NP, 1 + LF_ISO>);
     } else {
       return fail(JX_EINVAL, "local operator needs sym components");
     }
   } else {
-    set_lds_attr(k_rows_forward_reg<P, C, NP, false>, lds);
-    hipLaunchKernelGGL((k_rows_forward_reg<P, C, NP, false>), grid, block, lds,
-                       s, x, spec, g.rows, W, loc, st, br);
+    go(k_rows_forward_reg<P, C, NP, 0>);
   }
   JX_HIP(hipGetLastError());
   return JX_OK;

@@ -19,8 +19,16 @@ gamma_hat_filtered           green.pyx:607-694, 730-910                      bru
 green_apply                  green.pyx:182-216, 353-390, 453-495, 531-580    18 fixtures (rtol=ULP, atol=10 ULP)
 iso4_apply                   operators.pyx:292-299, 315-324, 352-365         J/K projector matrices (test_operators.py:180-227)
 block_diag_apply             operators.pyx:655-664, 706-716                  per-voxel loop (test_operators.py:409-450)
+cubic_tensor_2d              operators.pyx:411-450                           rotation of the 4th-rank cubic tensor in
+                                                                             index notation, T'_ijkl = R R R R T
+                                                                             (test_operators.py:238-320 cases)
+matrix_apply                 operators.pyx:739-775, 824-861                  explicit per-voxel loops on a tiny grid;
+                                                                             <y, A x> = <A^T y, x>
+                                                                             (test_operators.py:452-505)
 basic_scheme                 sphinx/tutorials/square_basic/square_basic.py   published C_1212 = 1.41903971282
-                             :17-97                                          (square_basic.rst:227)
+                             :17-97                                          (square_basic.rst:227); matrix form (mats)
+                                                                             = isotropic form on iso4_matrix blocks;
+                                                                             laminate closed form (Voigt/Reuss mix)
 rfft / irfft                 fft/serial/_serial_fft.pyx:152-177, 296-325     numpy.fft (test_fft.py:37-70) -- the
                                                                              oracle's FFT *is* numpy's pocketfft
 adjoint_gradient             (no janus counterpart) implicit                 central finite differences of basic_scheme
@@ -356,6 +364,43 @@ def iso4_matrix(sph, dev, dim):
     return a
 
 
+def cubic_tensor_2d(t1111, t1122, t1212, theta):
+    """Matrix of the 2D cubic tensor rotated by theta, from the closed form
+    in cos 4 theta, sin 4 theta (operators.pyx:413-426, 440-450)."""
+    c = np.cos(4. * theta)
+    s = np.sin(4. * theta)
+    aux = t1111 - t1122 - 2. * t1212
+    t11 = 0.25 * (3. * t1111 + t1122 + 2. * t1212 + c * aux)
+    t12 = 0.25 * (t1111 + 3. * t1122 - 2. * t1212 - c * aux)
+    t13 = 0.25 * SQRT2 * s * aux
+    t23 = -t13
+    t33 = 0.5 * (t1111 - t1122 + 2. * t1212 - c * aux)
+    return np.array([[t11, t12, t13], [t12, t11, t23], [t13, t23, t33]])
+
+
+def matrix_apply(a, x, transpose=False):
+    """Block-diagonal linear operator (operators.pyx:739-775, 824-861):
+    y[v]_i = sum_j a[v]_ij x[v]_j, or sum_j a[v]_ji x[v]_j transposed.
+    ``a`` has shape grid + (m, n) (or (m, n), broadcast over the grid)."""
+    a = np.asarray(a, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    if transpose:
+        return np.einsum('...ji,...j->...i', a, x)
+    return np.einsum('...ij,...j->...i', a, x)
+
+
+def _local(sph, dev, mats):
+    """The local operator x -> dC : x of a scheme: isotropic per-voxel
+    (sph, dev), or per-voxel matrices ``mats`` (grid + (sym, sym))."""
+    if mats is not None:
+        return lambda x: matrix_apply(mats, x)
+    return lambda x: iso4_apply(sph, dev, x)
+
+
+def _grid_shape(sph, mats):
+    return np.shape(sph) if mats is None else np.shape(mats)[:-2]
+
+
 def block_diag_apply(phase, table, x):
     """Block-diagonal isotropic operator from a phase map (operators.pyx
     :655-664, 706-716): y[v] = iso4(table[phase[v]]) : x[v]."""
@@ -370,18 +415,20 @@ def block_diag_apply(phase, table, x):
 # --------------------------------------------------------------------------
 
 def basic_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
-                 rhs=None, eps0=None):
+                 rhs=None, eps0=None, mats=None):
     """Moulinec-Suquet iteration eps <- R - Gamma0[dC : eps].
 
-    ``sph``/``dev`` are per-voxel fields (shape = grid) of dC = C - C0.
-    R = E (macroscopic strain, constant) plus the optional field ``rhs``.
+    ``sph``/``dev`` are per-voxel fields (shape = grid) of dC = C - C0, or
+    ``mats`` holds dC as per-voxel matrices (grid + (sym, sym); then sph
+    and dev are ignored).  R = E (macroscopic strain, constant) plus the optional field ``rhs``.
     The residual is ||eps_{k+1} - eps_k|| / (sqrt(N) ||E||)
     (square_basic.py:86, 91); when ||E|| = 0 the plain norm / sqrt(N) is used.
     Iteration stops after the first iteration whose residual is < tol, or
     after ``max_iter`` iterations.  Returns (eps, residual history).
     """
     E = np.asarray(E, dtype=np.float64)
-    shape = np.shape(sph)
+    shape = _grid_shape(sph, mats)
+    local = _local(sph, dev, mats)
     N = int(np.prod(shape))
     nE = np.linalg.norm(E)
     normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
@@ -389,7 +436,7 @@ def basic_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
     eps[...] = E if eps0 is None else eps0
     res = []
     for _ in range(max_iter):
-        tau = iso4_apply(sph, dev, eps)
+        tau = local(eps)
         new = green_apply(tau, kind, g0, nu0, h)
         new = E - new
         if rhs is not None:
@@ -402,12 +449,12 @@ def basic_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
     return eps, np.array(res)
 
 
-def mean_stress(sph, dev, eps, g0, nu0, E):
+def mean_stress(sph, dev, eps, g0, nu0, E, mats=None):
     """<sigma> = C0 : E + <dC : eps> (square_basic.rst:212-221)."""
     dim = 2 if eps.shape[-1] == 3 else 3
     k0 = bulk_modulus(g0, nu0, dim)
     c0 = iso4_matrix(dim * k0, 2 * g0, dim)
-    tau = iso4_apply(sph, dev, eps)
+    tau = _local(sph, dev, mats)(eps)
     return c0 @ np.asarray(E) + tau.reshape(-1, eps.shape[-1]).mean(axis=0)
 
 
@@ -471,7 +518,7 @@ def c0_inner(a, b, g0, nu0):
 
 
 def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
-              directions=None, rhs=None, res_scale=None):
+              directions=None, rhs=None, res_scale=None, mats=None):
     """Conjugate gradients on (I + Gamma0 dC) eps~ = -Gamma0[dC : E] for the
     fluctuation eps~ (eps = E + eps~), in the inner product
     <a, b> = sum a : C0 : b.
@@ -487,10 +534,12 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
     receiving the search directions (for tests).  ``rhs``: optional field
     added to E, as in basic_scheme: eps solves (I + Gamma0 dC) eps = E + rhs
     (rhs compatible and zero-mean, e.g. a Gamma0 output); ``res_scale``
-    replaces 1 / (sqrt(N) ||E||).  Returns (eps, residuals).
+    replaces 1 / (sqrt(N) ||E||).  ``mats``: dC as symmetric per-voxel
+    matrices, as in basic_scheme.  Returns (eps, residuals).
     """
     E = np.asarray(E, dtype=np.float64)
-    shape = np.shape(sph)
+    shape = _grid_shape(sph, mats)
+    local = _local(sph, dev, mats)
     N = int(np.prod(shape))
     nE = np.linalg.norm(E)
     normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
@@ -498,11 +547,11 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
         normalization = res_scale
 
     def op(v):
-        return v + green_apply(iso4_apply(sph, dev, v), kind, g0, nu0, h)
+        return v + green_apply(local(v), kind, g0, nu0, h)
 
     Ef = np.empty(shape + (E.size,), dtype=np.float64)
     Ef[...] = E
-    r = -green_apply(iso4_apply(sph, dev, Ef), kind, g0, nu0, h)
+    r = -green_apply(local(Ef), kind, g0, nu0, h)
     if rhs is not None:
         r = rhs + r
     x = np.zeros_like(r)
