@@ -117,6 +117,10 @@ def load(path=LIB_PATH):
             'janus_amd: {0} is missing; build it with '
             'python -c "import janus_amd.build as b; b.build_library()"'
             .format(path))
+    # torch's HIP runtime must be the process's one: loaded first, the
+    # library would bind libamdhip64 from the ROCm install and torch would
+    # then share it, and the first device call fails with hipErrorNoDevice.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     for name, restype, argtypes in SIGNATURES:
         fn = getattr(lib, name)
@@ -164,6 +168,7 @@ def require_gpu():
     if not torch.cuda.is_available():
         raise RuntimeError('janus_amd: no GPU visible; the HIP path is the '
                            'only implementation')
+    torch.cuda.init()
     lib()
 
 

@@ -126,6 +126,42 @@ __device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
   for (int c = 0; c < C; ++c) z[c * zs] = make_double2(yr[c], yi[c]);
 }
 
+// z <- D Gamma-hat D z for the C complex values z[c * zs] (real and
+// imaginary parts separately), D = diag(-1 on the Mandel components whose
+// bit is set in FLIP).  Negation is exact.
+// Each output row is stored as soon as it is summed (the inputs are all in
+// registers by then), which keeps the live set to g and the inputs.
+template <int C, int FLIP>
+__device__ __forceinline__ void gamma_apply_flip(const double *g, double2 *z,
+                                                 int zs) {
+  double xr[C], xi[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    double2 v = z[c * zs];
+    const bool f = (FLIP >> c) & 1;
+    xr[c] = f ? -v.x : v.x;
+    xi[c] = f ? -v.y : v.y;
+  }
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    double yr = 0., yi = 0.;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const double gij = g[tri_index(C, i < j ? i : j, i < j ? j : i)];
+      yr += gij * xr[j];
+      yi += gij * xi[j];
+    }
+    const bool f = (FLIP >> i) & 1;
+    z[i * zs] = f ? make_double2(-yr, -yi) : make_double2(yr, yi);
+  }
+}
+
+// Sign pattern of Gamma-hat(s k) = D Gamma-hat(k) D, D_(ij) = s_i s_j over
+// the Mandel components (3D: xx yy zz yz zx xy; 2D: xx yy xy), for k0
+// negated: zx and xy flip (2D: xy).
+constexpr int kFlip0_3 = 0x30;
+constexpr int kFlip0_2 = 0x4;
+
 // Gamma-hat at axis-0 frequencies e and L - e of inner index m, evaluated
 // once.  The truncated wave vector and the filtered alias set of L - e are
 // those of e with the axis-0 component negated (and the filtered weights
@@ -150,27 +186,8 @@ __device__ __forceinline__ void gamma_point_pair(const GreenDev &G, int e,
   }
   double g[NC];
   gamma_hat_moments<DIM>(G, b, g);
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    double2 *z = side ? z2 : z0;
-    if (z == nullptr) continue;
-    double xr[C], xi[C], yr[C], yi[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      double2 v = z[c * zs];
-      const bool flip = side && c >= C - (C == 3 ? 1 : 2);
-      xr[c] = flip ? -v.x : v.x;
-      xi[c] = flip ? -v.y : v.y;
-    }
-    sym_matvec<C>(g, xr, yr);
-    sym_matvec<C>(g, xi, yi);
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const bool flip = side && c >= C - (C == 3 ? 1 : 2);
-      z[c * zs] = flip ? make_double2(-yr[c], -yi[c])
-                       : make_double2(yr[c], yi[c]);
-    }
-  }
+  gamma_apply_flip<C, 0>(g, z0, zs);
+  if (z2 != nullptr) gamma_apply_flip<C, DIM == 2 ? kFlip0_2 : kFlip0_3>(g, z2, zs);
 }
 
 // Register-resident transform along a strided axis of the spectrum for the
@@ -256,7 +273,12 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
 #pragma unroll
     for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
     __syncthreads();
-    reg_fft<P, 1, TQ>(v, t, seq, Wt);
+    // the inverse transform reads the same twiddles as the forward one; an
+    // opaque copy of the table pointer keeps the compiler from holding them
+    // in registers across the Gamma-hat step (which spilled)
+    const double2 *Wi = Wt;
+    asm volatile("" : "+s"(Wi));
+    reg_fft<P, 1, TQ>(v, t, seq, Wi);
   }
   if (valid) {
 #pragma unroll

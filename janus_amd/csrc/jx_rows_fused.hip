@@ -17,16 +17,25 @@ namespace jx {
 #ifndef JX_FUSED_MIN_WAVES
 #define JX_FUSED_MIN_WAVES 3
 #endif
-// Same for the 8-value plans (1: the compiler's choice).
+// Same for the 8-value plans (1: the compiler's choice), and for the
+// 8-value 3D isotropic-family kernels without a field right-hand side,
+// which fit 5 waves per SIMD without spilling (6 blocks of 26 KB LDS per CU
+// instead of 5; the other 8-value kernels would spill there).
 #ifndef JX_FUSED8_MIN_WAVES
 #define JX_FUSED8_MIN_WAVES 1
+#endif
+#ifndef JX_FUSED8_ISO_MIN_WAVES
+#define JX_FUSED8_ISO_MIN_WAVES 5
 #endif
 
 template <class P, int C, int NP, int MODE, int LF>
 __global__ void __launch_bounds__(NP * reg_tpp<P>(),
                                   reg_v<P>() == 16 ? JX_FUSED_MIN_WAVES
-                                  : reg_v<P>() == 8 ? JX_FUSED8_MIN_WAVES
-                                                    : 1)
+                                  : reg_v<P>() == 8
+                                      ? (C == 6 && LF == LF_ISO && MODE == 1
+                                             ? JX_FUSED8_ISO_MIN_WAVES
+                                             : JX_FUSED8_MIN_WAVES)
+                                      : 1)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
                               double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st,
@@ -204,7 +213,11 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
 #pragma unroll
   for (int i = 0; i < V; ++i) v[i] = seq[lp(reg_in_pos<P>(t, i))];
   __syncthreads();
-  reg_fft<P, -1, JX_ROWS_TQ>(v, t, seq, Wt);
+  // same twiddles as the inverse transform: an opaque copy of the table
+  // pointer keeps the compiler from holding them in registers in between
+  const double2 *Wf = Wt;
+  asm volatile("" : "+s"(Wf));
+  reg_fft<P, -1, JX_ROWS_TQ>(v, t, seq, Wf);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];

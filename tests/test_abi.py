@@ -46,3 +46,14 @@ def test_invalid_arguments_map_to_value_error():
                              0.3, ctypes.byref(h))
     assert rc == _lib.JX_EINVAL
     assert b'dim' in lib.jx_last_error()
+
+
+def test_load_imports_torch_first():
+    """The library binds the HIP runtime torch loads (loaded first, it would
+    bind the ROCm install's and the first device call would find no GPU)."""
+    import subprocess
+    import sys
+    code = ('import sys; from janus_amd import _lib; '
+            'assert "torch" not in sys.modules; _lib.load(); '
+            'assert "torch" in sys.modules')
+    subprocess.run([sys.executable, '-c', code], cwd=ROOT, check=True)

@@ -83,3 +83,28 @@ def test_invalid():
         op.apply(np.zeros((4, 5, 2)))
     with pytest.raises(ValueError):
         op.apply(np.zeros((4, 5, 3)), np.zeros((5, 4, 3)))
+
+
+def test_library_loaded_before_torch():
+    """A janus-style caller that never touches torch: the library is loaded
+    (here by the host-side Gamma0 API) before anything initialises the GPU,
+    then a device operator is built and applied."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = '\n'.join([
+        'import numpy as np',
+        'import janus_amd.material.elastic.linear.isotropic as material',
+        'from janus_amd.operators import BlockDiagonalOperator3D',
+        'mat = material.create(1.0, 0.3, 3)',
+        'g = mat.green_operator(); g.set_frequency(np.array([1., 2., 3.]))',
+        'phase = np.zeros((8, 8, 8), np.uint8); phase[2:5] = 1',
+        'op = BlockDiagonalOperator3D.from_phases(phase, np.array([[1., 2.], [3., 4.]]))',
+        'y = np.asarray(op.apply(np.ones((8, 8, 8, 6))))',
+        'assert abs(y[0, 0, 0, 0] - 1.0) < 1e-12 and y[0, 0, 0, 5] == 2.0',
+        'assert abs(y[3, 0, 0, 0] - 3.0) < 1e-12 and y[3, 0, 0, 5] == 4.0',
+        'print("ok", y.shape)'])
+    out = subprocess.run([sys.executable, '-c', code], cwd=root, check=True,
+                         stdout=subprocess.PIPE, text=True, timeout=120)
+    assert 'ok' in out.stdout
