@@ -225,6 +225,21 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
     G.daux4 = sl->daux[3];
     st = sl->st;
   }
+  in = global_ptr(in);
+  out = global_ptr(out);
+  st = global_ptr(st);
+  W = global_ptr(W);
+  if constexpr (MODE == 0) {
+    for (int d = 0; d < kMaxDim; ++d) {
+      G.ax[d].kt = global_ptr(G.ax[d].kt);
+      G.ax[d].km = global_ptr(G.ax[d].km);
+      G.ax[d].kp = global_ptr(G.ax[d].kp);
+      G.ax[d].wm = global_ptr(G.ax[d].wm);
+      G.ax[d].wp = global_ptr(G.ax[d].wp);
+      G.ax[d].sw = global_ptr(G.ax[d].sw);
+      G.ax[d].cw = global_ptr(G.ax[d].cw);
+    }
+  }
   if (solve_done(st)) return;
   constexpr int TPP = reg_tpp<P>();
   constexpr int V = reg_v<P>();
@@ -277,7 +292,8 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
     // opaque copy of the table pointer keeps the compiler from holding them
     // in registers across the Gamma-hat step (which spilled)
     const double2 *Wi = Wt;
-    asm volatile("" : "+s"(Wi));
+    if constexpr (TQ) asm volatile("" : "+s"(Wi));
+    else Wi = global_ptr(Wt);
     reg_fft<P, 1, TQ>(v, t, seq, Wi);
   }
   if (valid) {

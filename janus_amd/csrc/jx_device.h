@@ -15,6 +15,24 @@ namespace jx {
 // power-of-two strides between sequences.
 static inline int lds_stride(int L) { return (L % 2 == 0) ? L + 1 : L; }
 
+// ------------------------------------------------------------ pointers
+// A pointer into device memory as the compiler should see it: global.  The
+// kernels that serve batches take their pointers from the slot table or the
+// kernel arguments, which leaves them generic; every access then becomes a
+// flat instruction, which needs 64-bit VGPR addresses and also counts
+// against the LDS wait counter (LDS steps then wait for outstanding HBM
+// loads and stores).  A plain pair of address-space casts is folded away,
+// so the global pointer passes through an empty asm; the result is also
+// opaque to common-subexpression elimination.  p must be uniform across the
+// block.
+template <class T>
+__device__ __forceinline__ T *global_ptr(T *p) {
+  typedef __attribute__((address_space(1))) T GT;
+  GT *g = (GT *)p;
+  asm volatile("" : "+s"(g));
+  return (T *)g;
+}
+
 // ------------------------------------------------------------ local operator
 // (sph J + dev K) : e in Mandel notation, janus/operators.pyx:315-324 (2D)
 // and :352-365 (3D), tr = (sph - dev) / dim (:299).  Contraction into FMA is

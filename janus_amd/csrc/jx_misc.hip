@@ -18,6 +18,9 @@ __global__ void __launch_bounds__(kFinalizeThreads)
     st = sl.st;
     res_hist = sl.res_hist;
   }
+  partials = global_ptr(partials);
+  st = global_ptr(st);
+  res_hist = global_ptr(res_hist);
   if (solve_done(st)) return;
   __shared__ double red[kFinalizeThreads / 64];
   // eight independent accumulators keep eight loads in flight per thread;

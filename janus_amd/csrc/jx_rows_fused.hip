@@ -51,6 +51,17 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
     loc = sl->loc;
     st = sl->st;
   }
+  spec = global_ptr(spec);
+  y = global_ptr(y);
+  ls.rhs = global_ptr(ls.rhs);
+  ls.partials = global_ptr(ls.partials);
+  loc.phase = global_ptr(loc.phase);
+  loc.table = global_ptr(loc.table);
+  loc.sph = global_ptr(loc.sph);
+  loc.dev = global_ptr(loc.dev);
+  loc.mat = global_ptr(loc.mat);
+  st = global_ptr(st);
+  W = global_ptr(W);
   if (solve_done(st)) return;
   constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
   constexpr int V = reg_v<P>();
@@ -78,12 +89,14 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
         xa = *(const double2 *)sp;
         xb = *(const double2 *)(sp + 2);
       }
+      // Z = A + i B for the two real rows: Z[k] = A[k] + i B[k] and
+      // Z[L - k] = conj(A[k]) + i conj(B[k]) from the half spectra
       double2 *Z = smem + (r * 3 + cp) * LS;
       if (k == 0 || 2 * k == L) {
         Z[lp(k)] = make_double2(xa.x, xb.x);
       } else {
-        Z[lp(k)] = xa;
-        Z[lp(L - k)] = xb;
+        Z[lp(k)] = make_double2(xa.x - xb.y, xa.y + xb.x);
+        Z[lp(L - k)] = make_double2(xa.x + xb.y, xb.x - xa.y);
       }
     }
   } else {
@@ -111,7 +124,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   }
   __syncthreads();
   constexpr int HALF = L / 2 - 1;
-  if (HALF > 0) {
+  if (C != 6 && HALF > 0) {
     for (int w = tid; w < NP * HALF; w += NT) {
       int pp = w / HALF;
       int k = 1 + (w - pp * HALF);
@@ -187,7 +200,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   if (tid == 0) ls.partials[blk] = sum;
 
   // ---- the new iterate into LDS, local operator voxel by voxel
-  __syncthreads();
+  // (block_sum's barrier has retired the inverse transform's LDS reads)
 #pragma unroll
   for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];
   __syncthreads();
@@ -216,7 +229,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   // same twiddles as the inverse transform: an opaque copy of the table
   // pointer keeps the compiler from holding them in registers in between
   const double2 *Wf = Wt;
-  asm volatile("" : "+s"(Wf));
+  if constexpr (JX_ROWS_TQ) asm volatile("" : "+s"(Wf));
+  else Wf = global_ptr(Wt);
   reg_fft<P, -1, JX_ROWS_TQ>(v, t, seq, Wf);
   __syncthreads();
 #pragma unroll

@@ -99,6 +99,15 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     loc = sl->loc;
     st = sl->st;
   }
+  x = global_ptr(x);
+  spec = global_ptr(spec);
+  loc.phase = global_ptr(loc.phase);
+  loc.table = global_ptr(loc.table);
+  loc.sph = global_ptr(loc.sph);
+  loc.dev = global_ptr(loc.dev);
+  loc.mat = global_ptr(loc.mat);
+  st = global_ptr(st);
+  W = global_ptr(W);
   if (solve_done(st)) return;
   constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
   constexpr int V = reg_v<P>();

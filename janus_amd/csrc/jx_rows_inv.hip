@@ -112,6 +112,12 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     ls.partials = sl->partials;
     st = sl->st;
   }
+  spec = global_ptr(spec);
+  y = global_ptr(y);
+  ls.rhs = global_ptr(ls.rhs);
+  ls.partials = global_ptr(ls.partials);
+  st = global_ptr(st);
+  W = global_ptr(W);
   if (solve_done(st)) return;
   constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
   constexpr int V = reg_v<P>();
@@ -127,8 +133,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
   const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
   // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
-  // (0 < k < L/2); DC and Nyquist slots <- (Re Xa, Re Xb), their imaginary
-  // parts dropped as FFTW's and pocketfft's c2r do
+  // (0 < k < L/2; for C = 6 already combined into Xa + i Xb); DC and
+  // Nyquist slots <- (Re Xa, Re Xb), their imaginary parts dropped as
+  // FFTW's and pocketfft's c2r do
   if constexpr (C == 6) {
     for (int w = tid; w < RB * NH * 3; w += NT) {
       int cp = w % 3;
@@ -141,12 +148,14 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
         xa = *(const double2 *)sp;
         xb = *(const double2 *)(sp + 2);
       }
+      // Z = A + i B for the two real rows: Z[k] = A[k] + i B[k] and
+      // Z[L - k] = conj(A[k]) + i conj(B[k]) from the half spectra
       double2 *Z = smem + (r * 3 + cp) * LS;
       if (k == 0 || 2 * k == L) {
         Z[lp(k)] = make_double2(xa.x, xb.x);
       } else {
-        Z[lp(k)] = xa;
-        Z[lp(L - k)] = xb;
+        Z[lp(k)] = make_double2(xa.x - xb.y, xa.y + xb.x);
+        Z[lp(L - k)] = make_double2(xa.x + xb.y, xb.x - xa.y);
       }
     }
   } else
@@ -173,7 +182,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   }
   __syncthreads();
   constexpr int HALF = L / 2 - 1;
-  if (HALF > 0) {
+  if (C != 6 && HALF > 0) {
     for (int w = tid; w < NP * HALF; w += NT) {
       int pp = w / HALF;
       int k = 1 + (w - pp * HALF);
