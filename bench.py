@@ -699,6 +699,44 @@ def workload_adjoint(args):
                         'forward_iterations': fwd.iterations,
                         'adjoint_iterations': adj.iterations,
                         'loss': float(loss.item())}
+    # the same loss through effective_stress: its backward reuses the
+    # forward field (dL/d<sigma> is parallel to E here) -- no adjoint solve
+    from janus_amd.autograd import effective_stress
+
+    def loss_es(rho, tol, max_iter, info=None, method='basic'):
+        g = 1.0 + rho ** 3 * 19.0
+        sph, dev = iso_delta(g, torch.full_like(rho, 0.3), mat0, 3)
+        sigma = effective_stress(sph, dev, E, green, mat0, tol=tol,
+                                 max_iter=max_iter, info=info, method=method)
+        return (0.5 * sigma[5] / E[5] - 3.0) ** 2
+
+    runs_es = {}
+    for method in ('basic', 'cg'):
+        times = []
+        for rep in range(args.adjoint_reps):
+            rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
+            info = {}
+            _sync()
+            t0 = time.perf_counter()
+            loss = loss_es(rho, 1e-10, 5000, info, method)
+            _sync()
+            t1 = time.perf_counter()
+            loss.backward()
+            _sync()
+            t2 = time.perf_counter()
+            times.append((t1 - t0, t2 - t1))
+        t_fwd, t_bwd = min(r[0] for r in times), min(r[1] for r in times)
+        g_es = rho.grad.cpu().numpy()
+        runs_es[method] = {
+            'seconds_forward': round(t_fwd, 4),
+            'seconds_backward': round(t_bwd, 5),
+            'ratio': round(t_bwd / t_fwd, 4),
+            'backward_solves': 0 if info['adjoint'] is None else 1,
+            'grad_vs_ls_solve_max_rel': float(
+                np.abs(g_es - grads[method]).max() /
+                np.abs(grads[method]).max())}
+        _log('effective_stress %s: forward %.3f s, backward %.4f s'
+             % (method, t_fwd, t_bwd))
     grad = grads['basic']
     fd = None
     if args.fd_voxels > 0:
@@ -736,6 +774,7 @@ def workload_adjoint(args):
                           'E=e_xy, L=(C_1212-3)^2, tol 1e-10' % n,
               'cg': runs['cg'],
               'basic_scheme': runs['basic'],
+              'effective_stress': runs_es,
               'grad_cg_vs_basic_max_rel': float(
                   np.abs(grads['cg'] - grads['basic']).max() /
                   np.abs(grads['basic']).max()),

@@ -17,6 +17,10 @@ one Green-operator application, one fixed-point solve and one contraction.
 Both solves use the same method: the basic scheme, or conjugate gradients
 (``method='cg'``), where the right-hand side G g enters the initial
 residual only, so an adjoint iteration costs what a forward one does.
+
+For losses of the macroscopic stress only (effective moduli),
+effective_stress needs no adjoint solve: there g = dC gbar / N and mu is
+the forward fluctuation under the load gbar (see its docstring).
 """
 
 import ctypes
@@ -147,6 +151,103 @@ def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None,
         _LSSolveFunction.fn = _make_function()
     return _LSSolveFunction.fn.apply(sph, dev, E, green, tol, max_iter, info,
                                      method)
+
+
+class _EffectiveStressFunction:
+    fn = None
+
+
+def _make_effective_stress():
+    t = _torch()
+
+    class EffectiveStressFunction(t.autograd.Function):
+        @staticmethod
+        def forward(ctx, sph, dev, E, green, mat0, tol, max_iter, info,
+                    method):
+            sph = sph.detach().contiguous()
+            dev = dev.detach().contiguous()
+            local = _local_from_fields(sph, dev)
+            solver = LSSolver(green, local)
+            En = E.detach().cpu().numpy()
+            r = _solve(solver, En, tol, max_iter, method)
+            if info is not None:
+                info['forward'] = r
+                info['adjoint'] = None
+            tau, _ = local.means(r.eps)
+            from .solver import reference_stiffness
+            sigma = reference_stiffness(mat0).dot(En) + tau
+            ctx.save_for_backward(r.eps)
+            ctx.E = En
+            ctx.c0 = reference_stiffness(mat0)
+            ctx.local = local
+            ctx.solver = solver
+            ctx.args = (tol, max_iter, info, method)
+            return t.from_numpy(sigma).to(E.device)
+
+        @staticmethod
+        def backward(ctx, gbar):
+            (eps,) = ctx.saved_tensors
+            tol, max_iter, info, method = ctx.args
+            gb = gbar.detach().cpu().numpy().astype(np.float64)
+            E = ctx.E
+            ee = float(E.dot(E))
+            alpha = float(gb.dot(E)) / ee if ee > 0 else 0.0
+            nb = float(np.linalg.norm(gb))
+            if nb == 0.0:
+                return None, None, t.zeros_like(gbar), None, None, None, \
+                    None, None, None
+            if ee > 0 and np.linalg.norm(gb - alpha * E) <= 1e-14 * nb:
+                # gbar parallel to E: eps(gbar) = alpha eps(E) (linearity)
+                eps_g, scale = eps, alpha
+            else:
+                r = _solve(ctx.solver, gb, tol, max_iter, method)
+                if info is not None:
+                    info['adjoint'] = r
+                eps_g, scale = r.eps, 1.0
+            N = eps.numel() // eps.shape[-1]
+            # contract() returns (-(J eps) . mu, -(K eps) . mu)
+            dsph, ddev = contract(eps, eps_g * (-scale / N))
+            # dL/dE = C_eff gbar = <sigma(gbar)> (C_eff is symmetric)
+            tau_g, _ = ctx.local.means(eps_g)
+            dE = ctx.c0.dot(gb) + scale * tau_g
+            return dsph, ddev, t.from_numpy(dE).to(gbar.device), None, None, \
+                None, None, None, None
+
+    return EffectiveStressFunction
+
+
+def _solve(solver, E, tol, max_iter, method):
+    if method == 'cg':
+        return solver.solve_cg(E, tol=tol, max_iter=max_iter)
+    return solver.solve(E, tol=tol, max_iter=max_iter)
+
+
+def effective_stress(sph, dev, E, green, mat0, tol=1e-10, max_iter=1000,
+                     info=None, method='basic'):
+    """Differentiable macroscopic stress <sigma> = C0 : E + <dC : eps(E)>.
+
+    The same solve as ls_solve followed by mean_stress, with a backward
+    that needs no adjoint solve.  For a loss L(<sigma>) with gbar =
+    dL/d<sigma>, dL/deps_x = dC_x gbar / N, so the adjoint state of
+    ls_solve's backward solves (I + Gamma0 dC) mu = Gamma0 dC gbar / N:
+    the forward equation of the fluctuation eps(gbar) - gbar under the
+    uniform load gbar, times -1 / N.  Hence mu = -(eps(gbar) - gbar) / N,
+    and with the explicit term of <dC : eps>, dL/dsph_x =
+    (J eps_x) . eps(gbar)_x / N and dL/ddev_x = (K eps_x) . eps(gbar)_x / N.
+    When gbar is parallel to E (a loss of the effective modulus that E
+    probes), eps(gbar) is a multiple of the forward field and the backward
+    is one contraction; otherwise it is one forward solve under gbar (no
+    field right-hand side).
+
+    Args: as ls_solve, plus mat0 (the isotropic reference material of
+    green).  info receives 'forward' and 'adjoint' (None when no solve was
+    needed).  Returns <sigma>, a (sym,) tensor."""
+    if method not in ('basic', 'cg'):
+        raise ValueError("method must be 'basic' or 'cg'")
+    if _EffectiveStressFunction.fn is None:
+        _EffectiveStressFunction.fn = _make_effective_stress()
+    return _EffectiveStressFunction.fn.apply(sph, dev, E, green, mat0, tol,
+                                             max_iter, info, method)
 
 
 def iso_delta(g, nu, mat0, dim):

@@ -483,6 +483,28 @@ def adjoint_gradient(sph, dev, eps, grad_eps, kind, g0, nu0, tol, max_iter,
     return dsph, ddev, mu, res
 
 
+def effective_stress_gradient(eps, eps_g):
+    """Gradient of gbar . <sigma(E)> w.r.t. the per-voxel (sph, dev) of dC,
+    from the forward fields eps = eps(E) and eps_g = eps(gbar) (the solve
+    under the load gbar), with <sigma> = C0 E + <dC : eps>.
+
+    With A = I + G dC (G = Gamma0 and dC symmetric), eps(a) = A^-1 a for a
+    uniform a, and the adjoint state of gbar . <sigma> is
+    mu = -(eps_g - gbar) / N: (I + G dC) mu = G dC gbar / N is the equation
+    of eps_g - gbar scaled by -1 / N.  adjoint_gradient's contraction plus
+    the explicit dependence of <dC : eps> then give
+    d/dsph_x = (J eps_x) . eps_g_x / N and d/ddev_x = (K eps_x) . eps_g_x / N.
+    Returns (dsph, ddev).
+    """
+    sym = eps.shape[-1]
+    dim = 2 if sym == 3 else 3
+    N = eps.size // sym
+    tr_e = eps[..., :dim].sum(axis=-1)
+    tr_g = eps_g[..., :dim].sum(axis=-1)
+    j_term = tr_e * tr_g / dim
+    return j_term / N, ((eps * eps_g).sum(axis=-1) - j_term) / N
+
+
 def _field_rhs_scheme(sph, dev, rhs, kind, g0, nu0, tol, max_iter, h):
     """mu <- rhs - Gamma0[dC : mu], residual ||mu_{k+1}-mu_k||/sqrt(N)/||rhs||_rms."""
     shape = np.shape(sph)

@@ -320,6 +320,54 @@ def test_adjoint_gradient_finite_differences():
         assert abs(fd - ddev[i, j]) < 1e-7 * max(1., abs(fd))
 
 
+@pytest.mark.parametrize('kind', ['filtered', 'truncated'])
+def test_effective_stress_gradient(kind):
+    """L = gbar . <sigma>: the forward-field gradient (no adjoint solve)
+    equals the adjoint-solve gradient plus the explicit term of <dC : eps>,
+    and central differences of L."""
+    rng = np.random.default_rng(11)
+    n, dim = 12, 2
+    g0, nu0 = 5.0, 0.3
+    sph = rng.uniform(-3., 3., (n, n))
+    dev = rng.uniform(-3., 3., (n, n))
+    E = np.array([0.1, -0.2, 1.0])
+    gbar = np.array([0.7, 0.3, -1.1])
+    N = n * n
+
+    def solve(s, d, load):
+        return jo.basic_scheme(s, d, load, kind, g0, nu0, tol=1e-14,
+                               max_iter=3000)[0]
+
+    def loss(s, d):
+        eps = solve(s, d, E)
+        return float(gbar @ jo.iso4_apply(s, d, eps).reshape(-1, 3).mean(0))
+
+    eps = solve(sph, dev, E)
+    eps_g = solve(sph, dev, gbar)
+    dsph, ddev = jo.effective_stress_gradient(eps, eps_g)
+    # the general adjoint: g = dL/deps = dC gbar / N, plus the explicit term
+    g = jo.iso4_apply(sph, dev, np.broadcast_to(gbar, eps.shape).copy()) / N
+    asph, adev, _, _ = jo.adjoint_gradient(sph, dev, eps, g, kind, g0, nu0,
+                                           tol=1e-14, max_iter=3000)
+    tr_e = eps[..., :dim].sum(axis=-1)
+    j_term = tr_e * gbar[:dim].sum() / dim
+    asph += j_term / N
+    adev += ((eps * gbar).sum(axis=-1) - j_term) / N
+    scale = max(np.abs(dsph).max(), np.abs(ddev).max())
+    np.testing.assert_allclose(dsph, asph, rtol=0, atol=1e-11 * scale)
+    np.testing.assert_allclose(ddev, adev, rtol=0, atol=1e-11 * scale)
+    h = 1e-6
+    for (i, j) in [(0, 0), (5, 9), (11, 2)]:
+        sp, sm = sph.copy(), sph.copy()
+        sp[i, j] += h
+        sm[i, j] -= h
+        fd = (loss(sp, dev) - loss(sm, dev)) / (2 * h)
+        assert abs(fd - dsph[i, j]) < 1e-7 * max(1e-3, abs(fd))
+    # a plausible mistake (eps in place of eps_g) fails
+    wrong = jo.effective_stress_gradient(eps, eps)[0]
+    assert np.abs(wrong - dsph).max() > 1e-3 * scale
+
+
 # ---------------------------------------------------------------- CG
 
 def _cg_setup(n, g0, kind='truncated'):
