@@ -900,6 +900,33 @@ def workload_large(args):
     _log('forward %.2f s (%d it), backward %.2f s (%d it), grad %s'
          % (t1 - t0, info['forward'].iterations, t2 - t1,
             info['adjoint'].iterations, grad))
+    # the same loss through effective_stress (dL/d<sigma> parallel to E:
+    # the backward is one contraction, no adjoint solve)
+    from janus_amd.autograd import effective_stress
+
+    def loss_es(mu_t, tol, max_iter, info=None):
+        g = masks[0] * mu_t[0]
+        for p in range(1, len(mus)):
+            g = g + masks[p] * mu_t[p]
+        sph, dev = iso_delta(g, torch.full_like(g, 0.3), mat0, 3)
+        return effective_stress(sph, dev, E, green, mat0, tol=tol,
+                                max_iter=max_iter, info=info,
+                                method=args.method)[5]
+
+    mu_es = torch.tensor(mus, dtype=torch.float64, device='cuda',
+                         requires_grad=True)
+    info_es = {}
+    _sync()
+    t3 = time.perf_counter()
+    L_es = loss_es(mu_es, 1e-10, args.max_iter, info_es)
+    _sync()
+    t4 = time.perf_counter()
+    L_es.backward()
+    _sync()
+    t5 = time.perf_counter()
+    grad_es = mu_es.grad.cpu().numpy()
+    _log('effective_stress: forward %.2f s, backward %.4f s, grad %s'
+         % (t4 - t3, t5 - t4, grad_es))
     fd = None
     if args.fd_voxels > 0:
         h = 1e-4
@@ -931,7 +958,14 @@ def workload_large(args):
                   'seconds_backward': round(t2 - t1, 3),
                   'forward_iterations': info['forward'].iterations,
                   'adjoint_iterations': info['adjoint'].iterations,
-                  'fd_check': fd}})
+                  'fd_check': fd},
+              'effective_stress': {
+                  'seconds_forward': round(t4 - t3, 3),
+                  'seconds_backward': round(t5 - t4, 4),
+                  'ratio': round((t5 - t4) / (t4 - t3), 4),
+                  'dL_dmu_phase': grad_es.tolist(),
+                  'grad_vs_adjoint_max_rel': float(
+                      np.abs(grad_es - grad).max() / np.abs(grad).max())}})
 
 
 WORKLOADS = {'plain2d': workload_plain2d, 'stiffness': workload_stiffness,
