@@ -228,9 +228,29 @@ inline int64_t batch_count(const Batch *b) { return b ? b->n : 1; }
 // Fixed-point update arguments (the macroscopic strain is read from the
 // solve state).
 struct LsArgs {
-  const double *rhs;
-  double *partials;
+  const double *rhs = nullptr;
+  double *partials = nullptr;
+  // conjugate gradients (pass E): the direction p and the C0 weights
+  const double *cgp = nullptr;
+  double wd = 0., wt = 0.;
 };
+
+// a : C0 : b of one voxel, C0 = dim k0 J + 2 g0 K in Mandel notation:
+// wd (a . b) + wt tr(a) tr(b) with wd = 2 g0, wt = k0 - 2 g0 / dim.
+template <int C>
+__device__ __forceinline__ double c0_dot(const double *a, const double *b,
+                                         double wd, double wt) {
+  constexpr int D = C == 3 ? 2 : 3;
+  double dot = 0., ta = 0., tb = 0.;
+#pragma unroll
+  for (int c = 0; c < C; ++c) dot += a[c] * b[c];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    ta += a[c];
+    tb += b[c];
+  }
+  return wd * dot + wt * ta * tb;
+}
 
 // Twiddle source of the last-axis kernels: quarter table in LDS (1) or the
 // global table (0, fewer registers; measured as fast or faster).

@@ -31,15 +31,6 @@ struct jx_local {
   bool symmetric = true;       // every block is a symmetric matrix
 };
 
-namespace jx {
-// Scalars of a conjugate-gradient solve, kept on the device.
-struct CgScalars {
-  double rr;     // <r, r> in the C0 inner product
-  double alpha;  // step of the current iteration
-  double beta;   // direction update of the current iteration
-};
-}  // namespace jx
-
 struct jx_solver {
   jx_green *green = nullptr;
   jx_local *local = nullptr;

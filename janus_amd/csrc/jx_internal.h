@@ -159,6 +159,22 @@ int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 double *out, const GreenDev *green, const SolveState *st,
                 hipStream_t s, const Batch *b = nullptr);
+// Scalars of a conjugate-gradient solve, kept on the device.
+struct CgScalars {
+  double rr;     // <r, r> in the C0 inner product
+  double alpha;  // step of the current iteration
+  double beta;   // direction update of the current iteration
+};
+
+// Fields of a conjugate-gradient solve that pass A updates before it
+// transforms: p <- r + beta p and x <- x + alpha p (the previous p).
+struct CgDev {
+  const double *r = nullptr;
+  double *p = nullptr;
+  double *x = nullptr;
+  const CgScalars *cs = nullptr;
+};
+
 // Inverse transform of the last axis (scaled by 1/N).  When enabled, the
 // fixed-point update eps <- E + rhs - y (E from the solve state) is applied
 // in place on y and per-block partial sums of (eps_new - eps_old)^2 go to
@@ -175,6 +191,18 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
 // Residual partial sums written by the update pass: the most blocks any
 // plan variant of the last axis uses (the size of a partials buffer).
 int rows_inverse_nblocks(const Grid &g, int ncomp);
+// Conjugate-gradient variants of passes A and E for the register lengths
+// (C = 3 or 6): pass A updates p and x (CgDev) and transforms dC : p; pass
+// E writes q = p + c2r(spec) / N and the block sums of p : C0 : q
+// (wd p . q + wt tr p tr q per voxel) into partials[block].
+bool rows_cg_fusable(const Grid &g, int ncomp);
+int launch_rows_forward_cg(const Grid &g, int ncomp, const CgDev &cg,
+                           double *spec, const LocalDev &loc,
+                           const SolveState *st, hipStream_t s);
+int launch_rows_inverse_cg(const Grid &g, int ncomp, const double *spec,
+                           double *q, const double *p, double *partials,
+                           double wd, double wt, const SolveState *st,
+                           int *nblocks_out, hipStream_t s);
 // Fused pass E of iteration k + pass A of iteration k + 1 (register path).
 bool rows_fusable(const Grid &g, int ncomp);
 int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,

@@ -79,17 +79,67 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 
+// Conjugate gradients: p <- r + beta p and x <- x + alpha p (the previous
+// p) at one voxel, with the operations of the separate vector kernels; the
+// new p is returned in e.
+template <int C>
+__device__ __forceinline__ void cg_update_voxel(const CgDev &cg, int64_t vox,
+                                                double alpha, double beta,
+                                                double *e) {
+#pragma clang fp contract(off)
+  double rv[C], pv[C], xv[C];
+  if constexpr (C == 6) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 a = ((const double2 *)(cg.r + vox * 6))[i];
+      double2 b = ((const double2 *)(cg.p + vox * 6))[i];
+      double2 c = ((const double2 *)(cg.x + vox * 6))[i];
+      rv[2 * i] = a.x; rv[2 * i + 1] = a.y;
+      pv[2 * i] = b.x; pv[2 * i + 1] = b.y;
+      xv[2 * i] = c.x; xv[2 * i + 1] = c.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      rv[c] = cg.r[vox * C + c];
+      pv[c] = cg.p[vox * C + c];
+      xv[c] = cg.x[vox * C + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    double ap = alpha * pv[c];
+    double bp = beta * pv[c];
+    xv[c] = xv[c] + ap;
+    e[c] = rv[c] + bp;
+  }
+  if constexpr (C == 6) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ((double2 *)(cg.x + vox * 6))[i] = make_double2(xv[2 * i], xv[2 * i + 1]);
+      ((double2 *)(cg.p + vox * 6))[i] = make_double2(e[2 * i], e[2 * i + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      cg.x[vox * C + c] = xv[c];
+      cg.p[vox * C + c] = e[c];
+    }
+  }
+}
+
 // Register-resident last-axis transform for the register FFT lengths.  A
 // block handles RB rows, i.e. 2 NP real sequences paired into NP complex
 // ones (sequence q = r C + c; pair p = q / 2, part q % 2), TPP = L / V
 // threads per pair.  Batched: the entry's iterate, spectrum, local operator
 // and state come from its slot.
-// LOCAL: 0 no local operator, else 1 + its LocalFamily.
-template <class P, int C, int NP, int LOCAL>
+// LOCAL: 0 no local operator, else 1 + its LocalFamily.  CGP: the input is
+// the conjugate-gradient direction, updated first (cg_update_voxel).
+template <class P, int C, int NP, int LOCAL, bool CGP>
 __global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
                        int64_t nrows, const double2 *W, LocalDev loc,
-                       const SolveState *st, BatchRef br) {
+                       const SolveState *st, BatchRef br, CgDev cg) {
   constexpr int L = P::L;
   const Slot *sl;
   const int64_t blk = batch_block(br, sl);
@@ -108,7 +158,17 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   loc.mat = global_ptr(loc.mat);
   st = global_ptr(st);
   W = global_ptr(W);
+  double alpha = 0., beta = 0.;
+  if constexpr (CGP) {
+    cg.r = global_ptr(cg.r);
+    cg.p = global_ptr(cg.p);
+    cg.x = global_ptr(cg.x);
+  }
   if (solve_done(st)) return;
+  if constexpr (CGP) {
+    alpha = cg.cs->alpha;
+    beta = cg.cs->beta;
+  }
   constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
   constexpr int V = reg_v<P>();
   constexpr int NT = NP * TPP;
@@ -127,7 +187,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     if (r < rb) {
       int64_t vox = (row0 + r) * L + j;
       double e[C];
-      if constexpr (C == 6) {
+      if constexpr (CGP) {
+        cg_update_voxel<C>(cg, vox, alpha, beta, e);
+      } else if constexpr (C == 6) {
         const double2 *xv = (const double2 *)(x + vox * 6);
         double2 a = xv[0], b = xv[1], cc = xv[2];
         e[0] = a.x; e[1] = a.y; e[2] = b.x; e[3] = b.y; e[4] = cc.x; e[5] = cc.y;
@@ -212,7 +274,8 @@ template <class P, int C>
 static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
                                       double *spec, const LocalDev &loc,
                                       const SolveState *st, const double2 *W,
-                                      hipStream_t s, const Batch *b) {
+                                      hipStream_t s, const Batch *b,
+                                      const CgDev *cg) {
   constexpr int L = P::L;
   constexpr int NP = reg_np<P>();
   constexpr int RB = 2 * NP / C;
@@ -222,24 +285,34 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
   const int64_t nbe = (g.rows + RB - 1) / RB;
   const BatchRef br = batch_ref(b, nbe);
   dim3 grid((unsigned)(nbe * batch_count(b))), block(NP * reg_tpp<P>());
+  const CgDev cgd = cg ? *cg : CgDev();
   auto go = [&](auto kernel) {
     set_lds_attr(kernel, lds);
     hipLaunchKernelGGL(kernel, grid, block, lds, s, x, spec, g.rows, W, loc, st,
-                       br);
+                       br, cgd);
   };
   // a batch always applies the local operator of its slots (all of one
   // family: the first slot's, which the caller passes as loc)
-  if (b || loc.mode != LOCAL_NONE) {
+  if (cg) {
     if constexpr (C == 3 || C == 6) {
       if (local_family(loc) == LF_MATRIX)
-        go(k_rows_forward_reg<P, C, NP, 1 + LF_MATRIX>);
+        go(k_rows_forward_reg<P, C, NP, 1 + LF_MATRIX, true>);
       else
-        go(k_rows_forward_reg<P, C, NP, 1 + LF_ISO>);
+        go(k_rows_forward_reg<P, C, NP, 1 + LF_ISO, true>);
+    } else {
+      return fail(JX_EINVAL, "conjugate gradients need sym components");
+    }
+  } else if (b || loc.mode != LOCAL_NONE) {
+    if constexpr (C == 3 || C == 6) {
+      if (local_family(loc) == LF_MATRIX)
+        go(k_rows_forward_reg<P, C, NP, 1 + LF_MATRIX, false>);
+      else
+        go(k_rows_forward_reg<P, C, NP, 1 + LF_ISO, false>);
     } else {
       return fail(JX_EINVAL, "local operator needs sym components");
     }
   } else {
-    go(k_rows_forward_reg<P, C, NP, 0>);
+    go(k_rows_forward_reg<P, C, NP, 0, false>);
   }
   JX_HIP(hipGetLastError());
   return JX_OK;
@@ -247,9 +320,26 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
 
 static int rows_forward_reg(const Grid &g, int C, const double *x, double *spec,
                             const LocalDev &loc, const SolveState *st,
-                            const double2 *W, hipStream_t s, const Batch *b) {
+                            const double2 *W, hipStream_t s, const Batch *b,
+                            const CgDev *cg = nullptr) {
   JX_WITH_PLAN_C((int)g.L, g.plan[PASS_ROWS_FWD], launch_rows_forward_reg_lc,
-                 g, x, spec, loc, st, W, s, b);
+                 g, x, spec, loc, st, W, s, b, cg);
+}
+
+bool rows_cg_fusable(const Grid &g, int ncomp) {
+  return (ncomp == 3 || ncomp == 6) && reg_fft_length((int)g.L);
+}
+
+int launch_rows_forward_cg(const Grid &g, int ncomp, const CgDev &cg,
+                           double *spec, const LocalDev &loc,
+                           const SolveState *st, hipStream_t s) {
+  if (!rows_cg_fusable(g, ncomp))
+    return fail(JX_EUNSUPPORTED, "rows_forward_cg needs a register FFT length");
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  return rows_forward_reg(g, ncomp, nullptr, spec, loc, st, P.W, s, nullptr,
+                          &cg);
 }
 
 int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,

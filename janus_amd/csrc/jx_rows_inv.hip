@@ -93,10 +93,42 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Conjugate gradients: q = p + y at one voxel (y = c2r(spec) / N, the
+// operations of the separate vector kernels); returns p : C0 : q.
+template <int C>
+__device__ __forceinline__ double cg_qp_voxel(const LsArgs &ls, double *q,
+                                              int64_t vox, const double *yv) {
+#pragma clang fp contract(off)
+  double pv[C], qv[C];
+  if constexpr (C == 6) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 a = ((const double2 *)(ls.cgp + vox * 6))[i];
+      pv[2 * i] = a.x;
+      pv[2 * i + 1] = a.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) pv[c] = ls.cgp[vox * C + c];
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) qv[c] = pv[c] + yv[c];
+  if constexpr (C == 6) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      ((double2 *)(q + vox * 6))[i] = make_double2(qv[2 * i], qv[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) q[vox * C + c] = qv[c];
+  }
+  return c0_dot<C>(pv, qv, ls.wd, ls.wt);
+}
+
 // Register-FFT variant for the register lengths.  MODE 0: y = c2r(spec);
 // 1: y <- E - c2r(spec) with residual partials; 2: y <- (E + rhs) -
-// c2r(spec) with residual partials.  Batched: the entry's spectrum,
-// iterate, rhs, partials and state come from its slot.
+// c2r(spec) with residual partials; 3 (conjugate gradients): y <- p +
+// c2r(spec) with the block sums of p : C0 : y in partials.  Batched: the
+// entry's spectrum, iterate, rhs, partials and state come from its slot.
 template <class P, int C, int NP, int MODE>
 __global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_inverse_reg(const double *__restrict__ spec, double *y,
@@ -118,12 +150,13 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   ls.partials = global_ptr(ls.partials);
   st = global_ptr(st);
   W = global_ptr(W);
+  if constexpr (MODE == 3) ls.cgp = global_ptr(ls.cgp);
   if (solve_done(st)) return;
   constexpr int TPP = reg_tpp<P>(), LS = reg_lds_stride(L), NH = L / 2 + 1;
   constexpr int V = reg_v<P>();
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
-  constexpr bool LS_UPDATE = MODE > 0;
+  constexpr bool LS_UPDATE = MODE == 1 || MODE == 2;
   extern __shared__ double2 smem[];
   __shared__ double red[4];
   const int tid = threadIdx.x;
@@ -200,7 +233,29 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   __syncthreads();
   reg_fft<P, 1, JX_ROWS_TQ>(v, t, seq, Wt);
   double acc = 0.;
-  if constexpr (C == 6) {
+  if constexpr (MODE == 3) {
+    // all C components of a voxel meet in LDS (the C0 product has a trace
+    // term)
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+      seq[lp(reg_out_pos<P>(t, i))] =
+          make_double2(scale * v[i].x, scale * v[i].y);
+    __syncthreads();
+    for (int w = tid; w < rb * L; w += NT) {
+      int r = w / L, j = w - r * L;
+      double yv[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        int q = r * C + c;
+        yv[c] = ((const double *)&smem[(q >> 1) * LS + lp(j)])[q & 1];
+      }
+      acc += cg_qp_voxel<C>(ls, y, (row0 + r) * L + j, yv);
+    }
+    double sum = block_sum(acc, red);
+    if (tid == 0) ls.partials[blk] = sum;
+    return;
+  } else if constexpr (C == 6) {
     // pair p = components (c0, c0 + 1) of row r: 16-byte accesses
     const int r = p / 3, c0 = 2 * (p - r * 3);
     if (r < rb) {
@@ -309,6 +364,51 @@ static int rows_inverse_reg(const Grid &g, int C, const double *spec, double *y,
                             const double2 *W, hipStream_t s, const Batch *b) {
   JX_WITH_PLAN_C((int)g.L, g.plan[PASS_ROWS_INV], launch_rows_inverse_reg_lc,
                  g, spec, y, la, upd, st, W, s, b);
+}
+
+template <class P, int C>
+static int launch_rows_inverse_cg_lc(const Grid &g, const double *spec,
+                                     double *q, const LsArgs &la,
+                                     const SolveState *st, const double2 *W,
+                                     hipStream_t s, int *nblocks_out) {
+  if constexpr (C == 3 || C == 6) {
+    constexpr int L = P::L;
+    constexpr int NP = reg_np<P>();
+    constexpr int RB = 2 * NP / C;
+    size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
+    int rc = check_lds(lds, "rows_inverse_cg");
+    if (rc) return rc;
+    const int64_t nb = (g.rows + RB - 1) / RB;
+    if (nblocks_out) *nblocks_out = (int)nb;
+    double scale = 1.0 / (double)g.N;
+    set_lds_attr(k_rows_inverse_reg<P, C, NP, 3>, lds);
+    hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 3>), dim3((unsigned)nb),
+                       dim3(NP * reg_tpp<P>()), lds, s, spec, q, g.rows, scale,
+                       W, la, st, batch_ref(nullptr, nb));
+    JX_HIP(hipGetLastError());
+    return JX_OK;
+  } else {
+    return fail(JX_EINVAL, "conjugate gradients need sym components");
+  }
+}
+
+int launch_rows_inverse_cg(const Grid &g, int ncomp, const double *spec,
+                           double *q, const double *p, double *partials,
+                           double wd, double wt, const SolveState *st,
+                           int *nblocks_out, hipStream_t s) {
+  if (!rows_cg_fusable(g, ncomp))
+    return fail(JX_EUNSUPPORTED, "rows_inverse_cg needs a register FFT length");
+  FftLen P;
+  int rc = get_fft_len((int)g.L, &P);
+  if (rc) return rc;
+  LsArgs la;
+  la.partials = partials;
+  la.cgp = p;
+  la.wd = wd;
+  la.wt = wt;
+  const int C = ncomp;
+  JX_WITH_PLAN_C((int)g.L, g.plan[PASS_ROWS_INV], launch_rows_inverse_cg_lc,
+                 g, spec, q, la, st, P.W, s, nblocks_out);
 }
 
 int rows_inverse_nblocks(const Grid &g, int ncomp) {
