@@ -348,13 +348,23 @@ static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
   }
 }
 
-template <class P>
-static int launch_axis_reg_p(const Grid &g, int ncomp, int axis, int dir,
-                             const double *in, double *out,
-                             const GreenDev *green, const SolveState *st,
-                             const FftLen &FL, hipStream_t s, const Batch *b) {
+// Passes B and D of the 16-thread-per-sequence plans (256 points, 16
+// values) take four points per block (384 threads, 105 KB of LDS, one block
+// per CU): a tile then spans 384 bytes, three whole 128-byte lines, and the
+// passes measured 0.313 against 0.324 ms at 256^3 with two points per block
+// (and 0.349 with one, 0.328 with three).  The Gamma-hat pass keeps two:
+// it needs the occupancy more (0.468 against 0.523 ms with four).
+#ifndef JX_AXIS_BD_POINTS4
+#define JX_AXIS_BD_POINTS4 1
+#endif
+
+template <class P, int NSEQ>
+static int launch_axis_reg_pn(const Grid &g, int ncomp, int axis, int dir,
+                              const double *in, double *out,
+                              const GreenDev *green, const SolveState *st,
+                              const FftLen &FL, hipStream_t s,
+                              const Batch *b) {
   constexpr int L = P::L;
-  constexpr int NSEQ = reg_nseq<P>();
   static_assert(NSEQ % 6 == 0, "sequences per block must hold whole points");
   AxisArgs a;
   a.L = L;
@@ -386,6 +396,20 @@ static int launch_axis_reg_p(const Grid &g, int ncomp, int axis, int dir,
   }
   JX_HIP(hipGetLastError());
   return JX_OK;
+}
+
+template <class P>
+static int launch_axis_reg_p(const Grid &g, int ncomp, int axis, int dir,
+                             const double *in, double *out,
+                             const GreenDev *green, const SolveState *st,
+                             const FftLen &FL, hipStream_t s, const Batch *b) {
+  if constexpr (reg_tpp<P>() == 16 && reg_nseq<P>() == 12) {
+    if (JX_AXIS_BD_POINTS4 && dir != 0)
+      return launch_axis_reg_pn<P, 24>(g, ncomp, axis, dir, in, out, green,
+                                       st, FL, s, b);
+  }
+  return launch_axis_reg_pn<P, reg_nseq<P>()>(g, ncomp, axis, dir, in, out,
+                                              green, st, FL, s, b);
 }
 
 // The Gamma-hat pass and the plain axis passes choose their plan variants
