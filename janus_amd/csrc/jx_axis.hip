@@ -353,7 +353,9 @@ static void launch_axis_reg_lc(int dir, dim3 grid, size_t lds, hipStream_t s,
 // per CU): a tile then spans 384 bytes, three whole 128-byte lines, and the
 // passes measured 0.313 against 0.324 ms at 256^3 with two points per block
 // (and 0.349 with one, 0.328 with three).  The Gamma-hat pass keeps two:
-// it needs the occupancy more (0.468 against 0.523 ms with four).
+// it needs the occupancy more (0.468 against 0.523 ms with four).  The
+// 512-point plans keep one point per block: two (one 384-thread block per
+// CU) measured slower at 512^3 for every pass.
 #ifndef JX_AXIS_BD_POINTS4
 #define JX_AXIS_BD_POINTS4 1
 #endif
