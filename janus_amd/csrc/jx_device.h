@@ -252,6 +252,52 @@ __device__ __forceinline__ double c0_dot(const double *a, const double *b,
   return wd * dot + wt * ta * tb;
 }
 
+// Last-axis c2r passes, 6 components: the half spectra of RB rows into the
+// NP = 3 RB pair sequences of LDS (stride LS), pair (A, B) as Z = A + i B:
+// Z[k] = A[k] + i B[k] and Z[L - k] = conj(A[k]) + i conj(B[k]); the DC and
+// Nyquist slots hold (Re A, Re B), their imaginary parts dropped as FFTW's
+// and pocketfft's c2r do.  Rows past rb are zero.  Every global load of the
+// block is issued before the first LDS store (a loop with a run-time trip
+// count would wait for each round's loads in turn).
+template <int L, int RB, int NT, int LS>
+__device__ __forceinline__ void load_half_spectra6(const double *spec,
+                                                   int64_t row0, int rb,
+                                                   int tid, double2 *smem) {
+  constexpr int NH = L / 2 + 1;
+  constexpr int ITEMS = RB * NH * 3;
+  constexpr int K = (ITEMS + NT - 1) / NT;
+  double2 xa[K], xb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int w = tid + k * NT;
+    xa[k] = make_double2(0., 0.);
+    xb[k] = make_double2(0., 0.);
+    const int rest = w / 3, r = rest / NH;
+    if (w < ITEMS && r < rb) {
+      const int cp = w - rest * 3, kk = rest - r * NH;
+      const double *sp = spec + ((row0 + r) * NH + kk) * 12 + 4 * cp;
+      xa[k] = *(const double2 *)sp;
+      xb[k] = *(const double2 *)(sp + 2);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int w = tid + k * NT;
+    if (w < ITEMS) {
+      const int rest = w / 3, r = rest / NH;
+      const int cp = w - rest * 3, kk = rest - r * NH;
+      double2 *Z = smem + (r * 3 + cp) * LS;
+      const double2 a = xa[k], b = xb[k];
+      if (kk == 0 || 2 * kk == L) {
+        Z[lp(kk)] = make_double2(a.x, b.x);
+      } else {
+        Z[lp(kk)] = make_double2(a.x - b.y, a.y + b.x);
+        Z[lp(L - kk)] = make_double2(a.x + b.y, b.x - a.y);
+      }
+    }
+  }
+}
+
 // Twiddle source of the last-axis kernels: quarter table in LDS (1) or the
 // global table (0, fewer registers; measured as fast or faster).
 #ifndef JX_ROWS_TQ

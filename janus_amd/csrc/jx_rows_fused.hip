@@ -78,27 +78,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
 
   // ---- inverse: half spectra into the pair slots (see k_rows_inverse_reg)
   if constexpr (C == 6) {
-    for (int w = tid; w < RB * NH * 3; w += NT) {
-      int cp = w % 3;
-      int rest = w / 3;
-      int k = rest % NH;
-      int r = rest / NH;
-      double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
-      if (r < rb) {
-        const double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
-        xa = *(const double2 *)sp;
-        xb = *(const double2 *)(sp + 2);
-      }
-      // Z = A + i B for the two real rows: Z[k] = A[k] + i B[k] and
-      // Z[L - k] = conj(A[k]) + i conj(B[k]) from the half spectra
-      double2 *Z = smem + (r * 3 + cp) * LS;
-      if (k == 0 || 2 * k == L) {
-        Z[lp(k)] = make_double2(xa.x, xb.x);
-      } else {
-        Z[lp(k)] = make_double2(xa.x - xb.y, xa.y + xb.x);
-        Z[lp(L - k)] = make_double2(xa.x + xb.y, xb.x - xa.y);
-      }
-    }
+    load_half_spectra6<L, RB, NT, LS>(spec, row0, rb, tid, smem);
   } else {
     for (int w = tid; w < RB * NH * C; w += NT) {
       int c = w % C;
@@ -204,7 +184,13 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
 #pragma unroll
   for (int i = 0; i < V; ++i) seq[lp(reg_out_pos<P>(t, i))] = v[i];
   __syncthreads();
-  for (int w = tid; w < rb * L; w += NT) {
+  // a compile-time trip count lets the phase loads of all rounds be issued
+  // together
+  constexpr int KV = (RB * L + NT - 1) / NT;
+#pragma unroll
+  for (int kv = 0; kv < KV; ++kv) {
+    const int w = tid + kv * NT;
+    if (w >= rb * L) continue;
     int r = w / L, j = w - r * L;
     int64_t vox = (row0 + r) * L + j;
     double e[C], tt[C];

@@ -180,8 +180,13 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   const int rb = (int)min((int64_t)RB, nrows - row0);
   __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
   const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
-  // stage the rows (local operator applied) into the pair sequences
-  for (int w = tid; w < RB * L; w += NT) {
+  // stage the rows (local operator applied) into the pair sequences; a
+  // compile-time trip count lets the loads of all rounds be issued together
+  constexpr int KV = (RB * L + NT - 1) / NT;
+#pragma unroll
+  for (int kv = 0; kv < KV; ++kv) {
+    const int w = tid + kv * NT;
+    if (w >= RB * L) continue;
     int r = w / L, j = w - r * L;
     double t[C];
     if (r < rb) {

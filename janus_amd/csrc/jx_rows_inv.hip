@@ -170,27 +170,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   // Nyquist slots <- (Re Xa, Re Xb), their imaginary parts dropped as
   // FFTW's and pocketfft's c2r do
   if constexpr (C == 6) {
-    for (int w = tid; w < RB * NH * 3; w += NT) {
-      int cp = w % 3;
-      int rest = w / 3;
-      int k = rest % NH;
-      int r = rest / NH;
-      double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
-      if (r < rb) {
-        const double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
-        xa = *(const double2 *)sp;
-        xb = *(const double2 *)(sp + 2);
-      }
-      // Z = A + i B for the two real rows: Z[k] = A[k] + i B[k] and
-      // Z[L - k] = conj(A[k]) + i conj(B[k]) from the half spectra
-      double2 *Z = smem + (r * 3 + cp) * LS;
-      if (k == 0 || 2 * k == L) {
-        Z[lp(k)] = make_double2(xa.x, xb.x);
-      } else {
-        Z[lp(k)] = make_double2(xa.x - xb.y, xa.y + xb.x);
-        Z[lp(L - k)] = make_double2(xa.x + xb.y, xb.x - xa.y);
-      }
-    }
+    load_half_spectra6<L, RB, NT, LS>(spec, row0, rb, tid, smem);
   } else
   for (int w = tid; w < RB * NH * C; w += NT) {
     int c = w % C;
