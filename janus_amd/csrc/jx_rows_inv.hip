@@ -222,7 +222,11 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
       seq[lp(reg_out_pos<P>(t, i))] =
           make_double2(scale * v[i].x, scale * v[i].y);
     __syncthreads();
-    for (int w = tid; w < rb * L; w += NT) {
+    constexpr int KV = (RB * L + NT - 1) / NT;
+#pragma unroll
+    for (int kv = 0; kv < KV; ++kv) {
+      const int w = tid + kv * NT;
+      if (w >= rb * L) continue;
       int r = w / L, j = w - r * L;
       double yv[C];
 #pragma unroll
