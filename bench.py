@@ -204,10 +204,10 @@ def kernel_pass(name):
     return None
 
 
-def stats_summary(stats_csv, cmd):
+def stats_summary(stats_csv, cmd, title='256^3 filtered LS iteration'):
     """Markdown table of a rocprofv3 --stats kernel summary (jx kernels)."""
     import csv
-    out = ['# Kernel durations, 256^3 filtered LS iteration', '',
+    out = ['# Kernel durations, ' + title, '',
            'Command:', '', '    ' + cmd, '',
            'rocprofv3 kernel statistics (jx kernels; the autotuner also '
            'launches the variants it does not keep):', '',
@@ -217,7 +217,8 @@ def stats_summary(stats_csv, cmd):
         name = r['Name']
         if 'jx::' not in name:
             continue
-        short = name.split('(')[0].replace('void ', '')
+        short = name.replace('(anonymous namespace)::', '').split('(')[0]
+        short = short.replace('void ', '')
         out.append('| %s | %s | %s | %.1f | %.1f | %.1f |'
                    % (short, kernel_pass(name) or '', r['Calls'],
                       float(r['AverageNs']) / 1e3, float(r['MinNs']) / 1e3,
@@ -1015,6 +1016,8 @@ def main():
     ap.add_argument('--stats-summary', nargs=2, metavar=('STATS_CSV', 'COMMAND'),
                     help='print the kernel table of a rocprofv3 --stats run '
                          '(no GPU work)')
+    ap.add_argument('--summary-title', default='256^3 filtered LS iteration',
+                    help='title of --stats-summary')
     ap.add_argument('--traffic-summary', nargs=3,
                     metavar=('FETCH_CSV', 'WRITE_CSV', 'COMMAND'),
                     help='print the counter summary of two rocprofv3 --pmc '
@@ -1024,7 +1027,8 @@ def main():
         print(traffic_summary(*args.traffic_summary), end='')
         return
     if args.stats_summary:
-        print(stats_summary(*args.stats_summary), end='')
+        print(stats_summary(*args.stats_summary, title=args.summary_title),
+              end="")
         return
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only; the multi-GPU '
