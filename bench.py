@@ -116,6 +116,57 @@ def cpu_baseline(n, kind, phase, table, mat0, iters=1):
     return iters / t, t
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_workers():
+    """Host threads for the all-cores baseline: the CPUs this process may
+    run on, at most 16 (one GPU's share of a GPU node)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline_threads(n, kind, phase, table, mat0, iters=1, workers=1):
+    """The same oracle iteration on `workers` host threads: pocketfft with
+    that many workers, and the local operator, the frequency loop and the
+    update on axis-0 slabs in a thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import janus_oracle as jo
+    sph = table[phase, 0]
+    dev = table[phase, 1]
+    E = np.zeros(6)
+    E[5] = 1.0
+    eps = np.empty((n, n, n, 6))
+    eps[...] = E
+    tau = np.empty_like(eps)
+    sq = np.zeros(n)
+    with ThreadPoolExecutor(workers) as pool:
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            def local(i):
+                tau[i] = jo.iso4_apply(sph[i], dev[i], eps[i])
+            list(pool.map(local, range(n)))
+            y = jo.green_apply(tau, kind, mat0.g, mat0.nu, 1.0,
+                               workers=workers)
+
+            def update(i):
+                new = E - y[i]
+                sq[i] = np.sum((new - eps[i]) ** 2)
+                eps[i] = new
+            list(pool.map(update, range(n)))
+            np.sqrt(sq.sum())
+        t = time.perf_counter() - t0
+    return iters / t, t
+
+
 def stream_copy_gbps(mib=2048, reps=10):
     """Bandwidth (read + write bytes / time) of the library's 16-byte
     stream-copy kernel: the practical HBM ceiling next to the 8 TB/s
@@ -1096,6 +1147,7 @@ def main():
           flush=True)
 
     cpu = None
+    cpu_all = None
     if not args.no_cpu:
         print('bench: CPU baseline (%d iteration(s), 1 thread)' % args.cpu_iters,
               file=sys.stderr, flush=True)
@@ -1105,6 +1157,19 @@ def main():
                'sample': '%d LS iteration(s) of the same %d^3 workload with '
                          'the numpy oracle (pocketfft), %.1f s'
                          % (args.cpu_iters, n, t)}
+        workers = host_workers()
+        print('bench: CPU baseline (%d iteration(s), %d threads)'
+              % (args.cpu_iters, workers), file=sys.stderr, flush=True)
+        v, t = cpu_baseline_threads(n, args.kind, phase, table, mat0,
+                                    args.cpu_iters, workers)
+        cpu_all = {'value': round(v, 6), 'unit': 'iterations/s',
+                   'cores': workers, 'kind': 'port',
+                   'cpus_available': len(os.sched_getaffinity(0)),
+                   'cpu_model': cpu_model(),
+                   'sample': '%d LS iteration(s) of the same %d^3 workload '
+                             'with the oracle on %d threads (scipy pocketfft '
+                             'workers, slab thread pool), %.1f s'
+                             % (args.cpu_iters, n, workers, t)}
 
     line = {
         'metric': 'LS iterations/s at 256^3 FP64 (% HBM roofline)',
@@ -1153,6 +1218,7 @@ def main():
                 else 8),
         },
         'cpu_baseline': cpu,
+        'cpu_baseline_all_cores': cpu_all,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mb > 0:

@@ -296,34 +296,65 @@ def irfft_janus(c, shape):
     return np.fft.irfftn(z, s=shape, axes=tuple(range(len(shape))))
 
 
-def _half_indices(n, i0):
-    """Integer frequency indices of the half spectrum for axis-0 slab i0."""
+def _half_indices(n, i0, i1=None):
+    """Integer frequency indices of the half spectrum for axis-0 slab i0
+    (or, given i1, for the slabs i0 ... i1 - 1, stacked on a first axis)."""
     dim = len(n)
-    axes = [np.array([i0])] + [np.arange(ni) for ni in n[1:-1]] \
+    first = np.arange(i0, i0 + 1 if i1 is None else i1)
+    axes = [first] + [np.arange(ni) for ni in n[1:-1]] \
         + [np.arange(n[-1] // 2 + 1)]
     grids = np.meshgrid(*axes, indexing='ij')
     b = np.stack(grids, axis=-1)
+    if i1 is not None:
+        return b
     return b.reshape(b.shape[1:]) if dim > 1 else b
 
 
-def green_apply(x, kind, g, nu, h=1.0):
+def green_apply(x, kind, g, nu, h=1.0, workers=1):
     """y = Gamma0[x] on the grid (green.pyx:182-216, 353-390, 453-580).
 
     r2c of each component, per-frequency symmetric matvec applied to Re
     and Im, c2r of each component (c2r drops the imaginary parts of the
     DC and Nyquist planes of the last axis, as FFTW and pocketfft do).
+
+    workers > 1 spreads the same arithmetic over host threads: scipy's
+    pocketfft with that many workers, and blocks of axis-0 slabs of the
+    frequency loop in a thread pool (numpy releases the GIL inside its
+    array loops, which blocks of slabs make long).  janus itself is
+    single-threaded; this form only times the path on every host core.
     """
     x = np.asarray(x, dtype=np.float64)
     n = x.shape[:-1]
     dim = len(n)
     axes = tuple(range(dim))
-    xh = np.fft.rfftn(x, axes=axes)
+    if workers > 1:
+        import scipy.fft
+        xh = scipy.fft.rfftn(x, axes=axes, workers=workers)
+    else:
+        xh = np.fft.rfftn(x, axes=axes)
     yh = np.empty_like(xh)
-    for i0 in range(n[0]):
+
+    def slab(i0):
         b = _half_indices(n, i0)
         G = gamma_hat(kind, b, n, h, g, nu)
         yh[i0] = np.einsum('...ij,...j->...i', G, xh[i0].real) \
             + 1j * np.einsum('...ij,...j->...i', G, xh[i0].imag)
+
+    if workers > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        step = max(1, -(-n[0] // (4 * workers)))
+
+        def block(a):
+            e = min(a + step, n[0])
+            G = gamma_hat(kind, _half_indices(n, a, e), n, h, g, nu)
+            yh[a:e] = np.einsum('...ij,...j->...i', G, xh[a:e].real) \
+                + 1j * np.einsum('...ij,...j->...i', G, xh[a:e].imag)
+
+        with ThreadPoolExecutor(workers) as pool:
+            list(pool.map(block, range(0, n[0], step)))
+        return scipy.fft.irfftn(yh, s=n, axes=axes, workers=workers)
+    for i0 in range(n[0]):
+        slab(i0)
     return np.fft.irfftn(yh, s=n, axes=axes)
 
 

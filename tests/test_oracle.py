@@ -234,6 +234,17 @@ def test_h_cancels():
         np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-16)
 
 
+@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+def test_green_apply_threaded_matches_serial(kind):
+    """The all-cores form of green_apply (timing baseline) computes the
+    same field as the serial one, up to the FFT library's rounding."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((12, 10, 14, 6))
+    a = jo.green_apply(x, kind, 0.75, 0.3)
+    b = jo.green_apply(x, kind, 0.75, 0.3, workers=4)
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-14 * np.abs(a).max())
+
+
 # ---------------------------------------------------------------- local ops
 
 @pytest.mark.parametrize('dim', [2, 3])
