@@ -67,7 +67,7 @@ def pass_bytes(n, fused, sym=6, m=1):
             0]
 
 
-def survey_bytes(n, sym=6, m=1):
+def five_pass_bytes(n, sym=6, m=1):
     """B_alg = (144 + m) N + 768 Nh, the 5-pass per-iteration byte count
     the roofline target is stated against."""
     N = n[0] * n[1] * n[2]
@@ -504,7 +504,7 @@ def workload_stiffness(args):
     C_b = Cs[0]
     _log('batched 6 loads: %.3f s, widths %s' % (t_b, batch.groups()))
     total_it = int(sum(its))
-    B = survey_bytes((n, n, n))
+    B = five_pass_bytes((n, n, n))
     _line('6x6 effective stiffness at %d^3 (six load cases, 1e-10)' % n,
           round(t_seq, 4), 's', {
               'workload': '%d^3 spheres r=%g vf~0.2 seed 2014, mu 1/50 nu '
@@ -913,7 +913,7 @@ def workload_large(args):
         t_b = time.perf_counter() - t0
     assert all(r.iterations == k for r in res)
     its = 6 * k / t_b
-    B = survey_bytes((n, n, n))
+    B = five_pass_bytes((n, n, n))
     _log('6-load batch: %d iterations each in %.3f s (%.2f load-it/s)'
          % (k, t_b, its))
     del batch, fields, res
@@ -1198,9 +1198,9 @@ def main():
             'iteration_GBps': round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
             'iteration_roofline_frac': round(
                 iter_bytes / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            'B_alg_5pass': int(survey_bytes((n, n, n))),
+            'B_alg_5pass': int(five_pass_bytes((n, n, n))),
             'B_alg_5pass_roofline_frac': round(
-                survey_bytes((n, n, n)) / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                five_pass_bytes((n, n, n)) / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 4),
             'passes': passes,
             'hbm_copy_GBps': round(copy_gbps, 1),
