@@ -187,6 +187,22 @@ int jx_solver_solve_cg(jx_solver *solver, double *eps, const double *E,
                        const double *rhs, double tol, int max_iter,
                        double res_scale, double *res_hist, int *iters,
                        int *status, void *stream);
+/* Restarted GMRES on the same operators (no janus counterpart in the
+ * library; janus's examples/square_inclusion_scipy.py:52-59, 88 hands the
+ * strain form (I + Gamma0 dC) x = -Gamma0[dC : E] to scipy's GMRES).  Same
+ * equation, arguments, rhs and residual definition as jx_solver_solve_cg,
+ * but in the Euclidean inner product and with no symmetry needed: any
+ * sym x sym local operator, including non-symmetric matrix fields.
+ * restart in [1, 256]: Arnoldi steps per cycle; the iterate is updated and
+ * the residual recomputed after each cycle.  The residual after iteration k
+ * is the Arnoldi estimate of ||b - A x_k|| / (sqrt(N) ||E||).  Allocates
+ * restart + 3 fields of the grid for the call and frees them before it
+ * returns; reads one Hessenberg column back per iteration.  Synchronizes
+ * the stream. */
+int jx_solver_solve_gmres(jx_solver *solver, double *eps, const double *E,
+                          const double *rhs, double tol, int max_iter,
+                          int restart, double res_scale, double *res_hist,
+                          int *iters, int *status, void *stream);
 /* Schedule choice (no janus counterpart).  fused = 1 (default): the last
  * pass of an iteration also applies the local operator to the new iterate
  * and transforms it for the next iteration, so the iterate is written once

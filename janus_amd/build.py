@@ -9,7 +9,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 SOURCES = ['jx_api.hip', 'jx_batch.hip', 'jx_rows_fwd.hip', 'jx_rows_inv.hip',
-           'jx_rows_fused.hip', 'jx_axis.hip', 'jx_misc.hip', 'jx_cg.hip']
+           'jx_rows_fused.hip', 'jx_axis.hip', 'jx_misc.hip', 'jx_cg.hip',
+           'jx_gmres.hip']
 OUTPUT = os.path.join(HERE, 'libjanus_amd.so')
 
 HIPCC_FLAGS = [

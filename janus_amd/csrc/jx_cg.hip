@@ -24,6 +24,25 @@
 
 namespace jx {
 
+int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
+              hipStream_t s) {
+  jx_green *G = S->green;
+  const Grid &g = G->g;
+  const int C = G->sym;
+  int rc = launch_rows_forward(g, C, x, G->spec, S->local->dev, st, s);
+  if (rc) return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, st, s)))
+    return rc;
+  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, st, s)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, st, s)))
+    return rc;
+  RowsInverseLs plain;
+  return launch_rows_inverse(g, C, G->spec, y, plain, st, nullptr, s);
+}
+
 namespace {
 
 enum CgMode { CG_FILL, CG_INIT, CG_QP, CG_XR, CG_P, CG_FINAL, CG_R, CG_FINAL_P };
@@ -36,37 +55,6 @@ constexpr int kCgBlocks = 4096;  // blocks of the vector kernels (partials)
 #define JX_CG_FUSED 1
 #endif
 constexpr int kScalarThreads = 1024;
-
-// The C components of voxel v: 16-byte accesses for C = 6 (48-byte voxels).
-template <int C>
-__device__ __forceinline__ void load_voxel(const double *a, int64_t v,
-                                           double *x) {
-  if constexpr (C == 6) {
-    const double2 *a2 = (const double2 *)(a + v * 6);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double2 t = a2[i];
-      x[2 * i] = t.x;
-      x[2 * i + 1] = t.y;
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < C; ++c) x[c] = a[v * C + c];
-  }
-}
-
-template <int C>
-__device__ __forceinline__ void store_voxel(double *a, int64_t v,
-                                            const double *x) {
-  if constexpr (C == 6) {
-    double2 *a2 = (double2 *)(a + v * 6);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) a2[i] = make_double2(x[2 * i], x[2 * i + 1]);
-  } else {
-#pragma unroll
-    for (int c = 0; c < C; ++c) a[v * C + c] = x[c];
-  }
-}
 
 template <int C, int MODE>
 __global__ void __launch_bounds__(kThreads)
@@ -259,32 +247,12 @@ int scalar(const double *partials, CgScalars *cs, SolveState *st,
   return JX_OK;
 }
 
-// y = G[dC : x] (five passes, the last one plain: scaled c2r).
-int matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
-           hipStream_t s) {
-  jx_green *G = S->green;
-  const Grid &g = G->g;
-  const int C = G->sym;
-  int rc = launch_rows_forward(g, C, x, G->spec, S->local->dev, st, s);
-  if (rc) return rc;
-  if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, st, s)))
-    return rc;
-  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, st, s)))
-    return rc;
-  if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, st, s)))
-    return rc;
-  RowsInverseLs plain;
-  return launch_rows_inverse(g, C, G->spec, y, plain, st, nullptr, s);
-}
-
 template <int C>
 int cg_iteration(jx_solver *S, double *x, double wd, double wt, hipStream_t s) {
   const Grid &g = S->green->g;
   double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
   int rc;
-  if ((rc = matvec(S, p, q, S->st, s))) return rc;
+  if ((rc = ls_matvec(S, p, q, S->st, s))) return rc;
   if ((rc = vec<C, CG_QP>(x, r, p, q, g, wd, wt, S->cg_s, S->st,
                           S->cg_partials, s)))
     return rc;
@@ -348,7 +316,7 @@ int cg_solve(jx_solver *S, double *eps, const double *rhs, int max_iter,
   if ((rc = vec<C, CG_FILL>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
                             S->cg_partials, s)))
     return rc;
-  if ((rc = matvec(S, eps, q, S->st, s))) return rc;
+  if ((rc = ls_matvec(S, eps, q, S->st, s))) return rc;
   if ((rc = vec<C, CG_INIT>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
                             S->cg_partials, s, rhs)))
     return rc;

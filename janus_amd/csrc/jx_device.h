@@ -235,6 +235,37 @@ struct LsArgs {
   double wd = 0., wt = 0.;
 };
 
+// The C components of voxel v: 16-byte accesses for C = 6 (48-byte voxels).
+template <int C>
+__device__ __forceinline__ void load_voxel(const double *a, int64_t v,
+                                           double *x) {
+  if constexpr (C == 6) {
+    const double2 *a2 = (const double2 *)(a + v * 6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double2 t = a2[i];
+      x[2 * i] = t.x;
+      x[2 * i + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) x[c] = a[v * C + c];
+  }
+}
+
+template <int C>
+__device__ __forceinline__ void store_voxel(double *a, int64_t v,
+                                            const double *x) {
+  if constexpr (C == 6) {
+    double2 *a2 = (double2 *)(a + v * 6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a2[i] = make_double2(x[2 * i], x[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) a[v * C + c] = x[c];
+  }
+}
+
 // a : C0 : b of one voxel, C0 = dim k0 J + 2 g0 K in Mandel notation:
 // wd (a . b) + wt tr(a) tr(b) with wd = 2 g0, wt = k0 - 2 g0 / dim.
 template <int C>

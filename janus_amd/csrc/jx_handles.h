@@ -57,4 +57,8 @@ int green_ensure_device(jx_green *G);
 int solver_iteration(jx_solver *S, double *eps, double *spec, SolveState *st,
                      const RowsInverseLs &ls, bool first, int *nb,
                      hipStream_t s);
+// y = Gamma0[dC : x] with the operators of solver S (five passes, the last
+// one a plain scaled c2r); x and y are fields of the grid (jx_cg.hip).
+int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
+              hipStream_t s);
 }  // namespace jx

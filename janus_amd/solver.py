@@ -179,6 +179,42 @@ class LSSolver:
         n = iters.value
         return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
 
+    def solve_gmres(self, E, tol=1e-10, max_iter=200, restart=20, eps=None,
+                    rhs=None, res_scale=None):
+        """Restarted GMRES on the same strain form as :meth:`solve_cg`
+        (examples/square_inclusion_scipy.py:52-59, 88 gives it to scipy's
+        GMRES, whose default restart is 20), in the Euclidean inner product:
+        no symmetry of dC is needed.  The residual is the Arnoldi estimate of
+        ||b - A x_k|| with the normalization of :meth:`solve`; ``restart``
+        Arnoldi steps per cycle (restart + 3 fields of the grid are
+        allocated for the call).  Other arguments as :meth:`solve_cg`.
+        Returns SolveResult(eps, residuals, iterations, status)."""
+        t = _device.torch()
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        if E.shape != (self.sym,):
+            raise ValueError('E must have shape ({0},)'.format(self.sym))
+        if eps is None:
+            eps = t.empty(self.shape, dtype=t.float64, device='cuda')
+        elif not (_device.is_device_tensor(eps) and eps.is_contiguous() and
+                  tuple(eps.shape) == self.shape and eps.dtype == t.float64):
+            raise ValueError('eps must be a contiguous float64 GPU tensor of '
+                             'shape {0}'.format(self.shape))
+        if rhs is not None and not (_device.is_device_tensor(rhs) and
+                                    rhs.is_contiguous() and
+                                    tuple(rhs.shape) == self.shape):
+            raise ValueError('rhs must be a contiguous GPU field of shape '
+                             '{0}'.format(self.shape))
+        hist = np.zeros(max_iter, dtype=np.float64)
+        iters = ctypes.c_int(0)
+        status = ctypes.c_int(0)
+        _lib.check(_lib.lib().jx_solver_solve_gmres(
+            self._handle, eps.data_ptr(), E.ctypes.data, _lib.ptr(rhs),
+            float(tol), int(max_iter), int(restart), float(res_scale or 0.0),
+            hist.ctypes.data, ctypes.byref(iters), ctypes.byref(status),
+            _lib.current_stream()))
+        n = iters.value
+        return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
+
     def mean_stress(self, eps, E, mat0):
         """<sigma> = C0 : E + <(C - C0) : eps> (square_basic.rst:212-221)."""
         tau, _ = self.local.means(eps)

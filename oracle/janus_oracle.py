@@ -40,6 +40,10 @@ cg_scheme                    examples/square_inclusion.py:40-47, 71-86       con
                              (Krylov on the same operators; strain form      Schwinger equation and equals the basic
                              in the C0 inner product, no janus counterpart)  scheme's; C0-conjugacy of the directions;
                                                                              dense direct solve on a 4x4 grid
+gmres_scheme                 examples/square_inclusion_scipy.py:52-59, 88    dense direct solve on 4x4 grids (symmetric
+                             (GMRES on the strain form, restarted; no        and non-symmetric dC, with and without
+                             janus counterpart)                              restarts); Arnoldi estimate = true residual;
+                                                                             finite termination; equals CG's solution
 ===========================  ==============================================  ==========================================
 """
 
@@ -628,4 +632,86 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
         beta = rr_new / rr
         p = r + beta * p
         rr = rr_new
+    return Ef + x, np.array(res)
+
+
+def gmres_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400,
+                 restart=30, h=1.0, rhs=None, res_scale=None, mats=None):
+    """Restarted GMRES on (I + Gamma0 dC) eps~ = -Gamma0[dC : E] (+ rhs) for
+    the fluctuation eps~, the strain form that
+    examples/square_inclusion_scipy.py:52-59, 88 hands to scipy's GMRES.
+
+    Euclidean inner product; Arnoldi with one classical Gram-Schmidt
+    projection per step; Givens rotations on the Hessenberg matrix.  The
+    residual after iteration k is the Arnoldi estimate |g_{k+1}| of
+    ||b - A eps~_k|| times 1 / (sqrt(N) ||E||) (``res_scale`` replaces it),
+    the normalization of the basic scheme's residual (square_basic.py:86,
+    91).  Stops after the first iteration below tol.  After ``restart``
+    steps the iterate is updated and the residual b - A eps~ recomputed.
+    No symmetry is needed: ``mats`` may hold non-symmetric per-voxel
+    matrices.  Returns (eps, residuals); no iteration when b = 0.
+    """
+    E = np.asarray(E, dtype=np.float64)
+    shape = _grid_shape(sph, mats)
+    local = _local(sph, dev, mats)
+    N = int(np.prod(shape))
+    nE = np.linalg.norm(E)
+    normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
+    if res_scale is not None:
+        normalization = res_scale
+
+    def op(v):
+        return v + green_apply(local(v), kind, g0, nu0, h)
+
+    Ef = np.empty(shape + (E.size,), dtype=np.float64)
+    Ef[...] = E
+    b = -green_apply(local(Ef), kind, g0, nu0, h)
+    if rhs is not None:
+        b = rhs + b
+    x = np.zeros_like(b)
+    r = b
+    res = []
+    while len(res) < max_iter:
+        beta = np.linalg.norm(r)
+        if beta == 0.:
+            break
+        V = [r / beta]
+        H = np.zeros((restart + 1, restart))
+        cs = np.zeros(restart)
+        sn = np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        k = 0
+        for j in range(restart):
+            w = op(V[j])
+            hj = np.array([np.sum(w * V[i]) for i in range(j + 1)])
+            for i in range(j + 1):
+                w = w - hj[i] * V[i]
+            hn = np.linalg.norm(w)
+            H[:j + 1, j] = hj
+            H[j + 1, j] = hn
+            for i in range(j):
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            d = np.hypot(H[j, j], H[j + 1, j])
+            cs[j] = H[j, j] / d
+            sn[j] = H[j + 1, j] / d
+            H[j, j] = d
+            H[j + 1, j] = 0.
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            k = j + 1
+            res.append(normalization * abs(g[j + 1]))
+            if res[-1] < tol or len(res) >= max_iter or hn == 0.:
+                break
+            V.append(w / hn)
+        y = np.zeros(k)
+        for i in range(k - 1, -1, -1):
+            y[i] = (g[i] - np.dot(H[i, i + 1:k], y[i + 1:k])) / H[i, i]
+        for i in range(k):
+            x = x + y[i] * V[i]
+        if res[-1] < tol or len(res) >= max_iter:
+            break
+        r = b - op(x)
     return Ef + x, np.array(res)

@@ -468,6 +468,95 @@ def test_cg_matches_dense_direct_solve():
     np.testing.assert_allclose(eps, direct, rtol=0, atol=1e-12)
 
 
+def _dense_operator(local, shape, sym, kind, g0):
+    """I + Gamma0 dC assembled column by column (tiny grids)."""
+    n = int(np.prod(shape)) * sym
+    cols = []
+    for i in range(n):
+        v = np.zeros(n)
+        v[i] = 1.0
+        v = v.reshape(shape + (sym,))
+        cols.append((v + jo.green_apply(local(v), kind, g0, 0.3)).ravel())
+    return np.array(cols).T
+
+
+def _nonsymmetric_mats(shape, sym, seed):
+    """Per-voxel dC = isotropic part + a non-symmetric perturbation."""
+    rng = np.random.default_rng(seed)
+    a = np.empty(shape + (sym, sym))
+    base = jo.iso4_matrix(*jo.delta_c_coefficients(4.0, 0.3, 1.0, 0.3,
+                                                   2 if sym == 3 else 3),
+                          2 if sym == 3 else 3)
+    a[...] = base
+    return a + 0.3 * rng.standard_normal(a.shape)
+
+
+@pytest.mark.parametrize('restart', [30, 7, 3])
+@pytest.mark.parametrize('symmetric', [True, False])
+def test_gmres_matches_dense_direct_solve(restart, symmetric):
+    """4 x 4 grid: GMRES (with and without restarts) reaches the direct
+    solution of the assembled operator, for symmetric and non-symmetric dC;
+    its Arnoldi residual estimate equals the true residual."""
+    g0 = 1.0
+    if symmetric:
+        sph, dev = _cg_setup(4, g0)
+        mats = None
+        local = lambda v: jo.iso4_apply(sph, dev, v)  # noqa: E731
+        shape = sph.shape
+    else:
+        mats = _nonsymmetric_mats((4, 4), 3, 1)
+        sph = dev = None
+        local = lambda v: jo.matrix_apply(mats, v)  # noqa: E731
+        shape = (4, 4)
+    E = np.array([0.2, 0.5, -1.0])
+    A = _dense_operator(local, shape, 3, 'truncated', g0)
+    rhs = np.empty(shape + (3,))
+    rhs[...] = E
+    direct = np.linalg.solve(A, rhs.ravel()).reshape(rhs.shape)
+    eps, res = jo.gmres_scheme(sph, dev, E, 'truncated', g0, 0.3, tol=1e-13,
+                               max_iter=300, restart=restart, mats=mats)
+    np.testing.assert_allclose(eps, direct, rtol=0, atol=1e-11)
+    assert res[-1] < 1e-13
+    true = np.linalg.norm(rhs.ravel() - A @ eps.ravel()) / 4. / \
+        np.linalg.norm(E)
+    assert true < 1e-12
+    if restart == 30:
+        # no restart: finite termination within the 48 unknowns
+        assert len(res) <= 48
+
+
+def test_gmres_residual_is_true_residual():
+    """Without restarts the Arnoldi estimate |g_k+1| equals ||b - A x_k||:
+    checked at every iteration count k against the solution truncated
+    there."""
+    g0 = 5.5
+    sph, dev = _cg_setup(8, g0)
+    E = np.array([0.1, -0.3, 1.0])
+    _, res = jo.gmres_scheme(sph, dev, E, 'truncated', g0, 0.3, tol=0.,
+                             max_iter=12, restart=30)
+    for k in (1, 4, 12):
+        eps_k, _ = jo.gmres_scheme(sph, dev, E, 'truncated', g0, 0.3, tol=0.,
+                                   max_iter=k, restart=30)
+        lhs = eps_k + jo.green_apply(jo.iso4_apply(sph, dev, eps_k),
+                                     'truncated', g0, 0.3)
+        true = np.linalg.norm(lhs - E) / np.sqrt(64) / np.linalg.norm(E)
+        np.testing.assert_allclose(true, res[k - 1], rtol=1e-9)
+
+
+@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+def test_gmres_equals_cg_solution(kind):
+    """Symmetric dC: GMRES and CG reach the same field (the operator's
+    unique solution)."""
+    g0 = 1.0
+    sph, dev = _cg_setup(16, g0)
+    E = np.array([0.1, -0.3, 1.0])
+    a, _ = jo.cg_scheme(sph, dev, E, kind, g0, 0.3, tol=1e-13, max_iter=500)
+    b, res = jo.gmres_scheme(sph, dev, E, kind, g0, 0.3, tol=1e-13,
+                             max_iter=500, restart=20)
+    assert res[-1] < 1e-13
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-11 * np.abs(a).max())
+
+
 def test_c0_inner_matches_matrix():
     rng = np.random.default_rng(3)
     for dim, sym in ((2, 3), (3, 6)):
