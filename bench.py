@@ -475,6 +475,22 @@ def workload_stiffness(args):
     _sync()
     t_cg = time.perf_counter() - t0
     _log('CG, C0 = matrix, 6 loads: %.3f s, iterations %s' % (t_cg, its_cg))
+    # the same six loads by restarted GMRES (no symmetry needed)
+    s_div.solve_gmres(_unit(6, 0), tol=1e-10, max_iter=3)  # warm-up
+    C_gm = np.zeros((6, 6))
+    its_gm = []
+    _sync()
+    t0 = time.perf_counter()
+    for j in range(6):
+        rg = s_div.solve_gmres(_unit(6, j), tol=1e-10, max_iter=2000,
+                               restart=args.gmres_restart)
+        its_gm.append(rg.iterations)
+        tau, _ = s_div.local.means(rg.eps)
+        C_gm[:, j] = reference_stiffness(mat_div).dot(_unit(6, j)) + tau
+    _sync()
+    t_gm = time.perf_counter() - t0
+    _log('GMRES(%d), C0 = matrix, 6 loads: %.3f s, iterations %s'
+         % (args.gmres_restart, t_gm, its_gm))
     del s_div
     mat0 = material.create(25.5, 0.3, 3)
     solver, table = _two_phase_solver(phase, mat_m, mat_i, mat0, 'filtered')
@@ -532,6 +548,12 @@ def workload_stiffness(args):
                                         np.abs(C_cg).max()),
                   'rel_diff_to_midpoint_basic_C_eff': float(
                       np.abs(C_cg - C_seq).max() / np.abs(C_seq).max())},
+              'C0_matrix_gmres': {
+                  'restart': args.gmres_restart,
+                  'iterations': its_gm,
+                  'seconds': round(t_gm, 4),
+                  'rel_diff_to_cg_C_eff': float(
+                      np.abs(C_gm - C_cg).max() / np.abs(C_cg).max())},
               'oracle_%d_iterations_strain_rel_err' % k: rel},
           higher=False)
 
@@ -1056,6 +1078,8 @@ def main():
     ap.add_argument('--batch-count', type=int, default=32,
                     help='microstructures of the batched workload')
     ap.add_argument('--iters-per-launch', type=int, default=8)
+    ap.add_argument('--gmres-restart', type=int, default=20,
+                    help='Arnoldi steps per GMRES cycle (stiffness workload)')
     ap.add_argument('--per-group', action='store_true',
                     help='batched workload: also time each grid size alone')
     ap.add_argument('--adjoint-reps', type=int, default=2)
