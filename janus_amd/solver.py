@@ -235,17 +235,28 @@ def delta_stiffness(mat, mat0):
     return mat.dim * (mat.k - mat0.k), 2. * (mat.g - mat0.g)
 
 
-def effective_stiffness(solver, mat0, tol=1e-10, max_iter=200):
+def effective_stiffness(solver, mat0, tol=1e-10, max_iter=200,
+                        method='basic', restart=20):
     """Columns C_eff[:, j] = <sigma> under the Mandel unit loads E = e_j.
 
+    method: 'basic' (the fixed point), 'cg' (:meth:`LSSolver.solve_cg`) or
+    'gmres' (:meth:`LSSolver.solve_gmres` with ``restart``).
     Returns (C_eff, iterations per load, status per load)."""
+    if method not in ('basic', 'cg', 'gmres'):
+        raise ValueError("method must be 'basic', 'cg' or 'gmres'")
     sym = solver.sym
     C = np.zeros((sym, sym))
     its, sts = [], []
     for j in range(sym):
         E = np.zeros(sym)
         E[j] = 1.0
-        r = solver.solve(E, tol=tol, max_iter=max_iter)
+        if method == 'cg':
+            r = solver.solve_cg(E, tol=tol, max_iter=max_iter)
+        elif method == 'gmres':
+            r = solver.solve_gmres(E, tol=tol, max_iter=max_iter,
+                                   restart=restart)
+        else:
+            r = solver.solve(E, tol=tol, max_iter=max_iter)
         C[:, j] = solver.mean_stress(r.eps, E, mat0)
         its.append(r.iterations)
         sts.append(r.status)

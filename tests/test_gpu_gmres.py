@@ -151,3 +151,29 @@ def test_gmres_rhs_and_stops():
         solver.solve_gmres(E, restart=0)
     with pytest.raises(ValueError):
         solver.solve_gmres(E, max_iter=0)
+
+
+def test_effective_stiffness_methods_agree():
+    """effective_stiffness by the basic scheme, CG and GMRES (midpoint C0,
+    where all three converge) gives the same 6 x 6 C_eff; with C0 = the
+    matrix only the Krylov methods converge, and they agree."""
+    from janus_amd.solver import effective_stiffness
+    solver, _, _ = iso_solver(3, 16, 'filtered', 10.0, 5.5)
+    mat0 = material.create(5.5, 0.3, 3)
+    Cs = [effective_stiffness(solver, mat0, tol=1e-12, max_iter=2000,
+                              method=m)
+          for m in ('basic', 'cg', 'gmres')]
+    for C, its, sts in Cs:
+        assert sts == ['converged'] * 6
+    for C, _, _ in Cs[1:]:
+        assert rel_err(C, Cs[0][0]) < 1e-10
+    solver, _, _ = iso_solver(3, 16, 'filtered', 10.0, 1.0)
+    mat0 = material.create(1.0, 0.3, 3)
+    Ccg, _, s1 = effective_stiffness(solver, mat0, tol=1e-12, max_iter=500,
+                                     method='cg')
+    Cgm, _, s2 = effective_stiffness(solver, mat0, tol=1e-12, max_iter=500,
+                                     method='gmres', restart=30)
+    assert s1 == s2 == ['converged'] * 6
+    assert rel_err(Cgm, Ccg) < 1e-10
+    with pytest.raises(ValueError):
+        effective_stiffness(solver, mat0, method='bicgstab')
