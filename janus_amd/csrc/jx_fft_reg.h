@@ -8,7 +8,7 @@
 // through LDS.  Plans (V; radices), variant 0 and, where listed, 1:
 //   16 (16; 16)   32 (16; 16 2)   64 (16; 16 4) | (8; 8 8)
 //   128 (16; 16 8) | (8; 8 4 4)   256 (16; 16 16) | (8; 8 8 4)
-//   512 (16; 16 16 2) | (8; 8 8 8)
+//   512 (16; 16 16 2) | (8; 8 8 8)   1024 (16; 16 16 4)
 //   80 (20; 5 4 4)   96 (12; 6 4 4)   160 (20; 10 4 4)
 // Variant 1 holds half the values per thread: fewer registers and more
 // waves per SIMD for one more LDS exchange; each pass picks its variant.
@@ -46,6 +46,7 @@ JX_REG_PLAN(256, 0, 16, 2, 16, 16, 1)
 JX_REG_PLAN(256, 1, 8, 3, 8, 8, 4)
 JX_REG_PLAN(512, 0, 16, 3, 16, 16, 2)
 JX_REG_PLAN(512, 1, 8, 3, 8, 8, 8)
+JX_REG_PLAN(1024, 0, 16, 3, 16, 16, 4)
 JX_REG_PLAN(80, 0, 20, 3, 5, 4, 4)
 JX_REG_PLAN(96, 0, 12, 3, 6, 4, 4)
 JX_REG_PLAN(160, 0, 20, 3, 10, 4, 4)
@@ -54,6 +55,7 @@ JX_REG_PLAN(160, 0, 20, 3, 10, 4, 4)
 // Lengths the register kernels are instantiated for.
 constexpr bool reg_fft_length(int L) {
   return L == 16 || L == 32 || L == 64 || L == 128 || L == 256 || L == 512 ||
+         L == 1024 ||
          L == 80 || L == 96 || L == 160;
 }
 
@@ -351,6 +353,7 @@ int with_plan(int L, int k, F &&f) {
     case 128: return with_variant<128>(k, f);
     case 256: return with_variant<256>(k, f);
     case 512: return with_variant<512>(k, f);
+    case 1024: return with_variant<1024>(k, f);
     case 80: return with_variant<80>(k, f);
     case 96: return with_variant<96>(k, f);
     case 160: return with_variant<160>(k, f);

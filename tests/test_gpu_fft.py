@@ -53,7 +53,8 @@ def test_c2r(shape, delta, inplace):
                                    (4, 4, 4), (5, 3, 7), (2, 9, 1), (1, 1, 1),
                                    (30, 50, 64), (40, 50, 60), (200, 300),
                                    (80, 96), (160, 80), (96, 160),
-                                   (16, 96, 80), (20, 160, 24), (80, 96, 160)])
+                                   (16, 96, 80), (20, 160, 24), (80, 96, 160),
+                                   (1024, 24), (6, 1024, 10), (12, 10, 1024)])
 def test_roundtrip_and_numpy(shape):
     rng = np.random.default_rng(11)
     transform = fft.create_real(shape)

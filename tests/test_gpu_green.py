@@ -46,7 +46,9 @@ def test_apply_fixtures(name, flag):
 @pytest.mark.parametrize('n', [(7, 9), (12, 10), (16, 16), (1, 8), (9, 1),
                                (5, 6, 7), (9, 8, 10), (8, 8, 8), (3, 1, 4),
                                (16, 12, 20), (80, 96), (160, 80),
-                               (12, 80, 96), (96, 10, 160), (24, 16, 80)])
+                               (12, 80, 96), (96, 10, 160), (24, 16, 80),
+                               (1024, 12), (12, 1024), (1024, 8, 16),
+                               (8, 1024, 16), (8, 16, 1024)])
 @pytest.mark.parametrize('h', [1.0, 0.25])
 def test_apply_vs_oracle(kind, n, h):
     rng = np.random.default_rng(zlib.crc32(repr((kind, n)).encode()))
