@@ -209,7 +209,7 @@ int jx_solver_solve_gmres(jx_solver *solver, double *eps, const double *E,
  * and not read back; fused = 0: five separate passes per iteration.  Both
  * give the same iterates up to rounding.  get returns the schedule actually
  * used (the fused pass needs a last-axis length with a register FFT:
- * 16, 32, 64, 128, 256, 512, 80, 96 or 160). */
+ * 16, 32, 64, 128, 256, 512, 1024, 80, 96 or 160). */
 int jx_solver_set_fused(jx_solver *solver, int fused);
 int jx_solver_get_fused(jx_solver *solver);
 
