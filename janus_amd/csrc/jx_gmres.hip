@@ -13,6 +13,7 @@
 // reads the new column once per step.  The residual after iteration k is
 // the Arnoldi estimate |g_{k+1}| = ||b - A x_k|| times 1 / (sqrt(N) ||E||).
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
