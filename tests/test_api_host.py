@@ -319,3 +319,16 @@ def test_willot_per_frequency_api(n):
         tau = rng.standard_normal(expected.shape[0])
         np.testing.assert_allclose(greend.apply_by_freq(tau), expected @ tau,
                                    rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize('shape', [(0, 8), (8, 0), (0, 4, 4), (4, 0, 4),
+                                   (4, 4, 0)])
+def test_empty_grids_are_rejected(shape):
+    """A zero-length axis is a shape error (ValueError) for the transform
+    and for both Green operator factories, as janus's negative sizes are."""
+    with pytest.raises(ValueError):
+        fft.create_real(shape)
+    g = material.create(MU, NU, len(shape)).green_operator()
+    for factory in (truncated, filtered):
+        with pytest.raises(ValueError):
+            factory(g, shape, 1.0)
