@@ -187,6 +187,21 @@ int jx_solver_solve_cg(jx_solver *solver, double *eps, const double *E,
                        const double *rhs, double tol, int max_iter,
                        double res_scale, double *res_hist, int *iters,
                        int *status, void *stream);
+/* Conjugate gradients on the polarization form, as janus's
+ * examples/square_inclusion.py:17-52, 77-85 drives PETSc CG: the solver's
+ * local operator L is the inverse of C - C0 (the example's tau2eps,
+ * isotropic_4(1 / (dim dk), 1 / (2 dg))), and tau solves (L + Gamma0) tau =
+ * E in the Euclidean inner product, from tau = 0; eps = L tau is written
+ * too.  L + Gamma0 is definite when C0 lies strictly below (or above) every
+ * phase.  The residual after iteration k is ||E - (L + Gamma0) tau_k|| /
+ * (sqrt(N) ||E||), PETSc's relative residual with PCNONE.  tau, eps:
+ * device fields, output only, not overlapping.  Other arguments as
+ * jx_solver_solve_cg; shares its buffers. */
+int jx_solver_solve_cg_polarization(jx_solver *solver, double *tau,
+                                    double *eps, const double *E, double tol,
+                                    int max_iter, double res_scale,
+                                    double *res_hist, int *iters, int *status,
+                                    void *stream);
 /* Restarted GMRES on the same operators (no janus counterpart in the
  * library; janus's examples/square_inclusion_scipy.py:52-59, 88 hands the
  * strain form (I + Gamma0 dC) x = -Gamma0[dC : E] to scipy's GMRES).  Same

@@ -80,6 +80,9 @@ SIGNATURES = [
     ('jx_solver_solve_cg', ctypes.c_int,
      [_VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
       _VP, _VP, _VP, _VP]),
+    ('jx_solver_solve_cg_polarization', ctypes.c_int,
+     [_VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+      _VP, _VP, _VP, _VP]),
     ('jx_solver_solve_gmres', ctypes.c_int,
      [_VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
       ctypes.c_double, _VP, _VP, _VP, _VP]),

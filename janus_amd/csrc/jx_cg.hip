@@ -25,11 +25,12 @@
 namespace jx {
 
 int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
-              hipStream_t s) {
+              hipStream_t s, const LocalDev *loc) {
   jx_green *G = S->green;
   const Grid &g = G->g;
   const int C = G->sym;
-  int rc = launch_rows_forward(g, C, x, G->spec, S->local->dev, st, s);
+  int rc = launch_rows_forward(g, C, x, G->spec, loc ? *loc : S->local->dev,
+                               st, s);
   if (rc) return rc;
   if (g.dim == 3 &&
       (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, st, s)))
@@ -45,7 +46,11 @@ int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
 
 namespace {
 
-enum CgMode { CG_FILL, CG_INIT, CG_QP, CG_XR, CG_P, CG_FINAL, CG_R, CG_FINAL_P };
+enum CgMode {
+  CG_FILL, CG_INIT, CG_QP, CG_XR, CG_P, CG_FINAL, CG_R, CG_FINAL_P,
+  CG_INIT_E,  // polarization form: r = p = E, x = 0
+  CG_QL       // polarization form: q = G[p] + L p
+};
 enum CgPhase { PH_INIT, PH_ALPHA, PH_BETA };
 
 constexpr int kCgBlocks = 4096;  // blocks of the vector kernels (partials)
@@ -60,7 +65,7 @@ template <int C, int MODE>
 __global__ void __launch_bounds__(kThreads)
     k_cg_vec(double *x, double *r, double *p, double *q, int64_t N, double wd,
              double wt, const CgScalars *cs, const SolveState *st,
-             double *partials, const double *rhs) {
+             double *partials, const double *rhs, LocalDev loc) {
 #pragma clang fp contract(off)
   // the fill and the final sum run whatever the state
   constexpr bool FINAL = MODE == CG_FINAL || MODE == CG_FINAL_P;
@@ -126,6 +131,29 @@ __global__ void __launch_bounds__(kThreads)
       acc0 += c0_dot<C>(rv, rv, wd, wt);
 #pragma unroll
       for (int c = 0; c < C; ++c) acc1 += rv[c] * rv[c];
+    } else if constexpr (MODE == CG_INIT_E) {
+      double z[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        e[c] = st->E[c];
+        z[c] = 0.;
+      }
+      store_voxel<C>(r, v, e);
+      store_voxel<C>(p, v, e);
+      store_voxel<C>(x, v, z);
+      acc0 += c0_dot<C>(e, e, wd, wt);
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc1 += e[c] * e[c];
+    } else if constexpr (MODE == CG_QL) {
+      // q = G[p] + L p (the local operator applied voxel by voxel)
+      double pv[C], qv[C], lv[C];
+      load_voxel<C>(p, v, pv);
+      load_voxel<C>(q, v, qv);
+      local_voxel<C>(loc, v, pv, lv);
+#pragma unroll
+      for (int c = 0; c < C; ++c) qv[c] = qv[c] + lv[c];
+      store_voxel<C>(q, v, qv);
+      acc0 += c0_dot<C>(pv, qv, wd, wt);
     } else if constexpr (MODE == CG_QP) {
       // q = p + G[dC : p]
       double pv[C], qv[C];
@@ -166,7 +194,7 @@ __global__ void __launch_bounds__(kThreads)
     }
   }
   if constexpr (MODE == CG_INIT || MODE == CG_QP || MODE == CG_XR ||
-                MODE == CG_R) {
+                MODE == CG_R || MODE == CG_INIT_E || MODE == CG_QL) {
     double s0 = block_sum(acc0, red);
     __syncthreads();
     double s1 = block_sum(acc1, red);
@@ -230,9 +258,10 @@ __global__ void __launch_bounds__(kScalarThreads)
 template <int C, int MODE>
 int vec(double *x, double *r, double *p, double *q, const Grid &g, double wd,
         double wt, const CgScalars *cs, const SolveState *st,
-        double *partials, hipStream_t s, const double *rhs = nullptr) {
+        double *partials, hipStream_t s, const double *rhs = nullptr,
+        const LocalDev &loc = LocalDev()) {
   hipLaunchKernelGGL((k_cg_vec<C, MODE>), dim3(kCgBlocks), dim3(kThreads), 0,
-                     s, x, r, p, q, g.N, wd, wt, cs, st, partials, rhs);
+                     s, x, r, p, q, g.N, wd, wt, cs, st, partials, rhs, loc);
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
@@ -346,6 +375,59 @@ int cg_solve(jx_solver *S, double *eps, const double *rhs, int max_iter,
                           S->cg_partials, s);
 }
 
+// Polarization form (examples/square_inclusion.py:17-52, 77-85): with the
+// solver's local operator L = dC^-1 (the example's tau2eps), tau solves
+// (L + G) tau = E by CG in the Euclidean inner product (wd = 1, wt = 0),
+// from tau = 0; eps = L tau.  Needs L + G definite: C0 strictly below (or
+// above) every phase.
+template <int C>
+int cg_pol_iteration(jx_solver *S, double *x, hipStream_t s) {
+  const Grid &g = S->green->g;
+  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
+  const LocalDev none;
+  int rc;
+  if ((rc = ls_matvec(S, p, q, S->st, s, &none))) return rc;
+  if ((rc = vec<C, CG_QL>(x, r, p, q, g, 1., 0., S->cg_s, S->st,
+                          S->cg_partials, s, nullptr, S->local->dev)))
+    return rc;
+  if ((rc = scalar<PH_ALPHA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  if ((rc = vec<C, CG_XR>(x, r, p, q, g, 1., 0., S->cg_s, S->st,
+                          S->cg_partials, s)))
+    return rc;
+  if ((rc = scalar<PH_BETA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  return vec<C, CG_P>(x, r, p, q, g, 1., 0., S->cg_s, S->st, S->cg_partials,
+                      s);
+}
+
+template <int C>
+int cg_pol_solve(jx_solver *S, double *tau, double *eps, int max_iter,
+                 hipStream_t s) {
+  const Grid &g = S->green->g;
+  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
+  int rc;
+  if ((rc = vec<C, CG_INIT_E>(tau, r, p, q, g, 1., 0., S->cg_s, S->st,
+                              S->cg_partials, s)))
+    return rc;
+  if ((rc = scalar<PH_INIT>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  int enqueued = 0, chunk = 4;
+  while (true) {
+    int n = std::min(chunk, max_iter - enqueued);
+    for (int i = 0; i < n; ++i)
+      if ((rc = cg_pol_iteration<C>(S, tau, s))) return rc;
+    enqueued += n;
+    JX_HIP(hipMemcpyAsync(S->st_host, S->st, sizeof(SolveState),
+                          hipMemcpyDeviceToHost, s));
+    JX_HIP(hipStreamSynchronize(s));
+    if (S->st_host->done || enqueued >= max_iter) break;
+    chunk = std::min(chunk * 2, 64);
+  }
+  // the fused-free iteration leaves tau complete; eps = L tau
+  return launch_local_apply(g, C, S->local->dev, tau, eps, s);
+}
+
 }  // namespace
 
 }  // namespace jx
@@ -402,6 +484,59 @@ int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E,
   const double wd = 2. * g0, wt = k0 - 2. * g0 / dim;
   int rc = C == 3 ? cg_solve<3>(S, eps, rhs, max_iter, wd, wt, s)
                   : cg_solve<6>(S, eps, rhs, max_iter, wd, wt, s);
+  if (rc) return rc;
+  JX_HIP(hipStreamSynchronize(s));
+  int it = S->st_host->iter;
+  if (iters) *iters = it;
+  if (status) *status = S->st_host->done ? S->st_host->done : JX_SOLVE_MAXITER;
+  if (res_hist && it > 0)
+    JX_HIP(hipMemcpy(res_hist, S->res_dev, sizeof(double) * it,
+                     hipMemcpyDeviceToHost));
+  return JX_OK;
+}
+
+int jx_solver_solve_cg_polarization(jx_solver *S, double *tau, double *eps,
+                                    const double *E, double tol, int max_iter,
+                                    double res_scale, double *res_hist,
+                                    int *iters, int *status, void *stream) {
+  if (!S || !tau || !eps || !E)
+    return fail(JX_EINVAL, "jx_solver_solve_cg_polarization: NULL argument");
+  if (tau == eps) return fail(JX_EINVAL, "tau and eps must not overlap");
+  if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
+  if (!S->local->symmetric)
+    return fail(JX_EINVAL, "conjugate gradients need a symmetric local operator");
+  hipStream_t s = (hipStream_t)stream;
+  const Grid &g = S->green->g;
+  const int C = S->green->sym;
+  if (res_scale <= 0.) {
+    double nE = 0.;
+    for (int c = 0; c < C; ++c) nE += E[c] * E[c];
+    nE = std::sqrt(nE);
+    if (nE == 0.) return fail(JX_EINVAL, "res_scale must be given when E = 0");
+    res_scale = 1.0 / std::sqrt((double)g.N) / nE;
+  }
+  if (!S->cg_buf) {
+    JX_HIP(hipMalloc((void **)&S->cg_buf, sizeof(double) * 3 * C * g.N));
+    JX_HIP(hipMalloc((void **)&S->cg_partials, sizeof(double) * 2 * kCgBlocks));
+    JX_HIP(hipMalloc((void **)&S->cg_s, sizeof(CgScalars)));
+  }
+  if (S->res_cap < max_iter) {
+    if (S->res_dev) (void)hipFree(S->res_dev);
+    S->res_dev = nullptr;
+    S->res_cap = 0;
+    JX_HIP(hipMalloc((void **)&S->res_dev, sizeof(double) * max_iter));
+    S->res_cap = max_iter;
+  }
+  SolveState init = {};
+  init.tol = tol;
+  init.res_scale = res_scale;
+  init.max_iter = max_iter;
+  for (int c = 0; c < 6; ++c) init.E[c] = c < C ? E[c] : 0.;
+  *S->st_host = init;
+  JX_HIP(hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState),
+                        hipMemcpyHostToDevice, s));
+  int rc = C == 3 ? cg_pol_solve<3>(S, tau, eps, max_iter, s)
+                  : cg_pol_solve<6>(S, tau, eps, max_iter, s);
   if (rc) return rc;
   JX_HIP(hipStreamSynchronize(s));
   int it = S->st_host->iter;

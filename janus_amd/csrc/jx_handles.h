@@ -58,7 +58,8 @@ int solver_iteration(jx_solver *S, double *eps, double *spec, SolveState *st,
                      const RowsInverseLs &ls, bool first, int *nb,
                      hipStream_t s);
 // y = Gamma0[dC : x] with the operators of solver S (five passes, the last
-// one a plain scaled c2r); x and y are fields of the grid (jx_cg.hip).
+// one a plain scaled c2r); x and y are fields of the grid (jx_cg.hip).  loc
+// replaces the solver's local operator (a default LocalDev: y = Gamma0[x]).
 int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
-              hipStream_t s);
+              hipStream_t s, const LocalDev *loc = nullptr);
 }  // namespace jx

@@ -179,6 +179,44 @@ class LSSolver:
         n = iters.value
         return SolveResult(eps, hist[:n], n, STATUS.get(status.value, 'unknown'))
 
+    def solve_cg_polarization(self, E, tol=1e-10, max_iter=200, tau=None,
+                              eps=None, res_scale=None):
+        """Conjugate gradients on the polarization form, as janus's
+        examples/square_inclusion.py:17-52, 77-85 runs PETSc CG: this
+        solver's local operator must be the inverse of C - C0 (the
+        example's ``tau2eps``), and tau solves (local + Gamma0) tau = E from
+        tau = 0, in the Euclidean inner product.  Needs C0 strictly below
+        (or above) every phase.  The residual is ||E - A tau_k|| /
+        (sqrt(N) ||E||) (PETSc's relative residual without preconditioner).
+
+        Returns (tau, SolveResult(eps, residuals, iterations, status)) with
+        eps = local : tau."""
+        t = _device.torch()
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        if E.shape != (self.sym,):
+            raise ValueError('E must have shape ({0},)'.format(self.sym))
+        out = []
+        for f in (tau, eps):
+            if f is None:
+                f = t.empty(self.shape, dtype=t.float64, device='cuda')
+            elif not (_device.is_device_tensor(f) and f.is_contiguous() and
+                      tuple(f.shape) == self.shape and f.dtype == t.float64):
+                raise ValueError('tau and eps must be contiguous float64 GPU '
+                                 'tensors of shape {0}'.format(self.shape))
+            out.append(f)
+        tau, eps = out
+        hist = np.zeros(max_iter, dtype=np.float64)
+        iters = ctypes.c_int(0)
+        status = ctypes.c_int(0)
+        _lib.check(_lib.lib().jx_solver_solve_cg_polarization(
+            self._handle, tau.data_ptr(), eps.data_ptr(), E.ctypes.data,
+            float(tol), int(max_iter), float(res_scale or 0.0),
+            hist.ctypes.data, ctypes.byref(iters), ctypes.byref(status),
+            _lib.current_stream()))
+        n = iters.value
+        return tau, SolveResult(eps, hist[:n], n,
+                                STATUS.get(status.value, 'unknown'))
+
     def solve_gmres(self, E, tol=1e-10, max_iter=200, restart=20, eps=None,
                     rhs=None, res_scale=None):
         """Restarted GMRES on the same strain form as :meth:`solve_cg`

@@ -40,6 +40,10 @@ cg_scheme                    examples/square_inclusion.py:40-47, 71-86       con
                              (Krylov on the same operators; strain form      Schwinger equation and equals the basic
                              in the C0 inner product, no janus counterpart)  scheme's; C0-conjugacy of the directions;
                                                                              dense direct solve on a 4x4 grid
+cg_polarization_scheme       examples/square_inclusion.py:17-52, 77-85       dense direct solve of (L + Gamma0) tau = E;
+                             (PETSc CG on the polarization form)             eps = L tau equals the strain-form solution
+                                                                             (cg_scheme at the same C0); the janus
+                                                                             example's own configuration
 gmres_scheme                 examples/square_inclusion_scipy.py:52-59, 88    dense direct solve on 4x4 grids (symmetric
                              (GMRES on the strain form, restarted; no        and non-symmetric dC, with and without
                              janus counterpart)                              restarts); Arnoldi estimate = true residual;
@@ -633,6 +637,52 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
         p = r + beta * p
         rr = rr_new
     return Ef + x, np.array(res)
+
+
+def cg_polarization_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400,
+                           h=1.0, res_scale=None, mats=None):
+    """Conjugate gradients on the polarization form, as
+    examples/square_inclusion.py:17-52, 77-85 runs PETSc CG (KSP 'cg', PC
+    'none', rtol): (L + Gamma0) tau = E, where the local operator L (sph,
+    dev or ``mats``) is the inverse of C - C0 (the example's tau2eps), in
+    the Euclidean inner product from tau = 0.  The residual after
+    iteration k is ||E - A tau_k|| / (sqrt(N) ||E||) (PETSc's relative
+    residual; ``res_scale`` replaces the normalization).  Stops after the
+    first iteration below tol.  Returns (tau, eps = L tau, residuals)."""
+    E = np.asarray(E, dtype=np.float64)
+    shape = _grid_shape(sph, mats)
+    local = _local(sph, dev, mats)
+    N = int(np.prod(shape))
+    nE = np.linalg.norm(E)
+    normalization = 1. / np.sqrt(N) / (nE if nE > 0 else 1.0)
+    if res_scale is not None:
+        normalization = res_scale
+
+    def op(v):
+        return local(v) + green_apply(v, kind, g0, nu0, h)
+
+    r = np.empty(shape + (E.size,), dtype=np.float64)
+    r[...] = E
+    x = np.zeros_like(r)
+    p = r.copy()
+    rr = np.sum(r * r)
+    res = []
+    if rr == 0.:
+        return x, local(x), np.zeros(0)
+    for _ in range(max_iter):
+        q = op(p)
+        alpha = rr / np.sum(p * q)
+        x = x + alpha * p
+        r = r - alpha * q
+        res_k = normalization * np.linalg.norm(r)
+        res.append(res_k)
+        if res_k < tol:
+            break
+        rr_new = np.sum(r * r)
+        beta = rr_new / rr
+        p = r + beta * p
+        rr = rr_new
+    return x, local(x), np.array(res)
 
 
 def gmres_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400,

@@ -1,0 +1,111 @@
+"""Conjugate gradients on the polarization form on the GPU
+(jx_solver_solve_cg_polarization), the solve janus's
+examples/square_inclusion.py runs with PETSc: the solver's local operator is
+(C - C0)^-1 (the example's tau2eps) and tau solves (L + Gamma0) tau = E.
+Checked against the oracle's cg_polarization_scheme iterate for iterate,
+and eps = L tau against the strain-form CG solution at the same C0."""
+
+import numpy as np
+import pytest
+
+import janus_amd.fft.serial as fft
+import janus_amd.material.elastic.linear.isotropic as material
+from janus_amd import microstructure as ms
+from janus_amd.green import filtered, truncated
+from janus_amd.operators import BlockDiagonalOperator2D, \
+    BlockDiagonalOperator3D, isotropic_4
+from janus_amd.solver import LSSolver, delta_stiffness
+from oracle import janus_oracle as jo
+
+pytestmark = pytest.mark.gpu
+
+FACTORIES = {'truncated': truncated, 'filtered': filtered}
+CLS = {2: BlockDiagonalOperator2D, 3: BlockDiagonalOperator3D}
+MATS = ((1.0, 0.3), (10.0, 0.2))  # matrix, inclusion (g, nu) of the example
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def setup(dim, n, kind, g0, centered=True):
+    phase = (ms.square_inclusion(n, 0.5, centered=centered) if dim == 2 else
+             ms.spheres(n, 3.0, 0.2, seed=3))
+    mat0 = material.create(g0, 0.3, dim)
+    dc = np.array([delta_stiffness(material.create(g, nu, dim), mat0)
+                   for g, nu in MATS])
+    inv = 1.0 / dc
+    green = FACTORIES[kind](mat0.green_operator(), phase.shape, 1.0,
+                            fft.create_real(phase.shape))
+    pol = LSSolver(green, CLS[dim].from_phases(phase, inv))
+    green2 = FACTORIES[kind](mat0.green_operator(), phase.shape, 1.0,
+                             fft.create_real(phase.shape))
+    strain = LSSolver(green2, CLS[dim].from_phases(phase, dc))
+    return pol, strain, phase, inv, dc
+
+
+@pytest.mark.parametrize('dim, n, kind, g0', [
+    (2, 32, 'truncated', 0.9), (2, 33, 'filtered', 0.9),
+    (2, 32, 'truncated', 12.0), (3, 16, 'filtered', 0.9),
+    (3, 15, 'truncated', 12.0)])
+def test_polarization_cg_matches_oracle(dim, n, kind, g0):
+    pol, strain, phase, inv, dc = setup(dim, n, kind, g0)
+    E = np.random.default_rng(n).standard_normal(pol.sym)
+    tau, r = pol.solve_cg_polarization(E, tol=1e-10, max_iter=500)
+    assert r.status == 'converged'
+    tau_ref, eps_ref, res_ref = jo.cg_polarization_scheme(
+        inv[phase, 0], inv[phase, 1], E, kind, g0, 0.3, tol=1e-10,
+        max_iter=500)
+    assert r.iterations == len(res_ref)
+    # L = dC^-1 is large where C0 is close to a phase (5 for the matrix at
+    # g0 = 0.9): the operator is worse conditioned than the strain form's,
+    # and the different summation orders of the dot products grow over the
+    # iterations, so the bars are set at the stopping tolerance's scale
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-6,
+                               atol=1e-9 * res_ref[0])
+    assert rel_err(tau.cpu().numpy(), tau_ref) < 1e-9
+    assert rel_err(r.eps.cpu().numpy(), eps_ref) < 1e-9
+    # the strain form at the same C0 reaches the same field.  GMRES, not
+    # CG: with the filtered operator on an odd grid the strain form is not
+    # self-adjoint enough for CG (33^2, g0 = 0.9: residual 2.4e-5 after 500
+    # iterations in the oracle too), while GMRES converges
+    s = strain.solve_gmres(E, tol=1e-12, max_iter=500, restart=60)
+    assert s.status == 'converged'
+    assert rel_err(r.eps.cpu().numpy(), s.eps.cpu().numpy()) < 1e-8
+
+
+def test_janus_example_configuration():
+    """examples/square_inclusion.py as written: 256^2, truncated, matrix
+    (1, 0.3), inclusion (10, 0.2), C0 (0.9, 0.3), a = 0.5, E = (0, 0, 1),
+    rtol 1e-10: tau from the polarization form, eps = tau2eps(tau), equal
+    to the strain-form solution; the mean stress gives the same C_1212."""
+    pol, strain, phase, inv, dc = setup(2, 256, 'truncated', 0.9,
+                                        centered=False)
+    E = np.array([0., 0., 1.])
+    tau, r = pol.solve_cg_polarization(E, tol=1e-10, max_iter=1000)
+    assert r.status == 'converged'
+    s = strain.solve_cg(E, tol=1e-12, max_iter=1000)
+    assert s.status == 'converged'
+    eps = r.eps.cpu().numpy()
+    assert rel_err(eps, s.eps.cpu().numpy()) < 1e-8
+    # tau2eps applied by the janus-style object array gives the same eps
+    ops = np.empty((256, 256), dtype=object)
+    ops[...] = isotropic_4(inv[0, 0], inv[0, 1], 2)
+    imax = int(np.ceil(256 * 0.5 - 0.5))
+    ops[:imax, :imax] = isotropic_4(inv[1, 0], inv[1, 1], 2)
+    tau2eps = BlockDiagonalOperator2D(ops)
+    assert rel_err(np.asarray(tau2eps.apply(tau.cpu().numpy())), eps) < 1e-14
+
+
+def test_polarization_cg_refuses_nonsymmetric_local():
+    rng = np.random.default_rng(4)
+    shape = (16, 16)
+    mat0 = material.create(2.0, 0.3, 2)
+    mats = rng.uniform(-0.5, 0.5, size=(2, 3, 3))
+    phase = rng.integers(0, 2, size=shape)
+    local = BlockDiagonalOperator2D.from_phase_matrices(phase, mats)
+    green = truncated(mat0.green_operator(), shape, 1.0,
+                      fft.create_real(shape))
+    S = LSSolver(green, local)
+    with pytest.raises(Exception, match='symmetric'):
+        S.solve_cg_polarization(np.array([1., 0., 0.]), tol=1e-8)

@@ -557,6 +557,67 @@ def test_gmres_equals_cg_solution(kind):
     np.testing.assert_allclose(b, a, rtol=0, atol=1e-11 * np.abs(a).max())
 
 
+def _inverse_iso(g, nu, g0, nu0, dim):
+    """(sph, dev) of (C - C0)^-1 for isotropic C, C0: isotropic_4(1 / (dim
+    dk), 1 / (2 dg)) as examples/square_inclusion.py:24-29 builds tau2eps
+    (dim = 2 there)."""
+    sph, dev = jo.delta_c_coefficients(g, nu, g0, nu0, dim)
+    return 1. / sph, 1. / dev
+
+
+def test_cg_polarization_matches_dense_direct_solve():
+    g0 = 0.9
+    from janus_amd import microstructure as ms
+    phase = ms.square_inclusion(4, 0.5)
+    t = np.array([_inverse_iso(g, nu, g0, 0.3, 2)
+                  for g, nu in ((1.0, 0.3), (10.0, 0.2))])
+    sph, dev = t[phase, 0], t[phase, 1]
+    local = lambda v: jo.iso4_apply(sph, dev, v)  # noqa: E731
+    n = 4 * 4 * 3
+    cols = []
+    for i in range(n):
+        v = np.zeros(n)
+        v[i] = 1.0
+        v = v.reshape((4, 4, 3))
+        cols.append((local(v) + jo.green_apply(v, 'truncated', g0,
+                                               0.3)).ravel())
+    A = np.array(cols).T
+    E = np.array([0.2, 0.5, -1.0])
+    b = np.empty((4, 4, 3))
+    b[...] = E
+    direct = np.linalg.solve(A, b.ravel()).reshape(b.shape)
+    tau, eps, res = jo.cg_polarization_scheme(sph, dev, E, 'truncated', g0,
+                                              0.3, tol=1e-14, max_iter=200)
+    np.testing.assert_allclose(tau, direct, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(eps, local(direct), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize('kind', ['truncated', 'filtered'])
+def test_cg_polarization_equals_strain_form(kind):
+    """janus's example configuration (square inclusion, matrix g = 1,
+    nu = 0.3, inclusion g = 10, nu = 0.2, C0: g0 = 0.9 below both) on a
+    32^2 grid: eps = L tau is the strain-form solution at the same C0."""
+    from janus_amd import microstructure as ms
+    g0, n = 0.9, 32
+    phase = ms.square_inclusion(n, 0.5)
+    mats = ((1.0, 0.3), (10.0, 0.2))
+    inv = np.array([_inverse_iso(g, nu, g0, 0.3, 2) for g, nu in mats])
+    dc = np.array([jo.delta_c_coefficients(g, nu, g0, 0.3, 2)
+                   for g, nu in mats])
+    E = np.array([0., 0., 1.])
+    tau, eps, res = jo.cg_polarization_scheme(inv[phase, 0], inv[phase, 1],
+                                              E, kind, g0, 0.3, tol=1e-12,
+                                              max_iter=500)
+    assert res[-1] < 1e-12
+    ref, _ = jo.cg_scheme(dc[phase, 0], dc[phase, 1], E, kind, g0, 0.3,
+                          tol=1e-13, max_iter=500)
+    np.testing.assert_allclose(eps, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+    # and tau = dC : eps
+    np.testing.assert_allclose(tau, jo.iso4_apply(dc[phase, 0], dc[phase, 1],
+                                                  eps),
+                               rtol=0, atol=1e-12 * np.abs(tau).max())
+
+
 def test_c0_inner_matches_matrix():
     rng = np.random.default_rng(3)
     for dim, sym in ((2, 3), (3, 6)):
