@@ -464,6 +464,41 @@ def workload_plain2d(args):
             np.abs(tau_ref).max())}
     _log('square_inclusion example: %s after %d iterations, %.4f s'
          % (rpol.status, rpol.iterations, t))
+    # examples/square_inclusion_scipy.py as written: 1024^2, filtered, a
+    # pore (g = 0, nu = 0.2) in the corner, C0 = matrix (1, 0.3), that
+    # file's local operator isotropic_4((k - k0) / 2, (g - g0) / 2), GMRES
+    # with scipy's defaults (restart 20, rtol 1e-5); it prints the mean
+    # strains
+    import torch
+    from janus_amd.green import filtered
+    m_i = material.create(0.0, 0.2, 2)
+    m_m = material.create(1.0, 0.3, 2)
+    m_0 = material.create(1.0, 0.3, 2)
+    ns = 1024
+    pore = ms.square_inclusion(ns, 0.5, centered=False)
+    tab = np.array([[(m.k - m_0.k) / 2., (m.g - m_0.g) / 2.]
+                    for m in (m_m, m_i)])
+    gm = LSSolver(filtered(m_0.green_operator(), (ns, ns), 1.0,
+                           fft.create_real((ns, ns))),
+                  BlockDiagonalOperator2D.from_phases(pore, tab))
+    gm.solve_gmres(E, tol=1e-5, max_iter=3)  # warm-up
+    _sync()
+    t0 = time.perf_counter()
+    rg = gm.solve_gmres(E, tol=1e-5, max_iter=2000, restart=20)
+    _sync()
+    t = time.perf_counter() - t0
+    eps = rg.eps
+    lhs = eps + torch.as_tensor(gm.green.apply(gm.local.apply(eps)))
+    Ef = torch.tensor(E, dtype=torch.float64, device=eps.device)
+    true_res = float(torch.linalg.vector_norm(lhs - Ef) /
+                     (ns * np.linalg.norm(E)))
+    means = eps.reshape(-1, 3).mean(dim=0).cpu().numpy()
+    out['janus square_inclusion_scipy example (GMRES)'] = {
+        'grid': [ns, ns], 'status': rg.status, 'iterations': rg.iterations,
+        'seconds': round(t, 4), 'true_relative_residual': true_res,
+        'avg_eps': [float(v) for v in means]}
+    _log('square_inclusion_scipy example: %s after %d iterations, %.3f s, '
+         'true residual %.2e' % (rg.status, rg.iterations, t, true_res))
     _line('2D 256^2 basic scheme (square inclusion, contrast 10)',
           out['C0=midpoint']['it_per_s'], 'iterations/s',
           {'workload': '2D 256^2 centred square vf 0.25, mu 1/10 nu 0.3, '
