@@ -24,6 +24,8 @@ Other workloads (one JSON line each, --workload NAME):
   adjoint    128^3 SIMP density: adjoint gradient time / forward solve time
   large      512^3 three-phase ellipsoids: 6-load batch iterations/s and
              an adjoint gradient with respect to the per-phase mu
+  fft        janus's fftw_python_benchmark_serial: r2c ms per call at
+             128^3, 256^3, 512^3 (device tensors and numpy arrays)
 """
 
 import argparse
@@ -1109,9 +1111,62 @@ def workload_large(args):
                       np.abs(grad_es - grad).max() / np.abs(grad).max())}})
 
 
+def workload_fft(args):
+    """examples/fftw_python_benchmark_serial.py: r2c of a uniform(-1, 1)
+    field (seed 20140121), mean and standard deviation of the time per call
+    in ms, at 128^3, 256^3 and 512^3 -- on device tensors (data resident,
+    each call bracketed by synchronizations as the script brackets its
+    calls with perf_counter) and on numpy arrays (copied in and out through
+    PCIe, as the janus API is called).  The script asks for 15000 / 10000 /
+    1000 calls; fewer are timed here."""
+    import torch
+    import janus_amd.fft.serial as fft
+    np.random.seed(20140121)
+    rows = {}
+    for shape, nd, nh in (((128,) * 3, 200, 20), ((256,) * 3, 100, 5),
+                          ((512,) * 3, 20, 2)):
+        t = fft.create_real(shape)
+        r = np.random.uniform(-1., 1., t.ishape)
+        rd = torch.from_numpy(r).cuda()
+        cd = torch.empty(t.oshape, dtype=torch.float64, device='cuda')
+        t.r2c(rd, cd)
+        times = []
+        for _ in range(nd):
+            _sync()
+            t0 = time.perf_counter()
+            t.r2c(rd, cd)
+            _sync()
+            times.append(1e3 * (time.perf_counter() - t0))
+        c = np.empty(t.oshape)
+        t.r2c(r, c)
+        host = []
+        for _ in range(nh):
+            t0 = time.perf_counter()
+            t.r2c(r, c)
+            host.append(1e3 * (time.perf_counter() - t0))
+        nbytes = 8 * (r.size + int(np.prod(t.oshape)))
+        rows['%d^3' % shape[0]] = {
+            'device_ms_mean': round(float(np.mean(times)), 4),
+            'device_ms_std': round(float(np.std(times)), 4),
+            'device_GBps_in_plus_out': round(
+                nbytes / (np.mean(times) * 1e-3) / 1e9, 1),
+            'host_arrays_ms_mean': round(float(np.mean(host)), 2),
+            'host_arrays_ms_std': round(float(np.std(host)), 2),
+            'calls': [nd, nh]}
+        _log('r2c %s: device %.3f ms, numpy arrays %.1f ms'
+             % (shape, np.mean(times), np.mean(host)))
+        del rd, cd
+        torch.cuda.empty_cache()
+    _line('r2c ms per call at 256^3 (janus fftw_python_benchmark_serial)',
+          rows['256^3']['device_ms_mean'], 'ms',
+          {'workload': 'r2c of uniform(-1, 1) fields, seed 20140121, '
+                       '128^3 / 256^3 / 512^3', 'shapes': rows},
+          higher=False)
+
+
 WORKLOADS = {'plain2d': workload_plain2d, 'stiffness': workload_stiffness,
              'batched': workload_batched, 'adjoint': workload_adjoint,
-             'large': workload_large}
+             'large': workload_large, 'fft': workload_fft}
 
 
 def main():
