@@ -332,3 +332,29 @@ def test_empty_grids_are_rejected(shape):
     for factory in (truncated, filtered):
         with pytest.raises(ValueError):
             factory(g, shape, 1.0)
+
+
+def test_example_filtered_green_operator():
+    """examples/example_filtered_green_operator.py: the class constructed
+    directly (no transform), the per-frequency API over every frequency of
+    the grid (the example takes 256 x 256; 32 x 48 here), eta[0] for
+    tau = (0, 0, 1), against the oracle's filtered Gamma-hat."""
+    import janus_amd.green as green
+    mat = material.create(1.0, 0.3, 2)
+    n0, n1 = 32, 48
+    greend = green.FilteredGreenOperator2D(mat.green_operator(), (n0, n1), 1.)
+    g = np.empty((n0, n1))
+    b = np.empty((2,), dtype=np.intc)
+    tau = np.array([0., 0., 1.])
+    eta = np.empty_like(tau)
+    for b0 in range(n0):
+        b[0] = b0
+        for b1 in range(n1):
+            b[1] = b1
+            greend.set_frequency(b)
+            greend.apply_by_freq(tau, eta)
+            g[b0, b1] = eta[0]
+    bs = np.stack(np.meshgrid(np.arange(n0), np.arange(n1), indexing='ij'),
+                  axis=-1).reshape(-1, 2)
+    ref = jo.gamma_hat('filtered', bs, (n0, n1), 1.0, 1.0, 0.3)[:, 0, 2]
+    np.testing.assert_allclose(g.ravel(), ref, rtol=ULP, atol=ULP)
