@@ -314,6 +314,13 @@ int jx_batch_stats(jx_batch *batch, double *ms, int *replays, int *captures);
 int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
                         const double *mu, double *dsph, double *ddev,
                         void *stream);
+/* Adjoint contraction for a per-voxel matrix local operator a (no janus
+ * counterpart): with eps and mu as in jx_adjoint_contract, except that mu
+ * solves the adjoint fixed point with the transposed matrices, mu <- G g -
+ * G[a^T : mu], the gradient is da[v][i][j] = -mu[v][i] eps[v][j].  eps, mu:
+ * (n0, n1[, n2], sym); da: (n0, n1[, n2], sym, sym) device arrays. */
+int jx_adjoint_outer(int dim, const int64_t *shape, const double *eps,
+                     const double *mu, double *da, void *stream);
 
 /* Measurement helper (no janus counterpart): device-to-device copy of
  * `bytes` (a multiple of 16; 16-byte aligned pointers) with one 16-byte load

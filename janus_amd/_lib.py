@@ -73,6 +73,8 @@ SIGNATURES = [
       _VP, _VP, _VP, _VP]),
     ('jx_adjoint_contract', ctypes.c_int,
      [ctypes.c_int, _c_int64_p, _VP, _VP, _VP, _VP, _VP]),
+    ('jx_adjoint_outer', ctypes.c_int,
+     [ctypes.c_int, _c_int64_p, _VP, _VP, _VP, _VP]),
     ('jx_stream_copy', ctypes.c_int, [_VP, _VP, ctypes.c_int64, _VP]),
     ('jx_solver_set_fused', ctypes.c_int, [_VP, ctypes.c_int]),
     ('jx_solver_get_fused', ctypes.c_int, [_VP]),

@@ -58,7 +58,7 @@ def adjoint_solve(green, local, g, tol, max_iter, solver=None,
     """mu solving mu = G g - G[local : mu]; returns (mu, SolveResult).
 
     ``solver``: an LSSolver on (green, local) to reuse (e.g. the forward
-    solve's); ``method``: 'basic' (fixed point) or 'cg'."""
+    solve's); ``method``: 'basic' (fixed point), 'cg' or 'gmres'."""
     t = _torch()
     gg = green.apply(g.contiguous())
     norm = float(t.linalg.vector_norm(gg))
@@ -70,6 +70,10 @@ def adjoint_solve(green, local, g, tol, max_iter, solver=None,
     if method == 'cg':
         r = solver.solve_cg(zero, tol=tol, max_iter=max_iter, rhs=gg,
                             res_scale=1.0 / norm)
+        return r.eps, r
+    if method == 'gmres':
+        r = solver.solve_gmres(zero, tol=tol, max_iter=max_iter, rhs=gg,
+                               res_scale=1.0 / norm)
         return r.eps, r
     mu = gg.clone()
     r = solver.solve(zero, tol=tol, max_iter=max_iter, eps=mu, rhs=gg,
@@ -219,6 +223,8 @@ def _make_effective_stress():
 def _solve(solver, E, tol, max_iter, method):
     if method == 'cg':
         return solver.solve_cg(E, tol=tol, max_iter=max_iter)
+    if method == 'gmres':
+        return solver.solve_gmres(E, tol=tol, max_iter=max_iter)
     return solver.solve(E, tol=tol, max_iter=max_iter)
 
 
@@ -248,6 +254,100 @@ def effective_stress(sph, dev, E, green, mat0, tol=1e-10, max_iter=1000,
         _EffectiveStressFunction.fn = _make_effective_stress()
     return _EffectiveStressFunction.fn.apply(sph, dev, E, green, mat0, tol,
                                              max_iter, info, method)
+
+
+def outer(eps, mu):
+    """da = -mu (x) eps voxel by voxel (GPU): the gradient with respect to a
+    per-voxel matrix local operator, shape grid + (sym, sym)."""
+    t = _torch()
+    grid = tuple(eps.shape[:-1])
+    sym = eps.shape[-1]
+    da = t.empty(grid + (sym, sym), dtype=t.float64, device=eps.device)
+    _lib.check(_lib.lib().jx_adjoint_outer(
+        len(grid), _lib.shape_array(grid), eps.data_ptr(), mu.data_ptr(),
+        da.data_ptr(), _lib.current_stream()))
+    return da
+
+
+class _LSSolveMatrixFunction:
+    fn = None
+
+
+def _make_matrix_function():
+    t = _torch()
+
+    class LSSolveMatrixFunction(t.autograd.Function):
+        @staticmethod
+        def forward(ctx, mats, E, green, tol, max_iter, info, method):
+            mats = mats.detach().contiguous()
+            cls = (BlockDiagonalOperator2D if mats.dim() == 4
+                   else BlockDiagonalOperator3D)
+            local = cls.from_matrix_field(mats)
+            solver = LSSolver(green, local)
+            r = _solve(solver, E.detach().cpu().numpy(), tol, max_iter,
+                       method)
+            if info is not None:
+                info['forward'] = r
+            if local.symmetric:
+                local_t, solver_t = local, solver
+            else:
+                # the adjoint fixed point runs on the transposed matrices;
+                # CG needs symmetry, so it falls back to GMRES there
+                mats_t = mats.transpose(-1, -2).contiguous()
+                local_t = cls.from_matrix_field(mats_t)
+                solver_t = LSSolver(green, local_t)
+                ctx.mats_t = mats_t
+            ctx.save_for_backward(r.eps)
+            ctx.green = green
+            ctx.local_t = local_t
+            ctx.solver_t = solver_t
+            ctx.args = (tol, max_iter, info,
+                        'gmres' if method == 'cg' and not local.symmetric
+                        else method)
+            return r.eps
+
+        @staticmethod
+        def backward(ctx, g):
+            (eps,) = ctx.saved_tensors
+            tol, max_iter, info, method = ctx.args
+            mu, r = adjoint_solve(ctx.green, ctx.local_t, g, tol, max_iter,
+                                  solver=ctx.solver_t, method=method)
+            if info is not None:
+                info['adjoint'] = r
+            da = outer(eps, mu)
+            lam = g.contiguous() - ctx.local_t.apply(mu)
+            dE = lam.sum(dim=tuple(range(lam.dim() - 1)))
+            return da, dE, None, None, None, None, None
+
+    return LSSolveMatrixFunction
+
+
+def ls_solve_matrix(mats, E, green, tol=1e-10, max_iter=1000, info=None,
+                    method='basic'):
+    """Differentiable Lippmann-Schwinger solve with a per-voxel matrix local
+    operator (BlockDiagonalLinearOperator, operators.pyx:719-890).
+
+    F(eps, A) = eps + G[A eps] - E = 0 with G = Gamma0.  For a loss L(eps)
+    with g = dL/deps, the adjoint state solves (I + A^T G) lambda = g, so
+    mu := G lambda solves the fixed point mu = G g - G[A^T mu] (the forward
+    one with the transposed matrices and the field right-hand side G g);
+    then dL/dA_x = -mu_x (x) eps_x (jx_adjoint_outer) and dL/dE =
+    sum_x (g_x - A_x^T mu_x).
+
+    Args:
+        mats: float64 GPU tensor grid + (sym, sym) (may require grad); need
+            not be symmetric.
+        E, green, tol, max_iter, info: as ls_solve.
+        method: 'basic', 'cg' (symmetric matrices; the adjoint of a
+            non-symmetric field uses GMRES) or 'gmres'.
+    Returns:
+        eps, the converged strain field."""
+    if method not in ('basic', 'cg', 'gmres'):
+        raise ValueError("method must be 'basic', 'cg' or 'gmres'")
+    if _LSSolveMatrixFunction.fn is None:
+        _LSSolveMatrixFunction.fn = _make_matrix_function()
+    return _LSSolveMatrixFunction.fn.apply(mats, E, green, tol, max_iter,
+                                           info, method)
 
 
 def iso_delta(g, nu, mat0, dim):

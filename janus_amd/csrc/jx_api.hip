@@ -860,6 +860,17 @@ int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
                                  (hipStream_t)stream);
 }
 
+int jx_adjoint_outer(int dim, const int64_t *shape, const double *eps,
+                     const double *mu, double *da, void *stream) {
+  int rc = check_shape(dim, shape);
+  if (rc) return rc;
+  if (!eps || !mu || !da)
+    return fail(JX_EINVAL, "jx_adjoint_outer: NULL argument");
+  Grid g = make_grid(dim, shape);
+  return launch_adjoint_outer(g, dim == 2 ? 3 : 6, eps, mu, da,
+                              (hipStream_t)stream);
+}
+
 int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
                       double *ms_out, void *stream) {
   if (!S || !eps || !E || !ms_out || iters < 1)

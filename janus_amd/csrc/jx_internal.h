@@ -229,6 +229,8 @@ int launch_block_linear(int64_t N, int m, int n, const double *a,
 int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,
                             const double *mu, double *dsph, double *ddev,
                             hipStream_t s);
+int launch_adjoint_outer(const Grid &g, int ncomp, const double *eps,
+                         const double *mu, double *da, hipStream_t s);
 int launch_local_sums(const Grid &g, int ncomp, const LocalDev &loc,
                       const double *x, double *partials, int *nblocks,
                       hipStream_t s);

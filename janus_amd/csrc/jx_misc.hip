@@ -53,6 +53,28 @@ __global__ void __launch_bounds__(kFinalizeThreads)
   }
 }
 
+// Gradient with respect to a per-voxel matrix field: da[v][i][j] =
+// -mu[v][i] eps[v][j] (row-major sym x sym blocks).
+template <int C>
+__global__ void __launch_bounds__(kThreads)
+    k_adjoint_outer(const double *eps, const double *mu, int64_t N,
+                    double *da) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double e[C], m[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      e[c] = eps[v * C + c];
+      m[c] = mu[v * C + c];
+    }
+    double *dv = da + v * (C * C);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+#pragma unroll
+      for (int j = 0; j < C; ++j) dv[i * C + j] = -(m[i] * e[j]);
+  }
+}
+
 // ------------------------------------------------------------ local apply
 // TRANSPOSE applies the transposed local operator; the isotropic modes are
 // symmetric, so only the matrix modes differ.
@@ -278,6 +300,21 @@ int launch_adjoint_contract(const Grid &g, int ncomp, const double *eps,
                        dim3(kThreads), 0, s, eps, mu, g.N, dsph, ddev);
   else
     return fail(JX_EINVAL, "adjoint_contract: ncomp must be 3 or 6");
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int launch_adjoint_outer(const Grid &g, int ncomp, const double *eps,
+                         const double *mu, double *da, hipStream_t s) {
+  int64_t nb = std::min<int64_t>((g.N + kThreads - 1) / kThreads, 65536);
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_adjoint_outer<3>, dim3((unsigned)nb), dim3(kThreads),
+                       0, s, eps, mu, g.N, da);
+  else if (ncomp == 6)
+    hipLaunchKernelGGL(k_adjoint_outer<6>, dim3((unsigned)nb), dim3(kThreads),
+                       0, s, eps, mu, g.N, da);
+  else
+    return fail(JX_EINVAL, "adjoint_outer: ncomp must be 3 or 6");
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
