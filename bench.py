@@ -1228,8 +1228,8 @@ def main():
               end="")
         return
     if args.gpus != 1:
-        print('bench.py: this build runs on one GPU only; the multi-GPU '
-              'slab decomposition is design-only (--gpus %d refused)'
+        print('bench.py: janus_amd runs on one GPU; the multi-GPU slab '
+              'decomposition is not implemented (--gpus %d refused)'
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
