@@ -689,7 +689,8 @@ def workload_batched(args):
     _log('sequential: %.3f s, %d iterations' % (t_seq,
                                                 sum(x[0] for x in seq)))
     eps_b = fields()
-    batch = BatchSolver(iters_per_launch=args.iters_per_launch, graph=True,
+    batch = BatchSolver(iters_per_launch=args.iters_per_launch,
+                        graph=args.graph,
                         autotune=True)
     for (i, j), e in zip(rhs, eps_b):
         batch.add(solvers[i], _unit(6, j), tol=1e-10, max_iter=args.max_iter,
@@ -719,7 +720,7 @@ def workload_batched(args):
                 _sync()
                 t_s += time.perf_counter() - t0
             bg = BatchSolver(iters_per_launch=args.iters_per_launch,
-                             graph=True, autotune=True)
+                             graph=args.graph, autotune=True)
             for k in idx:
                 i, j = rhs[k]
                 eps_seq[k][...] = torch.from_numpy(_unit(6, j)).cuda()
@@ -1200,6 +1201,9 @@ def main():
     ap.add_argument('--batch-count', type=int, default=32,
                     help='microstructures of the batched workload')
     ap.add_argument('--iters-per-launch', type=int, default=8)
+    ap.add_argument('--graph', action='store_true',
+                    help='batched workload: replay captured hipGraphs '
+                         '(default: enqueue the launches, 2.5%% faster here)')
     ap.add_argument('--gmres-restart', type=int, default=20,
                     help='Arnoldi steps per GMRES cycle (stiffness workload)')
     ap.add_argument('--per-group', action='store_true',
