@@ -283,6 +283,9 @@ def _make_matrix_function():
             cls = (BlockDiagonalOperator2D if mats.dim() == 4
                    else BlockDiagonalOperator3D)
             local = cls.from_matrix_field(mats)
+            if method == 'cg' and not local.symmetric:
+                raise ValueError("method 'cg' needs symmetric matrices "
+                                 "(use 'gmres' or 'basic')")
             solver = LSSolver(green, local)
             r = _solve(solver, E.detach().cpu().numpy(), tol, max_iter,
                        method)
@@ -291,8 +294,7 @@ def _make_matrix_function():
             if local.symmetric:
                 local_t, solver_t = local, solver
             else:
-                # the adjoint fixed point runs on the transposed matrices;
-                # CG needs symmetry, so it falls back to GMRES there
+                # the adjoint fixed point runs on the transposed matrices
                 mats_t = mats.transpose(-1, -2).contiguous()
                 local_t = cls.from_matrix_field(mats_t)
                 solver_t = LSSolver(green, local_t)
@@ -301,9 +303,7 @@ def _make_matrix_function():
             ctx.green = green
             ctx.local_t = local_t
             ctx.solver_t = solver_t
-            ctx.args = (tol, max_iter, info,
-                        'gmres' if method == 'cg' and not local.symmetric
-                        else method)
+            ctx.args = (tol, max_iter, info, method)
             return r.eps
 
         @staticmethod
@@ -338,8 +338,8 @@ def ls_solve_matrix(mats, E, green, tol=1e-10, max_iter=1000, info=None,
         mats: float64 GPU tensor grid + (sym, sym) (may require grad); need
             not be symmetric.
         E, green, tol, max_iter, info: as ls_solve.
-        method: 'basic', 'cg' (symmetric matrices; the adjoint of a
-            non-symmetric field uses GMRES) or 'gmres'.
+        method: 'basic', 'cg' (symmetric matrices only) or 'gmres', for
+            the forward and the adjoint solve.
     Returns:
         eps, the converged strain field."""
     if method not in ('basic', 'cg', 'gmres'):

@@ -119,3 +119,12 @@ def test_matrix_gradient_matches_isotropic_adjoint():
                                atol=1e-9 * scale)
     np.testing.assert_allclose(ddev, d.grad.cpu().numpy(), rtol=0,
                                atol=1e-9 * scale)
+
+
+def test_cg_refuses_nonsymmetric_matrix_field():
+    import torch
+    field, green, mat0, rng = setup(2, 8, 'truncated', False, seed=3)
+    A = torch.from_numpy(field).cuda().requires_grad_()
+    E = torch.tensor([1., 0., 0.], device='cuda')
+    with pytest.raises(ValueError, match='symmetric'):
+        ls_solve_matrix(A, E, green, method='cg')
