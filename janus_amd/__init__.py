@@ -16,8 +16,11 @@ janus.material.elastic.linear          janus_amd.material.elastic.linear
 janus.mandelvoigt                      janus_amd.mandelvoigt
 =====================================  ========================================
 
-Additions: :mod:`janus_amd.solver` (the fused on-device fixed-point loop),
-:mod:`janus_amd.autograd` (implicit-differentiation adjoint) and
+Additions: :mod:`janus_amd.solver` (the fused on-device fixed-point loop,
+conjugate gradients on the strain and polarization forms, restarted GMRES,
+plan autotuning), :mod:`janus_amd.batch` (many right-hand sides in one
+sequence of launches), :mod:`janus_amd.autograd` (implicit-differentiation
+adjoint for isotropic and matrix local operators, effective stress) and
 :mod:`janus_amd.microstructure` (seeded test microstructures).
 All compute runs in ``libjanus_amd.so`` (hand-written HIP for gfx950).
 """
