@@ -49,7 +49,8 @@ namespace {
 enum CgMode {
   CG_FILL, CG_INIT, CG_QP, CG_XR, CG_P, CG_FINAL, CG_R, CG_FINAL_P,
   CG_INIT_E,  // polarization form: r = p = E, x = 0
-  CG_QL       // polarization form: q = G[p] + L p
+  CG_QL,      // polarization form: q = G[p] + L p
+  CG_STEP_X   // x += alpha p (the fused iterations' last step)
 };
 enum CgPhase { PH_INIT, PH_ALPHA, PH_BETA };
 
@@ -68,12 +69,15 @@ __global__ void __launch_bounds__(kThreads)
              double *partials, const double *rhs, LocalDev loc) {
 #pragma clang fp contract(off)
   // the fill and the final sum run whatever the state
-  constexpr bool FINAL = MODE == CG_FINAL || MODE == CG_FINAL_P;
+  constexpr bool FINAL =
+      MODE == CG_FINAL || MODE == CG_FINAL_P || MODE == CG_STEP_X;
   if (MODE != CG_FILL && !FINAL && solve_done(st)) return;
   __shared__ double red[kThreads / 64];
   double acc0 = 0., acc1 = 0.;
-  const double alpha =
-      (MODE == CG_XR || MODE == CG_R || MODE == CG_FINAL_P) ? cs->alpha : 0.;
+  const double alpha = (MODE == CG_XR || MODE == CG_R || MODE == CG_FINAL_P ||
+                        MODE == CG_STEP_X)
+                           ? cs->alpha
+                           : 0.;
   const double beta = MODE == CG_P ? cs->beta : 0.;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
        v += (int64_t)gridDim.x * blockDim.x) {
@@ -144,6 +148,16 @@ __global__ void __launch_bounds__(kThreads)
       acc0 += c0_dot<C>(e, e, wd, wt);
 #pragma unroll
       for (int c = 0; c < C; ++c) acc1 += e[c] * e[c];
+    } else if constexpr (MODE == CG_STEP_X) {
+      double pv[C];
+      load_voxel<C>(x, v, e);
+      load_voxel<C>(p, v, pv);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        double ap = alpha * pv[c];
+        e[c] = e[c] + ap;
+      }
+      store_voxel<C>(x, v, e);
     } else if constexpr (MODE == CG_QL) {
       // q = G[p] + L p (the local operator applied voxel by voxel)
       double pv[C], qv[C], lv[C];
@@ -401,6 +415,46 @@ int cg_pol_iteration(jx_solver *S, double *x, hipStream_t s) {
                       s);
 }
 
+// The same iteration with the direction and iterate updates folded into
+// pass A (register lengths): pass A sets p <- r + beta p and x <- x +
+// alpha p (the previous p) and transforms p; then q = G[p] (plain pass E),
+// q += L p with the sums p . q, alpha, r -= alpha q with its sums, beta.
+// The last step of x is applied after the loop (CG_STEP_X).
+template <int C>
+int cg_pol_iteration_fused(jx_solver *S, double *x, hipStream_t s) {
+  jx_green *G = S->green;
+  const Grid &g = G->g;
+  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
+  CgDev cg;
+  cg.r = r;
+  cg.p = p;
+  cg.x = x;
+  cg.cs = S->cg_s;
+  const LocalDev none;
+  int rc = launch_rows_forward_cg(g, C, cg, G->spec, none, S->st, s);
+  if (rc) return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s)))
+    return rc;
+  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s)))
+    return rc;
+  RowsInverseLs plain;
+  if ((rc = launch_rows_inverse(g, C, G->spec, q, plain, S->st, nullptr, s)))
+    return rc;
+  if ((rc = vec<C, CG_QL>(x, r, p, q, g, 1., 0., S->cg_s, S->st,
+                          S->cg_partials, s, nullptr, S->local->dev)))
+    return rc;
+  if ((rc = scalar<PH_ALPHA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+    return rc;
+  if ((rc = vec<C, CG_R>(x, r, p, q, g, 1., 0., S->cg_s, S->st,
+                         S->cg_partials, s)))
+    return rc;
+  return scalar<PH_BETA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s);
+}
+
 template <int C>
 int cg_pol_solve(jx_solver *S, double *tau, double *eps, int max_iter,
                  hipStream_t s) {
@@ -412,11 +466,14 @@ int cg_pol_solve(jx_solver *S, double *tau, double *eps, int max_iter,
     return rc;
   if ((rc = scalar<PH_INIT>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
     return rc;
+  const bool fused = JX_CG_FUSED && rows_cg_fusable(g, C);
   int enqueued = 0, chunk = 4;
   while (true) {
     int n = std::min(chunk, max_iter - enqueued);
     for (int i = 0; i < n; ++i)
-      if ((rc = cg_pol_iteration<C>(S, tau, s))) return rc;
+      if ((rc = fused ? cg_pol_iteration_fused<C>(S, tau, s)
+                      : cg_pol_iteration<C>(S, tau, s)))
+        return rc;
     enqueued += n;
     JX_HIP(hipMemcpyAsync(S->st_host, S->st, sizeof(SolveState),
                           hipMemcpyDeviceToHost, s));
@@ -424,7 +481,10 @@ int cg_pol_solve(jx_solver *S, double *tau, double *eps, int max_iter,
     if (S->st_host->done || enqueued >= max_iter) break;
     chunk = std::min(chunk * 2, 64);
   }
-  // the fused-free iteration leaves tau complete; eps = L tau
+  // the fused iterations leave the last step of tau to here; eps = L tau
+  if (fused && (rc = vec<C, CG_STEP_X>(tau, r, p, q, g, 1., 0., S->cg_s,
+                                       S->st, S->cg_partials, s)))
+    return rc;
   return launch_local_apply(g, C, S->local->dev, tau, eps, s);
 }
 

@@ -300,7 +300,9 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
   // family: the first slot's, which the caller passes as loc)
   if (cg) {
     if constexpr (C == 3 || C == 6) {
-      if (local_family(loc) == LF_MATRIX)
+      if (loc.mode == LOCAL_NONE)  // polarization form: transform p itself
+        go(k_rows_forward_reg<P, C, NP, 0, true>);
+      else if (local_family(loc) == LF_MATRIX)
         go(k_rows_forward_reg<P, C, NP, 1 + LF_MATRIX, true>);
       else
         go(k_rows_forward_reg<P, C, NP, 1 + LF_ISO, true>);
