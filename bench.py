@@ -279,7 +279,8 @@ def stats_summary(stats_csv, cmd, title='256^3 filtered LS iteration'):
     return '\n'.join(out) + '\n'
 
 
-def traffic_summary(fetch_csv, write_csv, cmd):
+def traffic_summary(fetch_csv, write_csv, cmd,
+                    title='256^3 filtered LS iteration'):
     """Markdown summary of rocprofv3 FETCH_SIZE / WRITE_SIZE runs: each
     counter is scaled by what it reads for the stream-copy kernel, whose
     bytes are known (16 B per grid thread)."""
@@ -298,7 +299,7 @@ def traffic_summary(fetch_csv, write_csv, cmd):
                 ratios.append(int(r['Grid_Size']) * 16 / v)
         vals[counter] = {k: sum(v) / len(v) for k, v in agg.items()}
         copies[counter] = sum(ratios) / len(ratios)
-    out = ['# HBM traffic per launch, 256^3 filtered LS iteration', '',
+    out = ['# HBM traffic per launch, ' + title, '',
            'Commands (one counter per run):', '',
            '    ' + cmd.replace('COUNTER', 'FETCH_SIZE'),
            '    ' + cmd.replace('COUNTER', 'WRITE_SIZE'), '',
@@ -1225,7 +1226,8 @@ def main():
                          'runs (no GPU work)')
     args = ap.parse_args()
     if args.traffic_summary:
-        print(traffic_summary(*args.traffic_summary), end='')
+        print(traffic_summary(*args.traffic_summary,
+                              title=args.summary_title), end='')
         return
     if args.stats_summary:
         print(stats_summary(*args.stats_summary, title=args.summary_title),
