@@ -56,6 +56,17 @@ def to_host_into(dev, out=None):
     return out.view()
 
 
+def require_f64(a, name='array'):
+    """Raise ValueError unless the torch tensor ``a`` holds float64: the
+    kernels read and write 8 bytes per value, so a narrower tensor would be
+    overrun (janus's typed memoryviews refuse other dtypes with ValueError
+    too)."""
+    t = torch()
+    if isinstance(a, t.Tensor) and a.dtype != t.float64:
+        raise ValueError('{0} must be float64 (was {1})'.format(name, a.dtype))
+    return a
+
+
 def check_float_array(a, name='array'):
     if isinstance(a, np.ndarray) and a.dtype != np.float64:
         raise ValueError('{0} must be float64 (was {1})'.format(name, a.dtype))

@@ -14,9 +14,16 @@ L(eps) with g = dL/deps:
 
 Only the converged eps is kept from the forward pass.  The backward costs
 one Green-operator application, one fixed-point solve and one contraction.
-Both solves use the same method: the basic scheme, or conjugate gradients
-(``method='cg'``), where the right-hand side G g enters the initial
-residual only, so an adjoint iteration costs what a forward one does.
+Both solves use the same method: conjugate gradients (``method='cg'``, the
+default for the isotropic and other symmetric local operators), where the
+right-hand side G g enters the initial residual only, so an adjoint
+iteration costs what a forward one does; GMRES (the default for
+non-symmetric matrix fields); or the basic scheme (``method='basic'``),
+whose adjoint fixed point reads G g in every update.
+
+A forward or adjoint solve that stops without reaching ``tol`` (status
+'maxiter' or 'nonfinite') raises a RuntimeWarning: its field, and any
+gradient built on it, is not the converged one.
 
 For losses of the macroscopic stress only (effective moduli),
 effective_stress needs no adjoint solve: there g = dC gbar / N and mu is
@@ -24,6 +31,7 @@ the forward fluctuation under the load gbar (see its docstring).
 """
 
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -39,6 +47,30 @@ def _torch():
 def _local_from_fields(sph, dev):
     cls = BlockDiagonalOperator2D if sph.dim() == 2 else BlockDiagonalOperator3D
     return cls.from_fields(sph, dev)
+
+
+def _check_status(r, what):
+    """Warn when a solve stopped without converging (r: SolveResult or
+    None for a solve that was not needed)."""
+    if r is not None and r.status != 'converged':
+        warnings.warn('janus_amd: the {0} solve stopped with status {1!r} '
+                      'after {2} iterations (residual {3:.3e}); its result '
+                      'is not converged'.format(
+                          what, r.status, r.iterations,
+                          float(r.residuals[-1]) if len(r.residuals)
+                          else float('nan')),
+                      RuntimeWarning, stacklevel=3)
+
+
+def _resolve_method(method, symmetric, allowed):
+    """'auto' -> conjugate gradients for a symmetric local operator, GMRES
+    otherwise (GMRES only where ``allowed`` lists it)."""
+    if method == 'auto':
+        method = 'cg' if symmetric else 'gmres'
+    if method not in allowed:
+        raise ValueError('method must be one of {0}'.format(
+            ', '.join(repr(m) for m in ('auto',) + tuple(allowed))))
+    return method
 
 
 def contract(eps, mu):
@@ -70,14 +102,17 @@ def adjoint_solve(green, local, g, tol, max_iter, solver=None,
     if method == 'cg':
         r = solver.solve_cg(zero, tol=tol, max_iter=max_iter, rhs=gg,
                             res_scale=1.0 / norm)
+        _check_status(r, 'adjoint')
         return r.eps, r
     if method == 'gmres':
         r = solver.solve_gmres(zero, tol=tol, max_iter=max_iter, rhs=gg,
                                res_scale=1.0 / norm)
+        _check_status(r, 'adjoint')
         return r.eps, r
     mu = gg.clone()
     r = solver.solve(zero, tol=tol, max_iter=max_iter, eps=mu, rhs=gg,
                      res_scale=1.0 / norm)
+    _check_status(r, 'adjoint')
     return mu, r
 
 
@@ -97,12 +132,9 @@ def _make_function():
             dev = dev.detach().contiguous()
             local = _local_from_fields(sph, dev)
             solver = LSSolver(green, local)
-            if method == 'cg':
-                r = solver.solve_cg(E.detach().cpu().numpy(), tol=tol,
-                                    max_iter=max_iter)
-            else:
-                r = solver.solve(E.detach().cpu().numpy(), tol=tol,
-                                 max_iter=max_iter)
+            r = _solve(solver, E.detach().cpu().numpy(), tol, max_iter,
+                       method)
+            _check_status(r, 'forward')
             if info is not None:
                 info['forward'] = r
             ctx.save_for_backward(sph, dev, r.eps)
@@ -134,7 +166,7 @@ def _make_function():
 
 
 def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None,
-             method='basic'):
+             method='auto'):
     """Differentiable Lippmann-Schwinger solve.
 
     Args:
@@ -144,13 +176,13 @@ def ls_solve(sph, dev, E, green, tol=1e-10, max_iter=1000, info=None,
         green: a janus_amd.green operator with a transform (reference C0).
         info: optional dict receiving the SolveResult of the forward
             ('forward') and adjoint ('adjoint') solves.
-        method: 'basic' (the fixed point) or 'cg' (conjugate gradients,
-            LSSolver.solve_cg) for both the forward and the adjoint solve.
+        method: 'cg' (conjugate gradients, LSSolver.solve_cg; 'auto', the
+            default, picks it: isotropic dC is symmetric) or 'basic' (the
+            fixed point), for both the forward and the adjoint solve.
     Returns:
         eps, the converged strain field (grid + (sym,)).
     """
-    if method not in ('basic', 'cg'):
-        raise ValueError("method must be 'basic' or 'cg'")
+    method = _resolve_method(method, True, ('basic', 'cg'))
     if _LSSolveFunction.fn is None:
         _LSSolveFunction.fn = _make_function()
     return _LSSolveFunction.fn.apply(sph, dev, E, green, tol, max_iter, info,
@@ -174,6 +206,7 @@ def _make_effective_stress():
             solver = LSSolver(green, local)
             En = E.detach().cpu().numpy()
             r = _solve(solver, En, tol, max_iter, method)
+            _check_status(r, 'forward')
             if info is not None:
                 info['forward'] = r
                 info['adjoint'] = None
@@ -205,6 +238,7 @@ def _make_effective_stress():
                 eps_g, scale = eps, alpha
             else:
                 r = _solve(ctx.solver, gb, tol, max_iter, method)
+                _check_status(r, 'adjoint')
                 if info is not None:
                     info['adjoint'] = r
                 eps_g, scale = r.eps, 1.0
@@ -229,7 +263,7 @@ def _solve(solver, E, tol, max_iter, method):
 
 
 def effective_stress(sph, dev, E, green, mat0, tol=1e-10, max_iter=1000,
-                     info=None, method='basic'):
+                     info=None, method='auto'):
     """Differentiable macroscopic stress <sigma> = C0 : E + <dC : eps(E)>.
 
     The same solve as ls_solve followed by mean_stress, with a backward
@@ -248,8 +282,7 @@ def effective_stress(sph, dev, E, green, mat0, tol=1e-10, max_iter=1000,
     Args: as ls_solve, plus mat0 (the isotropic reference material of
     green).  info receives 'forward' and 'adjoint' (None when no solve was
     needed).  Returns <sigma>, a (sym,) tensor."""
-    if method not in ('basic', 'cg'):
-        raise ValueError("method must be 'basic' or 'cg'")
+    method = _resolve_method(method, True, ('basic', 'cg'))
     if _EffectiveStressFunction.fn is None:
         _EffectiveStressFunction.fn = _make_effective_stress()
     return _EffectiveStressFunction.fn.apply(sph, dev, E, green, mat0, tol,
@@ -283,12 +316,15 @@ def _make_matrix_function():
             cls = (BlockDiagonalOperator2D if mats.dim() == 4
                    else BlockDiagonalOperator3D)
             local = cls.from_matrix_field(mats)
+            method = _resolve_method(method, local.symmetric,
+                                     ('basic', 'cg', 'gmres'))
             if method == 'cg' and not local.symmetric:
                 raise ValueError("method 'cg' needs symmetric matrices "
                                  "(use 'gmres' or 'basic')")
             solver = LSSolver(green, local)
             r = _solve(solver, E.detach().cpu().numpy(), tol, max_iter,
                        method)
+            _check_status(r, 'forward')
             if info is not None:
                 info['forward'] = r
             if local.symmetric:
@@ -299,7 +335,11 @@ def _make_matrix_function():
                 local_t = cls.from_matrix_field(mats_t)
                 solver_t = LSSolver(green, local_t)
                 ctx.mats_t = mats_t
-            ctx.save_for_backward(r.eps)
+            # mats shares the caller's storage and version counter (the
+            # symmetric path's adjoint reuses the forward operator, which
+            # reads it): saving it makes autograd refuse a backward after an
+            # in-place change of the matrices
+            ctx.save_for_backward(r.eps, mats)
             ctx.green = green
             ctx.local_t = local_t
             ctx.solver_t = solver_t
@@ -308,7 +348,7 @@ def _make_matrix_function():
 
         @staticmethod
         def backward(ctx, g):
-            (eps,) = ctx.saved_tensors
+            eps, _ = ctx.saved_tensors
             tol, max_iter, info, method = ctx.args
             mu, r = adjoint_solve(ctx.green, ctx.local_t, g, tol, max_iter,
                                   solver=ctx.solver_t, method=method)
@@ -323,7 +363,7 @@ def _make_matrix_function():
 
 
 def ls_solve_matrix(mats, E, green, tol=1e-10, max_iter=1000, info=None,
-                    method='basic'):
+                    method='auto'):
     """Differentiable Lippmann-Schwinger solve with a per-voxel matrix local
     operator (BlockDiagonalLinearOperator, operators.pyx:719-890).
 
@@ -339,11 +379,12 @@ def ls_solve_matrix(mats, E, green, tol=1e-10, max_iter=1000, info=None,
             not be symmetric.
         E, green, tol, max_iter, info: as ls_solve.
         method: 'basic', 'cg' (symmetric matrices only) or 'gmres', for
-            the forward and the adjoint solve.
+            the forward and the adjoint solve; 'auto' (default): 'cg' for
+            symmetric matrices, 'gmres' otherwise.
     Returns:
         eps, the converged strain field."""
-    if method not in ('basic', 'cg', 'gmres'):
-        raise ValueError("method must be 'basic', 'cg' or 'gmres'")
+    if method not in ('auto', 'basic', 'cg', 'gmres'):
+        raise ValueError("method must be 'auto', 'basic', 'cg' or 'gmres'")
     if _LSSolveMatrixFunction.fn is None:
         _LSSolveMatrixFunction.fn = _make_matrix_function()
     return _LSSolveMatrixFunction.fn.apply(mats, E, green, tol, max_iter,

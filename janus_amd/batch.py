@@ -74,9 +74,10 @@ class BatchSolver:
                              'shape {0}'.format(solver.shape))
         if rhs is not None and not (_device.is_device_tensor(rhs) and
                                     rhs.is_contiguous() and
-                                    tuple(rhs.shape) == solver.shape):
-            raise ValueError('rhs must be a contiguous GPU field of shape '
-                             '{0}'.format(solver.shape))
+                                    tuple(rhs.shape) == solver.shape and
+                                    rhs.dtype == t.float64):
+            raise ValueError('rhs must be a contiguous float64 GPU field of '
+                             'shape {0}'.format(solver.shape))
         index = ctypes.c_int(-1)
         _lib.check(_lib.lib().jx_batch_add(
             self._handle, solver._handle, eps.data_ptr(), E.ctypes.data,

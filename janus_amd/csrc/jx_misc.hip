@@ -236,6 +236,9 @@ __global__ void __launch_bounds__(kThreads)
 
 int matrices_symmetric(int ncomp, const double *a, int64_t count, int *flag_dev,
                        bool *out) {
+  // the matrices may have been written on any stream of the caller (the
+  // creation call takes none): wait for all of them before reading
+  JX_HIP(hipDeviceSynchronize());
   JX_HIP(hipMemset(flag_dev, 0, sizeof(int)));
   int64_t nb = std::max<int64_t>(1, std::min<int64_t>((count + kThreads - 1) / kThreads, 4096));
   if (ncomp == 3)

@@ -117,6 +117,8 @@ class _RealFFT:
         if c is not None:
             self._check(c, self._oshape, INVALID_COMPLEX_ARRAY_SHAPE)
         device = _device.is_device_tensor(r)
+        if c is not None:
+            _device.require_f64(c, 'c')
         rd = _device.to_device(r)
         if device and c is not None and _device.is_device_tensor(c) \
                 and c.is_contiguous():
@@ -139,6 +141,8 @@ class _RealFFT:
         if r is not None:
             self._check(r, self._ishape, INVALID_REAL_ARRAY_SHAPE)
         device = _device.is_device_tensor(c)
+        if r is not None:
+            _device.require_f64(r, 'r')
         cd = _device.to_device(c)
         if device and r is not None and _device.is_device_tensor(r) \
                 and r.is_contiguous():

@@ -191,7 +191,9 @@ class _DiscreteGreenOperator:
             _lib.check(L.jx_green_apply(self._handle, xd.data_ptr(),
                                         yd.data_ptr(), _lib.current_stream()))
         if _device.is_device_tensor(x):
-            xd = x.contiguous()
+            xd = _device.require_f64(x, 'x').contiguous()
+            if y is not None:
+                _device.require_f64(y, 'y')
             if y is not None and _device.is_device_tensor(y) and \
                     y.is_contiguous():
                 run(xd, y)

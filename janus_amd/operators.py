@@ -283,7 +283,9 @@ def _run_device(fn, x, y, oshape):
     """Run fn(x_dev, y_dev) on device copies of host arrays, or in place on
     device tensors; returns what janus's apply returns."""
     if _device.is_device_tensor(x):
-        xd = x.contiguous()
+        xd = _device.require_f64(x, 'x').contiguous()
+        if y is not None:
+            _device.require_f64(y, 'y')
         if y is not None and _device.is_device_tensor(y) and y.is_contiguous():
             yd = y
         else:

@@ -118,9 +118,10 @@ class LSSolver:
         rhs_ptr = None
         if rhs is not None:
             if not (_device.is_device_tensor(rhs) and rhs.is_contiguous() and
-                    tuple(rhs.shape) == self.shape):
-                raise ValueError('rhs must be a contiguous GPU field of '
-                                 'shape {0}'.format(self.shape))
+                    tuple(rhs.shape) == self.shape and
+                    rhs.dtype == t.float64):
+                raise ValueError('rhs must be a contiguous float64 GPU field '
+                                 'of shape {0}'.format(self.shape))
             rhs_ptr = rhs.data_ptr()
         if res_scale is None:
             res_scale = 0.0
@@ -164,9 +165,10 @@ class LSSolver:
                              'shape {0}'.format(self.shape))
         if rhs is not None and not (_device.is_device_tensor(rhs) and
                                     rhs.is_contiguous() and
-                                    tuple(rhs.shape) == self.shape):
-            raise ValueError('rhs must be a contiguous GPU field of shape '
-                             '{0}'.format(self.shape))
+                                    tuple(rhs.shape) == self.shape and
+                                    rhs.dtype == t.float64):
+            raise ValueError('rhs must be a contiguous float64 GPU field of '
+                             'shape {0}'.format(self.shape))
         hist = np.zeros(max_iter, dtype=np.float64)
         iters = ctypes.c_int(0)
         status = ctypes.c_int(0)
@@ -239,9 +241,10 @@ class LSSolver:
                              'shape {0}'.format(self.shape))
         if rhs is not None and not (_device.is_device_tensor(rhs) and
                                     rhs.is_contiguous() and
-                                    tuple(rhs.shape) == self.shape):
-            raise ValueError('rhs must be a contiguous GPU field of shape '
-                             '{0}'.format(self.shape))
+                                    tuple(rhs.shape) == self.shape and
+                                    rhs.dtype == t.float64):
+            raise ValueError('rhs must be a contiguous float64 GPU field of '
+                             'shape {0}'.format(self.shape))
         hist = np.zeros(max_iter, dtype=np.float64)
         iters = ctypes.c_int(0)
         status = ctypes.c_int(0)

@@ -1,0 +1,291 @@
+"""The batched, differentiable and large configurations on their stated
+workloads, through checks that hold at any size plus the oracle where it
+finishes in seconds.
+
+* Batched (grids {64, 80, 96, 128, 160}^3, Boolean spheres r = 8,
+  contrast logU[2, 100], filtered Gamma0, six Mandel unit loads each): ten
+  of the configuration's 32 draws -- the lowest and the highest contrast of
+  every grid size, up to 73 -- as one batch of 60 right-hand sides.  Every
+  batched entry equals its sequential solve bit for bit (iterates,
+  residual histories, iteration counts); one entry per grid size equals
+  the oracle's basic scheme (square_basic.py:83-92) for its first iterates.
+* Differentiable (128^3 smoothed density rho, mu = 1 + 19 rho^3, nu = 0.3,
+  E = e_xy, L = (C_1212 - 3)^2, C_1212 from janus's post-process
+  <sigma> = C0 : E + <tau>, square_basic.py:95-97): the gradients of
+  ls_solve by conjugate gradients and by the basic scheme and of
+  effective_stress agree to 1e-9, and a central difference of L along a
+  smooth seeded direction v matches <dL/drho, v>.
+* Large (512^3, three phases of randomly oriented 1:1:4 ellipsoids,
+  mu = 1, 10, 0.1, filtered Gamma0): a 6-load batch equals the sequential
+  solve of each load checked bit for bit; the converged e_xy solve has mean
+  strain E and satisfies the Lippmann-Schwinger equation evaluated with the
+  separate Green and block-diagonal operators; the per-phase gradient of
+  C_eff[5, 5] through the adjoint solve equals effective_stress's to 1e-9.
+"""
+
+import numpy as np
+import pytest
+
+import janus_amd.fft.serial as fft
+import janus_amd.material.elastic.linear.isotropic as material
+from janus_amd import microstructure as ms
+from janus_amd.autograd import (effective_stress, iso_delta, ls_solve,
+                                mean_stress)
+from janus_amd.batch import BatchSolver
+from janus_amd.green import filtered
+from janus_amd.operators import BlockDiagonalOperator3D
+from janus_amd.solver import LSSolver, delta_stiffness, reference_stiffness
+from oracle import janus_oracle as jo
+
+pytestmark = pytest.mark.gpu
+
+
+def _unit(j, sym=6):
+    e = np.zeros(sym)
+    e[j] = 1.0
+    return e
+
+
+def _two_phase(phase, contrast):
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(contrast, 0.25, 3)
+    mat0 = material.create(0.5 * (1.0 + contrast), 0.3, 3)
+    table = np.array([delta_stiffness(m, mat0) for m in (mat_m, mat_i)])
+    green = filtered(mat0.green_operator(), phase.shape, 1.0,
+                     fft.create_real(phase.shape))
+    return LSSolver(green, BlockDiagonalOperator3D.from_phases(phase, table)), \
+        table, mat0
+
+
+# ------------------------------------------------------------------ batched
+def _batched_subset():
+    """Lowest- and highest-contrast draw of every grid size."""
+    cases = ms.batched_cases()
+    pick = []
+    for n in (64, 80, 96, 128, 160):
+        sub = [c for c in cases if c[0] == n]
+        pick.append(min(sub, key=lambda c: c[2]))
+        pick.append(max(sub, key=lambda c: c[2]))
+    return pick
+
+
+def test_batched_configuration_bitwise_and_oracle():
+    import torch
+    cases = _batched_subset()
+    assert sorted(set(c[0] for c in cases)) == [64, 80, 96, 128, 160]
+    assert max(c[2] for c in cases) > 70
+    built = []
+    for n, vf, c, seed in cases:
+        phase = ms.spheres(n, 8.0, vf, seed=seed)
+        S, table, mat0 = _two_phase(phase, c)
+        built.append((S, phase, table, mat0))
+    rhs = [(i, j) for i in range(len(built)) for j in range(6)]
+    max_iter = 3000
+
+    def fields():
+        out = []
+        for i, j in rhs:
+            e = torch.empty(built[i][0].shape, dtype=torch.float64,
+                            device='cuda')
+            e[...] = torch.from_numpy(_unit(j)).cuda()
+            out.append(e)
+        return out
+
+    seq_eps = fields()
+    seq = [built[i][0].solve(_unit(j), tol=1e-10, max_iter=max_iter, eps=e)
+           for (i, j), e in zip(rhs, seq_eps)]
+    assert all(r.status == 'converged' for r in seq)
+    batch = BatchSolver(iters_per_launch=8, graph=False, autotune=True)
+    bat_eps = fields()
+    for (i, j), e in zip(rhs, bat_eps):
+        batch.add(built[i][0], _unit(j), tol=1e-10, max_iter=max_iter, eps=e)
+    res = batch.run()
+    for k, (r, s) in enumerate(zip(res, seq)):
+        assert r.iterations == s.iterations, (rhs[k], r.iterations,
+                                              s.iterations)
+        assert r.status == s.status
+        assert np.array_equal(r.residuals, s.residuals), rhs[k]
+        assert torch.equal(bat_eps[k], seq_eps[k]), rhs[k]
+    its = [s.iterations for s in seq]
+    print('batched configuration: %d entries, %d-%d iterations, widths %s'
+          % (len(rhs), min(its), max(its), [g[2] for g in batch.groups()]),
+          flush=True)
+    del batch, bat_eps, seq_eps
+    torch.cuda.empty_cache()
+
+    # first iterates of one entry per grid size (the highest contrast)
+    for S, phase, table, mat0 in built[1::2]:
+        n = phase.shape[0]
+        k = 2 if n >= 128 else 3
+        E = _unit(5)
+        r = S.solve(E, tol=0.0, max_iter=k)
+        eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1],
+                                           E, 'filtered', mat0.g, mat0.nu,
+                                           tol=0.0, max_iter=k)
+        err = np.abs(r.eps.cpu().numpy() - eps_ref).max() / \
+            np.abs(eps_ref).max()
+        print('batched configuration %d^3: %d iterates vs oracle, max rel '
+              'err %.3g' % (n, k, err), flush=True)
+        np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-10)
+        assert err < 1e-12, (n, err)
+
+
+# ----------------------------------------------------------- differentiable
+def test_simp_128_gradients_agree_and_match_directional_difference():
+    import torch
+    n = 128
+    rho_np = ms.smoothed_density(n, 4.0, seed=2014)
+    mat0 = material.create(10.5, 0.3, 3)
+    green = filtered(mat0.green_operator(), (n,) * 3, 1.0,
+                     fft.create_real((n,) * 3))
+    E = torch.zeros(6, dtype=torch.float64, device='cuda')
+    E[5] = 1.0
+
+    def fields(rho):
+        g = 1.0 + rho ** 3 * 19.0
+        return iso_delta(g, torch.full_like(rho, 0.3), mat0, 3)
+
+    def grad(kind, method):
+        rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
+        sph, dev = fields(rho)
+        info = {}
+        if kind == 'ls':
+            eps = ls_solve(sph, dev, E, green, tol=1e-12, max_iter=5000,
+                           info=info, method=method)
+            sigma = mean_stress(sph, dev, eps, E, mat0)
+        else:
+            sigma = effective_stress(sph, dev, E, green, mat0, tol=1e-12,
+                                     max_iter=5000, info=info, method=method)
+        loss = (0.5 * sigma[5] / E[5] - 3.0) ** 2
+        loss.backward()
+        assert info['forward'].status == 'converged'
+        return rho.grad.cpu().numpy(), float(loss.item())
+
+    g_cg, L0 = grad('ls', 'cg')
+    g_basic, _ = grad('ls', 'basic')
+    g_es, _ = grad('es', 'cg')
+    scale = np.abs(g_cg).max()
+    d1 = np.abs(g_cg - g_basic).max() / scale
+    d2 = np.abs(g_cg - g_es).max() / scale
+    print('SIMP 128^3: L = %.12g, |cg - basic| %.3g, |cg - effective_stress| '
+          '%.3g (relative to max |dL/drho|)' % (L0, d1, d2), flush=True)
+    assert d1 < 1e-9 and d2 < 1e-9
+
+    # central difference along a smooth seeded direction
+    v = ms.smoothed_density(n, 4.0, seed=7) - 0.5
+    h = 1e-6 / np.abs(v).max()
+    C0 = reference_stiffness(mat0)
+
+    def loss_at(rho_np_):
+        rho = torch.tensor(rho_np_, device='cuda')
+        sph, dev = fields(rho)
+        local = BlockDiagonalOperator3D.from_fields(sph, dev)
+        r = LSSolver(green, local).solve_cg(_unit(5), tol=1e-14,
+                                            max_iter=400)
+        tau, _ = local.means(r.eps)
+        sigma = C0.dot(_unit(5)) + tau
+        return (0.5 * sigma[5] - 3.0) ** 2
+
+    fd = (loss_at(rho_np + h * v) - loss_at(rho_np - h * v)) / (2 * h)
+    dd = float((g_cg * v).sum())
+    rel = abs(fd - dd) / abs(dd)
+    print('SIMP 128^3: directional derivative %.12g, central difference '
+          '%.12g, rel err %.3g' % (dd, fd, rel), flush=True)
+    assert rel < 1e-6
+
+
+# -------------------------------------------------------------------- large
+def test_large_512_three_phase():
+    import torch
+    n = 512
+    phase_d, count, axes = ms.large_three_phase(n)
+    frac = torch.bincount(phase_d.flatten().long(), minlength=3).double() \
+        / phase_d.numel()
+    print('512^3 three-phase map: %d ellipsoids, fractions %s'
+          % (count, [round(x, 4) for x in frac.tolist()]), flush=True)
+    assert 0.6 < float(frac[0]) < 0.8
+    phase = phase_d.cpu().numpy()
+    mus = [1.0, 10.0, 0.1]
+    mats = [material.create(m, 0.3, 3) for m in mus]
+    mat0 = material.create(0.5 * (10.0 + 0.1), 0.3, 3)
+    table = np.array([delta_stiffness(m, mat0) for m in mats])
+    green = filtered(mat0.green_operator(), (n,) * 3, 1.0,
+                     fft.create_real((n,) * 3))
+    local = BlockDiagonalOperator3D.from_phases(phase, table)
+    solver = LSSolver(green, local)
+
+    # six loads as one batch, k iterations each; loads 0 and 5 once more
+    # on their own
+    k = 3
+    batch = BatchSolver(iters_per_launch=k, graph=False)
+    fields = []
+    for j in range(6):
+        e = torch.empty(solver.shape, dtype=torch.float64, device='cuda')
+        e[...] = torch.from_numpy(_unit(j)).cuda()
+        fields.append(e)
+        batch.add(solver, _unit(j), tol=0., max_iter=k, eps=e)
+    res = batch.run()
+    assert all(r.iterations == k for r in res)
+    for j in (0, 5):
+        r = solver.solve(_unit(j), tol=0., max_iter=k)
+        assert np.array_equal(r.residuals, res[j].residuals), j
+        assert torch.equal(r.eps, fields[j]), j
+        del r
+    print('512^3: 6-load batch equals the sequential solves (%d iterations)'
+          % k, flush=True)
+    del batch, fields, res
+    torch.cuda.empty_cache()
+
+    # converged e_xy load and the per-phase gradient through the adjoint
+    ph = torch.from_numpy(phase.astype(np.int64)).cuda()
+    masks = [(ph == p).double() for p in range(3)]
+    del ph
+    Et = torch.zeros(6, dtype=torch.float64, device='cuda')
+    Et[5] = 1.0
+
+    def moduli(mu_t):
+        g = masks[0] * mu_t[0]
+        for p in (1, 2):
+            g = g + masks[p] * mu_t[p]
+        return iso_delta(g, torch.full_like(g, 0.3), mat0, 3)
+
+    mu_t = torch.tensor(mus, dtype=torch.float64, device='cuda',
+                        requires_grad=True)
+    sph, dev = moduli(mu_t)
+    info = {}
+    eps = ls_solve(sph, dev, Et, green, tol=1e-10, max_iter=1000, info=info)
+    assert info['forward'].status == 'converged'
+    L = mean_stress(sph, dev, eps, Et, mat0)[5]
+    L.backward()
+    grad = mu_t.grad.cpu().numpy()
+    assert info['adjoint'].status == 'converged'
+
+    e = eps.detach()
+    mean = e.reshape(-1, 6).mean(dim=0)
+    assert float((mean - Et).abs().max()) < 1e-12
+    tau = torch.empty_like(e)
+    local.apply(e, tau)
+    gt = torch.empty_like(e)
+    green.apply(tau, gt)
+    lhs = e + gt - Et
+    resid = float(torch.linalg.vector_norm(lhs)) / np.sqrt(n ** 3)
+    print('512^3: e_xy solve in %d iterations, mean strain ok, '
+          'Lippmann-Schwinger residual %.3g' % (info['forward'].iterations,
+                                                resid), flush=True)
+    assert resid < 2e-10
+    del e, tau, gt, lhs, eps, sph, dev
+    torch.cuda.empty_cache()
+
+    mu_es = torch.tensor(mus, dtype=torch.float64, device='cuda',
+                         requires_grad=True)
+    sph, dev = moduli(mu_es)
+    info_es = {}
+    sigma = effective_stress(sph, dev, Et, green, mat0, tol=1e-10,
+                             max_iter=1000, info=info_es)
+    sigma[5].backward()
+    grad_es = mu_es.grad.cpu().numpy()
+    d = np.abs(grad - grad_es).max() / np.abs(grad).max()
+    print('512^3: dL/dmu %s, effective_stress %s, max rel diff %.3g'
+          % (grad.tolist(), grad_es.tolist(), d), flush=True)
+    assert info_es['adjoint'] is None
+    assert d < 1e-9
