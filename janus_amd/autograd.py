@@ -22,7 +22,7 @@ non-symmetric matrix fields); or the basic scheme (``method='basic'``),
 whose adjoint fixed point reads G g in every update.
 
 A forward or adjoint solve that stops without reaching ``tol`` (status
-'maxiter' or 'nonfinite') raises a RuntimeWarning: its field, and any
+'max_iter' or 'nonfinite', with tol > 0) raises a RuntimeWarning: its field, and any
 gradient built on it, is not the converged one.
 
 For losses of the macroscopic stress only (effective moduli),
@@ -49,10 +49,12 @@ def _local_from_fields(sph, dev):
     return cls.from_fields(sph, dev)
 
 
-def _check_status(r, what):
+def _check_status(r, what, tol):
     """Warn when a solve stopped without converging (r: SolveResult or
-    None for a solve that was not needed)."""
-    if r is not None and r.status != 'converged':
+    None for a solve that was not needed).  tol <= 0 asks for a fixed
+    iteration count: stopping at max_iter is then the request."""
+    if r is not None and r.status != 'converged' and \
+            not (tol <= 0 and r.status == 'max_iter'):
         warnings.warn('janus_amd: the {0} solve stopped with status {1!r} '
                       'after {2} iterations (residual {3:.3e}); its result '
                       'is not converged'.format(
@@ -102,17 +104,17 @@ def adjoint_solve(green, local, g, tol, max_iter, solver=None,
     if method == 'cg':
         r = solver.solve_cg(zero, tol=tol, max_iter=max_iter, rhs=gg,
                             res_scale=1.0 / norm)
-        _check_status(r, 'adjoint')
+        _check_status(r, 'adjoint', tol)
         return r.eps, r
     if method == 'gmres':
         r = solver.solve_gmres(zero, tol=tol, max_iter=max_iter, rhs=gg,
                                res_scale=1.0 / norm)
-        _check_status(r, 'adjoint')
+        _check_status(r, 'adjoint', tol)
         return r.eps, r
     mu = gg.clone()
     r = solver.solve(zero, tol=tol, max_iter=max_iter, eps=mu, rhs=gg,
                      res_scale=1.0 / norm)
-    _check_status(r, 'adjoint')
+    _check_status(r, 'adjoint', tol)
     return mu, r
 
 
@@ -134,7 +136,7 @@ def _make_function():
             solver = LSSolver(green, local)
             r = _solve(solver, E.detach().cpu().numpy(), tol, max_iter,
                        method)
-            _check_status(r, 'forward')
+            _check_status(r, 'forward', tol)
             if info is not None:
                 info['forward'] = r
             ctx.save_for_backward(sph, dev, r.eps)
@@ -206,7 +208,7 @@ def _make_effective_stress():
             solver = LSSolver(green, local)
             En = E.detach().cpu().numpy()
             r = _solve(solver, En, tol, max_iter, method)
-            _check_status(r, 'forward')
+            _check_status(r, 'forward', tol)
             if info is not None:
                 info['forward'] = r
                 info['adjoint'] = None
@@ -238,7 +240,7 @@ def _make_effective_stress():
                 eps_g, scale = eps, alpha
             else:
                 r = _solve(ctx.solver, gb, tol, max_iter, method)
-                _check_status(r, 'adjoint')
+                _check_status(r, 'adjoint', tol)
                 if info is not None:
                     info['adjoint'] = r
                 eps_g, scale = r.eps, 1.0
@@ -324,7 +326,7 @@ def _make_matrix_function():
             solver = LSSolver(green, local)
             r = _solve(solver, E.detach().cpu().numpy(), tol, max_iter,
                        method)
-            _check_status(r, 'forward')
+            _check_status(r, 'forward', tol)
             if info is not None:
                 info['forward'] = r
             if local.symmetric:
