@@ -81,6 +81,21 @@ int jx_green_create(int dim, const int64_t *shape, double h, int kind,
 int jx_green_destroy(jx_green *green);
 /* y = Gamma0[x]; x, y: (n0, n1[, n2], sym) device arrays; x == y allowed. */
 int jx_green_apply(jx_green *green, const double *x, double *y, void *stream);
+/* Allocation-free variant of jx_green_apply (no counterpart in janus: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results; the spectrum scratch is a file-scope static device
+ * array of 64 MiB that each GPU holds until the process ends, instead of
+ * the handle's own buffer.  The call makes no hipMalloc, hipFree or other
+ * allocating call: the grid's spectrum, 16 sym (n0 n1 [n2 -- the last axis
+ * halved] + 1) bytes, must fit the 64 MiB (3D up to about 100^3, 2D up to
+ * 1024^2), else JX_EINVAL; the handle must have been created with a GPU
+ * present and this device current (its tables and FFT plans are made by
+ * jx_green_create then), else JX_EINVAL.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (two streams, two threads) race and
+ * must be serialized by the caller. */
+int jx_green_apply_static(jx_green *green, const double *x, double *y,
+                          void *stream);
 /* Host-side per-frequency matrix Gamma-hat(b), b in [0, n_i), written to
  * out[sym * sym] (row major).  Replaces set_frequency + to_memoryview
  * (janus/green.pyx:146-171, 312-346). */

@@ -46,6 +46,7 @@ SIGNATURES = [
       ctypes.c_double, ctypes.c_double, _PP]),
     ('jx_green_destroy', ctypes.c_int, [_VP]),
     ('jx_green_apply', ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    ('jx_green_apply_static', ctypes.c_int, [_VP, _VP, _VP, _VP]),
     ('jx_green_matrix_at', ctypes.c_int, [_VP, _VP, _VP]),
     ('jx_gamma0_at', ctypes.c_int,
      [ctypes.c_int, ctypes.c_double, ctypes.c_double, _VP, _VP]),

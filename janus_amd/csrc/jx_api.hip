@@ -125,6 +125,15 @@ struct LenEntry {
 static std::mutex g_plan_mu;
 static std::map<std::pair<int, int>, LenEntry> g_plans;  // (device, L)
 
+// True when the plan of length L exists on the current device (a lookup
+// only: nothing is allocated).
+static bool fft_len_cached(int L) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  return g_plans.find({dev, L}) != g_plans.end();
+}
+
 int get_fft_len(int L, FftLen *out) {
   if (L <= 0) return fail(JX_EINVAL, "FFT length must be > 0");
   int dev = 0;
@@ -270,6 +279,60 @@ static void point_tables(GreenDev &D, const Grid &g, const double *base) {
   }
 }
 
+// ------------------------------------------------------------ static scratch
+// Spectrum scratch of jx_green_apply_static: a file-scope device array that
+// every GPU the library runs on holds until the process ends, so that the
+// call allocates nothing.  In host code its name is a host-side stand-in,
+// not a device address: the kernels get the per-device address that
+// hipGetSymbolAddress returns.
+#define JX_STATIC_SCRATCH_BYTES (64ll << 20)
+static __device__ __align__(256) double
+    scratch[JX_STATIC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;  // device -> address of scratch
+
+// Address of `scratch` on the current device, looked up once per device.
+static int static_scratch(double **out) {
+  int dev = 0;
+  JX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (ptr == nullptr) return fail(JX_EHIP, "hipGetSymbolAddress(scratch): NULL");
+    it = g_scratch_ptr.emplace(dev, (double *)ptr).first;
+  }
+  *out = it->second;
+  return JX_OK;
+}
+
+}  // extern "C"
+
+// Device tables and FFT plans of a Green operator on the current device
+// (small; no spectrum), and the static scratch's address there.
+static int green_prepare_device(jx_green *G) {
+  if (G->tables) return JX_OK;
+  int dev = 0;
+  JX_HIP(hipGetDevice(&dev));
+  JX_HIP(hipMalloc((void **)&G->tables, sizeof(double) * G->host_tables.size()));
+  JX_HIP(hipMemcpy(G->tables, G->host_tables.data(),
+                   sizeof(double) * G->host_tables.size(), hipMemcpyHostToDevice));
+  G->device = dev;
+  point_tables(G->dev, G->g, G->tables);
+  FftLen P;
+  for (int d = 0; d < G->g.dim; ++d) {
+    int rc = get_fft_len((int)G->g.n[d], &P);
+    if (rc) return rc;
+  }
+  double *unused = nullptr;
+  return static_scratch(&unused);
+}
+
+extern "C" {
+
 int jx_green_create(int dim, const int64_t *shape, double h, int kind, double g0,
                     double nu0, jx_green **out) {
   if (!out) return fail(JX_EINVAL, "out is NULL");
@@ -298,25 +361,31 @@ int jx_green_create(int dim, const int64_t *shape, double h, int kind, double g0
   G->host = D;
   point_tables(G->host, G->g, G->host_tables.data());
   G->dev = D;
+  // with a GPU present the device tables and FFT plans are made now (the
+  // spectrum waits for the first apply); without one the host-side
+  // per-frequency API still works
+  int count = 0;
+  if (hipGetDeviceCount(&count) == hipSuccess && count > 0) {
+    rc = green_prepare_device(G);
+    if (rc) {
+      jx_green_destroy(G);
+      return rc;
+    }
+  } else {
+    (void)hipGetLastError();
+  }
   *out = G;
   return JX_OK;
 }
 
 }  // extern "C"
 
-// Device tables, spectrum scratch and FFT plans are set up on first use, so
-// that the host-side per-frequency API works without a GPU.
+// Device tables, spectrum scratch and FFT plans, on first use when the
+// handle was created without a GPU.
 int jx::green_ensure_device(jx_green *G) {
   if (G->spec) return JX_OK;
-  JX_HIP(hipMalloc((void **)&G->tables, sizeof(double) * G->host_tables.size()));
-  JX_HIP(hipMemcpy(G->tables, G->host_tables.data(),
-                   sizeof(double) * G->host_tables.size(), hipMemcpyHostToDevice));
-  point_tables(G->dev, G->g, G->tables);
-  FftLen P;
-  for (int d = 0; d < G->g.dim; ++d) {
-    int rc = get_fft_len((int)G->g.n[d], &P);
-    if (rc) return rc;
-  }
+  int rc = green_prepare_device(G);
+  if (rc) return rc;
   JX_HIP(hipMalloc((void **)&G->spec, sizeof(double) * 2 * G->sym * G->g.Nh));
   return JX_OK;
 }
@@ -353,6 +422,35 @@ int jx_green_apply(jx_green *G, const double *x, double *y, void *stream) {
   LocalDev none;
   RowsInverseLs plain;
   return green_passes(G, G->spec, x, y, none, plain, nullptr, nullptr,
+                      (hipStream_t)stream);
+}
+
+int jx_green_apply_static(jx_green *G, const double *x, double *y,
+                          void *stream) {
+  if (!G || !x || !y) return fail(JX_EINVAL, "jx_green_apply_static: NULL argument");
+  const long long need = (long long)sizeof(double) * 2 * G->sym * G->g.Nh;
+  if (need > JX_STATIC_SCRATCH_BYTES)
+    return fail(JX_EINVAL, "jx_green_apply_static: the grid needs " +
+                               std::to_string(need) + " bytes of spectrum "
+                               "scratch, more than the static " +
+                               std::to_string(JX_STATIC_SCRATCH_BYTES) +
+                               " (use jx_green_apply)");
+  int dev = 0;
+  JX_HIP(hipGetDevice(&dev));
+  if (!G->tables || G->device != dev)
+    return fail(JX_EINVAL, "jx_green_apply_static: the operator's device tables "
+                           "are not on the current device (create it with "
+                           "that device current)");
+  for (int d = 0; d < G->g.dim; ++d)
+    if (!fft_len_cached((int)G->g.n[d]))
+      return fail(JX_EINVAL, "jx_green_apply_static: FFT plans missing on the "
+                             "current device");
+  double *spec = nullptr;
+  int rc = static_scratch(&spec);
+  if (rc) return rc;
+  LocalDev none;
+  RowsInverseLs plain;
+  return green_passes(G, spec, x, y, none, plain, nullptr, nullptr,
                       (hipStream_t)stream);
 }
 

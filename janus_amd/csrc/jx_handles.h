@@ -17,6 +17,7 @@ struct jx_green {
   double g0 = 0, nu0 = 0;
   jx::GreenDev dev;
   double *tables = nullptr;   // device per-axis tables
+  int device = -1;            // device holding tables (and spec)
   std::vector<double> host_tables;
   jx::GreenDev host;          // same tables, host pointers
   double *spec = nullptr;     // spectrum scratch (Nh * 2 sym)
