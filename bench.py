@@ -13,7 +13,22 @@ over the whole grid, one load case):
   * Mandel macroscopic strain E = e_xy.
 Inputs are resident in HBM before the timed region.
 
+The default JSON line also carries, measured in the same process:
+  * roofline: the slowest pass's algorithmic bytes over its hipEvent time,
+    and its HBM traffic per launch from the committed rocprofv3 counter
+    summary (null when that summary was recorded for other kernel sources);
+  * cpu_baseline / cpu_baseline_all_cores: the C restatement of janus's
+    serial path (oracle/janus_serial.c) on the same 256^3 workload, 5
+    iterations, 1 thread (janus is single-threaded) and all host cores;
+  * adjoint_over_forward and configs: the differentiable 128^3 SIMP
+    configuration (adjoint-gradient time / forward time, the metric's second
+    half), the batched {64..160}^3 mix (speedup over sequential solves and
+    the bandwidth bound of the mix) and the 512^3 three-phase configuration
+    (6-load batch iterations/s, roofline fraction, per-phase adjoint);
+    --no-configs skips them.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       [--no-configs]
 
 Other workloads (one JSON line each, --workload NAME):
   plain2d    2D 256^2 square inclusion, truncated, C0 = matrix: the stated
@@ -97,25 +112,38 @@ def build_workload(n, kind):
     return LSSolver(green, local), phase, table, mat0
 
 
-def cpu_baseline(n, kind, phase, table, mat0, iters=1):
-    """The numpy oracle (pocketfft, one thread) on the same workload."""
-    from threadpoolctl import threadpool_limits
-    from oracle import janus_oracle as jo
-    sph = table[phase, 0]
-    dev = table[phase, 1]
+def cpu_baseline(n, kind, phase, table, mat0, iters=5, threads=1):
+    """The C restatement of janus's serial path (oracle/janus_serial.c:
+    per-component r2c, janus's per-frequency Gamma-hat loop, c2r; the basic
+    scheme of square_basic.py:83-92) on the same workload, single-threaded
+    like janus (threads=1) or with OpenMP on `threads` host threads.
+    Returns (iterations/s, seconds, residuals)."""
+    import ctypes
+    lib_path = os.path.join(ROOT, 'oracle', 'libjanus_serial.so')
+    if not os.path.exists(lib_path):
+        raise RuntimeError('oracle/libjanus_serial.so is missing (built by '
+                           '__graft_entry__.build())')
+    lib = ctypes.CDLL(lib_path)
+    phase = np.ascontiguousarray(phase, dtype=np.uint8)
+    table = np.ascontiguousarray(table, dtype=np.float64)
     E = np.zeros(6)
     E[5] = 1.0
     eps = np.empty((n, n, n, 6))
     eps[...] = E
-    with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        for _ in range(iters):
-            tau = jo.iso4_apply(sph, dev, eps)
-            new = E - jo.green_apply(tau, kind, mat0.g, mat0.nu, 1.0)
-            np.linalg.norm(new - eps)
-            eps = new
-        t = time.perf_counter() - t0
-    return iters / t, t
+    res = np.zeros(iters)
+    shape = (ctypes.c_int64 * 3)(n, n, n)
+    vp = ctypes.c_void_p
+    t0 = time.perf_counter()
+    rc = lib.js_basic_iterations(
+        3, shape, 0 if kind == 'truncated' else 1, ctypes.c_double(mat0.g),
+        ctypes.c_double(mat0.nu), ctypes.c_double(1.0),
+        phase.ctypes.data_as(vp), table.ctypes.data_as(vp),
+        E.ctypes.data_as(vp), eps.ctypes.data_as(vp), int(iters),
+        res.ctypes.data_as(vp), int(threads))
+    t = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError('js_basic_iterations failed (%d)' % rc)
+    return iters / t, t, res
 
 
 def cpu_model():
@@ -134,39 +162,6 @@ def host_workers():
     """Host threads for the all-cores baseline: the CPUs this process may
     run on, at most 16 (one GPU's share of a GPU node)."""
     return max(1, min(16, len(os.sched_getaffinity(0))))
-
-
-def cpu_baseline_threads(n, kind, phase, table, mat0, iters=1, workers=1):
-    """The same oracle iteration on `workers` host threads: pocketfft with
-    that many workers, and the local operator, the frequency loop and the
-    update on axis-0 slabs in a thread pool."""
-    from concurrent.futures import ThreadPoolExecutor
-    from oracle import janus_oracle as jo
-    sph = table[phase, 0]
-    dev = table[phase, 1]
-    E = np.zeros(6)
-    E[5] = 1.0
-    eps = np.empty((n, n, n, 6))
-    eps[...] = E
-    tau = np.empty_like(eps)
-    sq = np.zeros(n)
-    with ThreadPoolExecutor(workers) as pool:
-        t0 = time.perf_counter()
-        for _ in range(iters):
-            def local(i):
-                tau[i] = jo.iso4_apply(sph[i], dev[i], eps[i])
-            list(pool.map(local, range(n)))
-            y = jo.green_apply(tau, kind, mat0.g, mat0.nu, 1.0,
-                               workers=workers)
-
-            def update(i):
-                new = E - y[i]
-                sq[i] = np.sum((new - eps[i]) ** 2)
-                eps[i] = new
-            list(pool.map(update, range(n)))
-            np.sqrt(sq.sum())
-        t = time.perf_counter() - t0
-    return iters / t, t
 
 
 def stream_copy_gbps(mib=2048, reps=10):
@@ -200,6 +195,22 @@ def stream_copy_gbps(mib=2048, reps=10):
 
 
 TRAFFIC_TAG = 'traffic_per_launch'
+SOURCES_TAG = 'kernel_sources_sha256'
+
+
+def kernel_sources_hash():
+    """sha256 of the HIP sources the library is built from (file names and
+    contents, in name order): a counter summary recorded for other sources
+    describes other kernels."""
+    import hashlib
+    csrc = os.path.join(ROOT, 'janus_amd', 'csrc')
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(csrc)):
+        if f.endswith(('.hip', '.h')):
+            h.update(f.encode())
+            with open(os.path.join(csrc, f), 'rb') as fh:
+                h.update(fh.read())
+    return h.hexdigest()
 
 
 def traffic_file(n, kind):
@@ -210,16 +221,30 @@ def traffic_file(n, kind):
 def read_traffic(n, kind, kernel, v):
     """HBM bytes per launch of a pass run with the v-values-per-thread plan,
     from the committed counter summary (lines '<tag> <pass> <v> <bytes>'),
-    or None."""
+    and where it came from.  (None, reason) when the summary is missing,
+    lists no such kernel, or was recorded for other kernel sources than
+    these (its '<sources tag> <sha256>' line)."""
     path = traffic_file(n, kind)
+    rel = os.path.relpath(path, ROOT)
     if not os.path.exists(path):
-        return None
+        return None, 'no counter summary %s' % rel
+    recorded = None
+    value = None
     for line in open(path):
         f = line.strip().strip('`').split()
+        if len(f) == 2 and f[0] == SOURCES_TAG:
+            recorded = f[1]
         if len(f) == 4 and f[0] == TRAFFIC_TAG and f[1] == kernel and \
                 f[2] == str(v):
-            return int(f[3])
-    return None
+            value = int(f[3])
+    if recorded != kernel_sources_hash():
+        return None, ('%s was recorded for other kernel sources (%s %s); '
+                      'rerun the two --pmc passes' % (rel, SOURCES_TAG,
+                                                      recorded))
+    if value is None:
+        return None, '%s lists no %s with %d values per thread' % (rel,
+                                                                   kernel, v)
+    return value, '%s (%s %s)' % (rel, SOURCES_TAG, recorded[:16])
 
 
 def plan_v(name):
@@ -322,7 +347,9 @@ def traffic_summary(fetch_csv, write_csv, cmd,
         if pname:
             tags.append('    %s %s %s %d' % (TRAFFIC_TAG, pname, plan_v(name),
                                             int(fr + wr)))
-    out += ['', 'Traffic per launch (bytes) read by bench.py:', ''] + tags
+    out += ['', 'Traffic per launch (bytes) read by bench.py, for the '
+            'kernels built from these sources:', '',
+            '    %s %s' % (SOURCES_TAG, kernel_sources_hash())] + tags
     return '\n'.join(out) + '\n'
 
 
@@ -628,28 +655,28 @@ def workload_stiffness(args):
           higher=False)
 
 
-def _config3_list(seed=3, count=32):
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        n = int(rng.choice([64, 80, 96, 128, 160]))
-        vf = float(rng.uniform(0.1, 0.35))
-        c = float(np.exp(rng.uniform(np.log(2.0), np.log(100.0))))
-        out.append((n, vf, c, 1000 + i))
-    return out
+def iteration_bytes(shape, fused, sym=6, m=1):
+    """Algorithmic HBM bytes of one iteration of a solve on a 3D grid in its
+    schedule: fused (96 + m) N + 768 Nh, five passes (144 + m) N + 768 Nh."""
+    N = int(np.prod(shape))
+    Nh = int(np.prod(shape[:-1])) * (shape[-1] // 2 + 1)
+    return ((2 if fused else 3) * 8 * sym + m) * N + 8 * 16 * sym * Nh
 
 
-def workload_batched(args):
+def workload_batched(args, emit=True):
     """32 seeded microstructures, n ~ U{64, 80, 96, 128, 160}, Boolean
     spheres r = 8, vf ~ U[0.1, 0.35], inclusion/matrix contrast ~
     logU[2, 100] (matrix mu 1 nu 0.3, inclusions nu 0.25), filtered Gamma0,
     midpoint C0, six Mandel unit loads each: 192 right-hand sides at 1e-10,
-    one autotuned batch vs 192 sequential solves."""
+    one autotuned batch vs 192 sequential solves.  The bound printed beside
+    them is the entries' algorithmic bytes (sum over entries of iterations x
+    bytes per iteration in the solver's schedule) over the measured copy
+    bandwidth: the time of a batch that streams every byte at copy speed."""
     import torch
     import janus_amd.material.elastic.linear.isotropic as material
     from janus_amd import microstructure as ms
     from janus_amd.batch import BatchSolver
-    cases = _config3_list(count=args.batch_count)
+    cases = ms.batched_cases(count=args.batch_count)
     solvers = []
     for n, vf, c, seed in cases:
         phase = ms.spheres(n, 8.0, vf, seed=seed)
@@ -753,45 +780,75 @@ def workload_batched(args):
     conv = sum(s[1] == 'converged' for s in seq)
     groups = [{'grid': list(g[0]), 'entries': g[1], 'width': g[2]}
               for g in batch.groups()]
-    _line('batched vs sequential plain solves, %d right-hand sides (%d '
-          'microstructures x 6 loads) at 1e-10' % (len(rhs), len(solvers)),
-          round(t_seq / t_b, 3), 'x (speedup)', {
-              'workload': '%d seeded microstructures, n~U{64,80,96,128,160}, '
-                          'spheres r=8, vf~U[0.1,0.35], contrast~logU[2,100], '
-                          'filtered Gamma0, midpoint C0, 6 Mandel unit loads, '
-                          'tol 1e-10, max_iter %d'
-                          % (len(solvers), args.max_iter),
-              'grids': [c[0] for c in cases],
-              'contrasts': [round(c[2], 3) for c in cases],
-              'seconds_sequential': round(t_seq, 4),
-              'seconds_batched': round(t_b, 4),
-              'iterations_total': int(sum(its)),
-              'iterations_min_max': [int(min(its)), int(max(its))],
-              'converged': conv,
-              'replays': replays, 'graph_captures': captures,
-              'iters_per_launch': args.iters_per_launch,
-              'groups': groups,
-              'per_grid': per_grid,
-              'parity': {'entries_bitwise_equal': equal,
-                         'entries': len(rhs),
-                         'max_abs_diff': maxdiff,
-                         'same_iteration_counts': same_iters}})
+    del eps_seq, eps_b, batch
+    torch.cuda.empty_cache()
+    copy_gbps = stream_copy_gbps()
+    total_bytes = sum(it * iteration_bytes(solvers[i].shape[:-1],
+                                           solvers[i].fused)
+                      for (i, j), it in zip(rhs, its))
+    bound = total_bytes / (copy_gbps * 1e9)
+    bound_peak = total_bytes / (HBM_PEAK_GBS * 1e9)
+    _log('bound: %.3e B over %.1f GB/s copy = %.3f s (%.3f s at 8 TB/s); '
+         'sequential %.3f s, batched %.3f s'
+         % (total_bytes, copy_gbps, bound, bound_peak, t_seq, t_b))
+    summary = {
+        'speedup': round(t_seq / t_b, 3),
+        'seconds_sequential': round(t_seq, 4),
+        'seconds_batched': round(t_b, 4),
+        'bound_seconds_copy_bw': round(bound, 4),
+        'bound_seconds_8TBps': round(bound_peak, 4),
+        'bytes_alg_total': int(total_bytes),
+        'copy_GBps': round(copy_gbps, 1),
+        'speedup_bound_copy_bw': round(t_seq / bound, 3),
+        'batched_over_bound': round(t_b / bound, 3),
+        'entries': len(rhs),
+        'entries_bitwise_equal': equal,
+        'same_iteration_counts': same_iters,
+        'converged': conv,
+        'iterations_total': int(sum(its)),
+    }
+    if emit:
+        _line('batched vs sequential plain solves, %d right-hand sides (%d '
+              'microstructures x 6 loads) at 1e-10' % (len(rhs), len(solvers)),
+              round(t_seq / t_b, 3), 'x (speedup)', {
+                  'workload': '%d seeded microstructures, n~U{64,80,96,128,'
+                              '160}, spheres r=8, vf~U[0.1,0.35], contrast~'
+                              'logU[2,100], filtered Gamma0, midpoint C0, 6 '
+                              'Mandel unit loads, tol 1e-10, max_iter %d'
+                              % (len(solvers), args.max_iter),
+                  'grids': [c[0] for c in cases],
+                  'contrasts': [round(c[2], 3) for c in cases],
+                  'summary': summary,
+                  'iterations_min_max': [int(min(its)), int(max(its))],
+                  'replays': replays, 'graph_captures': captures,
+                  'iters_per_launch': args.iters_per_launch,
+                  'groups': groups,
+                  'per_grid': per_grid,
+                  'parity': {'entries_bitwise_equal': equal,
+                             'entries': len(rhs),
+                             'max_abs_diff': maxdiff,
+                             'same_iteration_counts': same_iters}})
+    return summary
 
 
-def workload_adjoint(args):
+def workload_adjoint(args, emit=True):
     """128^3 SIMP density: rho = periodic Gaussian-smoothed noise (sigma 4
     voxels, seed 2014) mapped to [0, 1], mu = 1 + 19 rho^3, nu = 0.3,
     midpoint C0 (mu0 = 10.5), filtered Gamma0, E = e_xy, L = (C_1212 - 3)^2.
     Time of the backward (Gamma0 g, adjoint solve, contraction) over the
-    forward solve, both at 1e-10; dL/drho at 20 seeded voxels against
-    central differences (step 1e-6, solves at a fixed iteration count past
-    1e-14)."""
+    forward solve, both at 1e-10, for conjugate gradients (ls_solve's
+    default) and the basic scheme; the same loss through effective_stress;
+    <dL/drho, v> against a central difference of L along a smooth seeded
+    direction v (step 1e-6 relative, solves at 1e-14)."""
     import torch
     import janus_amd.fft.serial as fft
     import janus_amd.material.elastic.linear.isotropic as material
     from janus_amd import microstructure as ms
-    from janus_amd.autograd import iso_delta, ls_solve, mean_stress
+    from janus_amd.autograd import (effective_stress, iso_delta, ls_solve,
+                                    mean_stress)
     from janus_amd.green import filtered
+    from janus_amd.operators import BlockDiagonalOperator3D
+    from janus_amd.solver import LSSolver, reference_stiffness
     n = args.grid or 128
     rho_np = ms.smoothed_density(n, 4.0, seed=2014)
     mat0 = material.create(10.5, 0.3, 3)
@@ -801,69 +858,36 @@ def workload_adjoint(args):
     E[5] = 1.0
     if not args.no_autotune:
         # tune the Green operator's plans on the same grid
-        from janus_amd.operators import BlockDiagonalOperator3D
-        from janus_amd.solver import LSSolver
         z = torch.zeros((n,) * 3, dtype=torch.float64, device='cuda')
         LSSolver(green, BlockDiagonalOperator3D.from_fields(z, z)).autotune(3)
 
-    def loss_fn(rho, tol, max_iter, info=None, method='basic'):
+    def moduli(rho):
         g = 1.0 + rho ** 3 * 19.0
-        nu = torch.full_like(rho, 0.3)
-        sph, dev = iso_delta(g, nu, mat0, 3)
+        return iso_delta(g, torch.full_like(rho, 0.3), mat0, 3)
+
+    def loss_fn(rho, tol, max_iter, info=None, method='cg'):
+        sph, dev = moduli(rho)
         eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
                        info=info, method=method)
         sigma = mean_stress(sph, dev, eps, E, mat0)
         return (0.5 * sigma[5] / E[5] - 3.0) ** 2
 
-    runs = {}
-    grads = {}
-    for method in ('basic', 'cg'):
-        with torch.no_grad():  # warm-up
-            loss_fn(torch.tensor(rho_np, device='cuda'), 0., 3, method=method)
-        ratios = []
-        for rep in range(args.adjoint_reps):
-            rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
-            info = {}
-            _sync()
-            t0 = time.perf_counter()
-            loss = loss_fn(rho, 1e-10, 5000, info, method)
-            _sync()
-            t1 = time.perf_counter()
-            loss.backward()
-            _sync()
-            t2 = time.perf_counter()
-            ratios.append((t1 - t0, t2 - t1))
-        t_fwd, t_bwd = min(r[0] for r in ratios), min(r[1] for r in ratios)
-        fwd, adj = info['forward'], info['adjoint']
-        _log('%s: forward %.3f s (%d it), backward %.3f s (adjoint %d it)'
-             % (method, t_fwd, fwd.iterations, t_bwd, adj.iterations))
-        grads[method] = rho.grad.cpu().numpy()
-        runs[method] = {'seconds_forward': round(t_fwd, 4),
-                        'seconds_backward': round(t_bwd, 4),
-                        'ratio': round(t_bwd / t_fwd, 4),
-                        'forward_iterations': fwd.iterations,
-                        'adjoint_iterations': adj.iterations,
-                        'loss': float(loss.item())}
-    # the same loss through effective_stress: its backward reuses the
-    # forward field (dL/d<sigma> is parallel to E here) -- no adjoint solve
-    from janus_amd.autograd import effective_stress
-
-    def loss_es(rho, tol, max_iter, info=None, method='basic'):
-        g = 1.0 + rho ** 3 * 19.0
-        sph, dev = iso_delta(g, torch.full_like(rho, 0.3), mat0, 3)
+    def loss_es(rho, tol, max_iter, info=None, method='cg'):
+        sph, dev = moduli(rho)
         sigma = effective_stress(sph, dev, E, green, mat0, tol=tol,
                                  max_iter=max_iter, info=info, method=method)
         return (0.5 * sigma[5] / E[5] - 3.0) ** 2
 
-    runs_es = {}
-    for method in ('basic', 'cg'):
+    def timed(fn, method):
+        with torch.no_grad():  # warm-up
+            fn(torch.tensor(rho_np, device='cuda'), 0., 3, method=method)
         times = []
         for rep in range(args.adjoint_reps):
             rho = torch.tensor(rho_np, device='cuda', requires_grad=True)
             info = {}
             _sync()
             t0 = time.perf_counter()
-            loss = loss_es(rho, 1e-10, 5000, info, method)
+            loss = fn(rho, 1e-10, 5000, info, method)
             _sync()
             t1 = time.perf_counter()
             loss.backward()
@@ -871,112 +895,107 @@ def workload_adjoint(args):
             t2 = time.perf_counter()
             times.append((t1 - t0, t2 - t1))
         t_fwd, t_bwd = min(r[0] for r in times), min(r[1] for r in times)
-        g_es = rho.grad.cpu().numpy()
+        return rho.grad.cpu().numpy(), info, t_fwd, t_bwd, float(loss.item())
+
+    runs, grads = {}, {}
+    for method in ('cg', 'basic'):
+        g, info, t_fwd, t_bwd, L = timed(loss_fn, method)
+        fwd, adj = info['forward'], info['adjoint']
+        _log('%s: forward %.4f s (%d it), backward %.4f s (adjoint %d it)'
+             % (method, t_fwd, fwd.iterations, t_bwd, adj.iterations))
+        grads[method] = g
+        runs[method] = {'seconds_forward': round(t_fwd, 4),
+                        'seconds_backward': round(t_bwd, 4),
+                        'ratio': round(t_bwd / t_fwd, 4),
+                        'forward_iterations': fwd.iterations,
+                        'adjoint_iterations': adj.iterations,
+                        'loss': L}
+    # effective_stress: dL/d<sigma> is parallel to E here, so its backward
+    # reuses the forward field (no adjoint solve)
+    runs_es = {}
+    for method in ('cg', 'basic'):
+        g, info, t_fwd, t_bwd, L = timed(loss_es, method)
         runs_es[method] = {
             'seconds_forward': round(t_fwd, 4),
             'seconds_backward': round(t_bwd, 5),
             'ratio': round(t_bwd / t_fwd, 4),
             'backward_solves': 0 if info['adjoint'] is None else 1,
             'grad_vs_ls_solve_max_rel': float(
-                np.abs(g_es - grads[method]).max() /
+                np.abs(g - grads[method]).max() /
                 np.abs(grads[method]).max())}
-        _log('effective_stress %s: forward %.3f s, backward %.4f s'
+        _log('effective_stress %s: forward %.4f s, backward %.5f s'
              % (method, t_fwd, t_bwd))
-    grad = grads['basic']
-    fd = None
-    if args.fd_voxels > 0:
-        info2 = {}
-        rho2 = torch.tensor(rho_np, device='cuda', requires_grad=True)
-        l2 = loss_fn(rho2, 1e-14, 5000, info2)
-        l2.backward()
-        grad14 = rho2.grad.cpu().numpy()
-        iters = info2['forward'].iterations + 50
-        rng = np.random.default_rng(20)
-        h = 1e-6
-        errs = []
-        for _ in range(args.fd_voxels):
-            v = tuple(rng.integers(0, n, 3))
-            rp = rho_np.copy()
-            rp[v] += h
-            rm = rho_np.copy()
-            rm[v] -= h
-            with torch.no_grad():
-                lp = loss_fn(torch.tensor(rp, device='cuda'), 0., iters).item()
-                lm = loss_fn(torch.tensor(rm, device='cuda'), 0., iters).item()
-            d = (lp - lm) / (2 * h)
-            errs.append(abs(d - grad14[v]) / abs(grad14[v]))
-        fd = {'voxels': args.fd_voxels, 'step': h,
-              'fd_solve_iterations': iters,
-              'max_rel_err': float(max(errs)),
-              'median_rel_err': float(np.median(errs)),
-              'grad_1e10_vs_1e14_max_rel': float(
-                  np.abs(grad - grad14).max() / np.abs(grad14).max())}
-    _line('adjoint gradient time / forward solve time at %d^3 (1e-10), '
-          'conjugate gradients' % n,
-          runs['cg']['ratio'], 'ratio', {
-              'workload': '%d^3 SIMP rho smoothed noise sigma 4 seed 2014, '
-                          'mu=1+19 rho^3 nu 0.3, filtered Gamma0, mu0 10.5, '
-                          'E=e_xy, L=(C_1212-3)^2, tol 1e-10' % n,
-              'cg': runs['cg'],
-              'basic_scheme': runs['basic'],
-              'effective_stress': runs_es,
-              'grad_cg_vs_basic_max_rel': float(
-                  np.abs(grads['cg'] - grads['basic']).max() /
-                  np.abs(grads['basic']).max()),
-              'finite_differences': fd},
-          higher=False)
+    # directional central difference along a smooth seeded direction
+    v = ms.smoothed_density(n, 4.0, seed=7) - 0.5
+    h = 1e-6 / np.abs(v).max()
+    C0 = reference_stiffness(mat0)
+    Eh = _unit(6, 5)
+
+    def loss_at(r_np):
+        sph, dev = moduli(torch.tensor(r_np, device='cuda'))
+        local = BlockDiagonalOperator3D.from_fields(sph, dev)
+        r = LSSolver(green, local).solve_cg(Eh, tol=1e-14, max_iter=400)
+        tau, _ = local.means(r.eps)
+        return (0.5 * (C0.dot(Eh) + tau)[5] - 3.0) ** 2
+
+    fd = (loss_at(rho_np + h * v) - loss_at(rho_np - h * v)) / (2 * h)
+    dd = float((grads['cg'] * v).sum())
+    fd_check = {'direction': 'smoothed noise sigma 4 seed 7, minus 0.5',
+                'step': h, 'central_difference': fd,
+                'adjoint_directional': dd,
+                'rel_err': abs(fd - dd) / abs(dd)}
+    _log('directional derivative %.10g, central difference %.10g'
+         % (dd, fd))
+    summary = {
+        'value': runs['cg']['ratio'],
+        'method': 'cg (ls_solve default)',
+        'cg': runs['cg'],
+        'basic_scheme': runs['basic'],
+        'effective_stress': runs_es,
+        'grad_cg_vs_basic_max_rel': float(
+            np.abs(grads['cg'] - grads['basic']).max() /
+            np.abs(grads['basic']).max()),
+        'directional_fd': fd_check,
+    }
+    if emit:
+        _line('adjoint gradient time / forward solve time at %d^3 (1e-10), '
+              'conjugate gradients' % n,
+              runs['cg']['ratio'], 'ratio', {
+                  'workload': '%d^3 SIMP rho smoothed noise sigma 4 seed '
+                              '2014, mu=1+19 rho^3 nu 0.3, filtered Gamma0, '
+                              'mu0 10.5, E=e_xy, L=(C_1212-3)^2, tol 1e-10'
+                              % n,
+                  'summary': summary},
+              higher=False)
+    return summary
 
 
-def _ellipsoids_gpu(n, semi_axes, count, seed, labels=(1, 2)):
-    """Same draws and voxel rule as microstructure.ellipsoids, evaluated on
-    the GPU (the numpy version takes minutes at 512^3)."""
-    import torch
-    from janus_amd.microstructure import _random_rotation
-    rng = np.random.default_rng(seed)
-    phase = torch.zeros((n, n, n), dtype=torch.uint8, device='cuda')
-    r = max(semi_axes)
-    ax = torch.tensor(semi_axes, dtype=torch.float64, device='cuda')
-    for i in range(count):
-        c = rng.uniform(0., n, size=3)
-        rot = torch.from_numpy(_random_rotation(rng)).cuda()
-        idx = [torch.arange(int(np.floor(c[d] - r - 0.5)),
-                            int(np.ceil(c[d] + r + 0.5)) + 1, device='cuda')
-               for d in range(3)]
-        g = torch.meshgrid(*idx, indexing='ij')
-        x = torch.stack([g[d].double() + 0.5 - c[d] for d in range(3)],
-                        dim=-1) @ rot
-        inside = ((x / ax) ** 2).sum(dim=-1) <= 1.0
-        sel = tuple(torch.remainder(g[d][inside], n) for d in range(3))
-        phase[sel] = labels[i % len(labels)]
-    return phase
-
-
-def workload_large(args):
+def workload_large(args, emit=True):
     """512^3, three phases: matrix mu 1 and randomly oriented 1:1:4
     ellipsoids (semi-axes 8, 8, 32) of mu 10 and 0.1 (nu 0.3), filtered
     Gamma0, midpoint C0 (mu0 = 5.05).  Iterations/s of the six Mandel loads
     as one batch, and one adjoint gradient of L = C_eff[5, 5] (E = e_xy)
-    with respect to the three phase moduli, checked against a central
-    difference on the mu of phase 1."""
+    with respect to the three phase moduli, checked against effective_stress
+    and (--fd-voxels > 0) a central difference on the mu of phase 1."""
     import torch
     import janus_amd.fft.serial as fft
     import janus_amd.material.elastic.linear.isotropic as material
-    from janus_amd.autograd import iso_delta, ls_solve, mean_stress
+    from janus_amd import microstructure as ms
+    from janus_amd.autograd import (effective_stress, iso_delta, ls_solve,
+                                    mean_stress)
     from janus_amd.batch import BatchSolver
     from janus_amd.green import filtered
     from janus_amd.operators import BlockDiagonalOperator3D
     from janus_amd.solver import LSSolver, delta_stiffness
     n = args.grid or 512
-    vol = 4. / 3. * np.pi * 8 * 8 * 32 * (n / 512.) ** 3
-    count = int(-np.log(1 - 0.3) * n ** 3 / vol)
-    axes = tuple(a * n / 512. for a in (8., 8., 32.))
     t0 = time.perf_counter()
-    phase_d = _ellipsoids_gpu(n, axes, count, seed=2014)
+    phase_d, count, axes = ms.large_three_phase(n)
     frac = torch.bincount(phase_d.flatten().long(), minlength=3).double() \
         / phase_d.numel()
     _log('phase map %d^3 (%d ellipsoids) in %.1f s, fractions %s'
          % (n, count, time.perf_counter() - t0, frac.tolist()))
     phase = phase_d.cpu().numpy()
+    del phase_d
     mus = [1.0, 10.0, 0.1]
     mats = [material.create(m, 0.3, 3) for m in mus]
     mat0 = material.create(0.5 * (10.0 + 0.1), 0.3, 3)
@@ -1019,98 +1038,99 @@ def workload_large(args):
     # mu(x) = sum_p mu_p [phase(x) = p]: its backward is three reductions
     # (an indexed gather would scatter-add 134 M values into three bins)
     masks = [(ph == p).double() for p in range(len(mus))]
+    del ph
 
-    def loss_fn(mu_t, tol, max_iter, info=None):
+    def moduli(mu_t):
         g = masks[0] * mu_t[0]
         for p in range(1, len(mus)):
             g = g + masks[p] * mu_t[p]
-        nu = torch.full_like(g, 0.3)
-        sph, dev = iso_delta(g, nu, mat0, 3)
+        return iso_delta(g, torch.full_like(g, 0.3), mat0, 3)
+
+    def loss_fn(mu_t, tol, max_iter, info=None):
+        sph, dev = moduli(mu_t)
         eps = ls_solve(sph, dev, E, green, tol=tol, max_iter=max_iter,
                        info=info, method=args.method)
         return mean_stress(sph, dev, eps, E, mat0)[5]
 
-    mu_t = torch.tensor(mus, dtype=torch.float64, device='cuda',
-                        requires_grad=True)
-    info = {}
-    _sync()
-    t0 = time.perf_counter()
-    L = loss_fn(mu_t, 1e-10, args.max_iter, info)
-    _sync()
-    t1 = time.perf_counter()
-    L.backward()
-    _sync()
-    t2 = time.perf_counter()
-    grad = mu_t.grad.cpu().numpy()
-    _log('forward %.2f s (%d it), backward %.2f s (%d it), grad %s'
-         % (t1 - t0, info['forward'].iterations, t2 - t1,
-            info['adjoint'].iterations, grad))
-    # the same loss through effective_stress (dL/d<sigma> parallel to E:
-    # the backward is one contraction, no adjoint solve)
-    from janus_amd.autograd import effective_stress
-
     def loss_es(mu_t, tol, max_iter, info=None):
-        g = masks[0] * mu_t[0]
-        for p in range(1, len(mus)):
-            g = g + masks[p] * mu_t[p]
-        sph, dev = iso_delta(g, torch.full_like(g, 0.3), mat0, 3)
+        sph, dev = moduli(mu_t)
         return effective_stress(sph, dev, E, green, mat0, tol=tol,
                                 max_iter=max_iter, info=info,
                                 method=args.method)[5]
 
-    mu_es = torch.tensor(mus, dtype=torch.float64, device='cuda',
-                         requires_grad=True)
-    info_es = {}
-    _sync()
-    t3 = time.perf_counter()
-    L_es = loss_es(mu_es, 1e-10, args.max_iter, info_es)
-    _sync()
-    t4 = time.perf_counter()
-    L_es.backward()
-    _sync()
-    t5 = time.perf_counter()
-    grad_es = mu_es.grad.cpu().numpy()
+    def timed(fn):
+        mu_t = torch.tensor(mus, dtype=torch.float64, device='cuda',
+                            requires_grad=True)
+        info = {}
+        _sync()
+        t0 = time.perf_counter()
+        L = fn(mu_t, 1e-10, args.max_iter, info)
+        _sync()
+        t1 = time.perf_counter()
+        L.backward()
+        _sync()
+        t2 = time.perf_counter()
+        return mu_t.grad.cpu().numpy(), info, t1 - t0, t2 - t1, float(L.item())
+
+    grad, info, t_f, t_bk, Lv = timed(loss_fn)
+    _log('forward %.2f s (%d it), backward %.2f s (%d it), grad %s'
+         % (t_f, info['forward'].iterations, t_bk,
+            info['adjoint'].iterations, grad))
+    torch.cuda.empty_cache()
+    grad_es, info_es, t_fe, t_be, _ = timed(loss_es)
     _log('effective_stress: forward %.2f s, backward %.4f s, grad %s'
-         % (t4 - t3, t5 - t4, grad_es))
+         % (t_fe, t_be, grad_es))
     fd = None
     if args.fd_voxels > 0:
-        h = 1e-4
+        hh = 1e-4
         vals = []
         for sgn in (1, -1):
             m = torch.tensor(mus, dtype=torch.float64, device='cuda')
-            m[1] += sgn * h
+            m[1] += sgn * hh
             with torch.no_grad():
                 vals.append(loss_fn(m, 1e-10, args.max_iter).item())
-        d = (vals[0] - vals[1]) / (2 * h)
-        fd = {'phase': 1, 'step': h, 'fd': d, 'adjoint': float(grad[1]),
+        d = (vals[0] - vals[1]) / (2 * hh)
+        fd = {'phase': 1, 'step': hh, 'fd': d, 'adjoint': float(grad[1]),
               'rel_err': abs(d - grad[1]) / abs(grad[1])}
-    _line('6-load batch LS iterations/s at %d^3 FP64 (three phases)' % n,
-          round(its, 3), 'load-iterations/s', {
-              'workload': '%d^3, %d ellipsoids 1:1:4 (semi-axes %g %g %g), '
-                          'mu 1/10/0.1 nu 0.3, filtered Gamma0, mu0 5.05, '
-                          '6 Mandel unit loads as one batch, %d iterations'
-                          % (n, count, axes[0], axes[1], axes[2], k),
-              'phase_fractions': [round(x, 4) for x in frac.tolist()],
-              'B_alg_per_load_iteration': int(B),
-              'GBps_B_alg': round(B * its / 1e9, 1),
-              'roofline_frac_B_alg': round(B * its / 1e9 / HBM_PEAK_GBS, 4),
-              'adjoint': {
-                  'method': args.method,
-                  'loss': 'C_eff[5,5] under E=e_xy',
-                  'value': float(L.item()),
-                  'dL_dmu_phase': grad.tolist(),
-                  'seconds_forward': round(t1 - t0, 3),
-                  'seconds_backward': round(t2 - t1, 3),
-                  'forward_iterations': info['forward'].iterations,
-                  'adjoint_iterations': info['adjoint'].iterations,
-                  'fd_check': fd},
-              'effective_stress': {
-                  'seconds_forward': round(t4 - t3, 3),
-                  'seconds_backward': round(t5 - t4, 4),
-                  'ratio': round((t5 - t4) / (t4 - t3), 4),
-                  'dL_dmu_phase': grad_es.tolist(),
-                  'grad_vs_adjoint_max_rel': float(
-                      np.abs(grad_es - grad).max() / np.abs(grad).max())}})
+    del masks
+    torch.cuda.empty_cache()
+    summary = {
+        'value': round(its, 3),
+        'unit': 'load-iterations/s',
+        'iterations_per_load': k,
+        'B_alg_per_load_iteration': int(B),
+        'GBps_B_alg': round(B * its / 1e9, 1),
+        'roofline_frac_B_alg': round(B * its / 1e9 / HBM_PEAK_GBS, 4),
+        'phase_fractions': [round(x, 4) for x in frac.tolist()],
+        'adjoint': {
+            'method': args.method,
+            'loss': 'C_eff[5,5] under E=e_xy',
+            'value': Lv,
+            'dL_dmu_phase': grad.tolist(),
+            'seconds_forward': round(t_f, 3),
+            'seconds_backward': round(t_bk, 3),
+            'ratio': round(t_bk / t_f, 4),
+            'forward_iterations': info['forward'].iterations,
+            'adjoint_iterations': info['adjoint'].iterations,
+            'fd_check': fd},
+        'effective_stress': {
+            'seconds_forward': round(t_fe, 3),
+            'seconds_backward': round(t_be, 4),
+            'ratio': round(t_be / t_fe, 4),
+            'dL_dmu_phase': grad_es.tolist(),
+            'grad_vs_adjoint_max_rel': float(
+                np.abs(grad_es - grad).max() / np.abs(grad).max())},
+    }
+    if emit:
+        _line('6-load batch LS iterations/s at %d^3 FP64 (three phases)' % n,
+              round(its, 3), 'load-iterations/s', {
+                  'workload': '%d^3, %d ellipsoids 1:1:4 (semi-axes %g %g '
+                              '%g), mu 1/10/0.1 nu 0.3, filtered Gamma0, mu0 '
+                              '5.05, 6 Mandel unit loads as one batch, %d '
+                              'iterations' % (n, count, axes[0], axes[1],
+                                              axes[2], k),
+                  'summary': summary})
+    return summary
 
 
 def workload_fft(args):
@@ -1139,6 +1159,17 @@ def workload_fft(args):
             t.r2c(rd, cd)
             _sync()
             times.append(1e3 * (time.perf_counter() - t0))
+        back = torch.empty_like(rd)
+        t.c2r(cd, back)
+        times_inv = []
+        for _ in range(nd):
+            _sync()
+            t0 = time.perf_counter()
+            t.c2r(cd, back)
+            _sync()
+            times_inv.append(1e3 * (time.perf_counter() - t0))
+        roundtrip = float((back - rd).abs().max())
+        del back
         c = np.empty(t.oshape)
         t.r2c(r, c)
         host = []
@@ -1152,11 +1183,15 @@ def workload_fft(args):
             'device_ms_std': round(float(np.std(times)), 4),
             'device_GBps_in_plus_out': round(
                 nbytes / (np.mean(times) * 1e-3) / 1e9, 1),
+            'device_c2r_ms_mean': round(float(np.mean(times_inv)), 4),
+            'device_c2r_GBps_in_plus_out': round(
+                nbytes / (np.mean(times_inv) * 1e-3) / 1e9, 1),
+            'c2r_r2c_roundtrip_max_abs_err': roundtrip,
             'host_arrays_ms_mean': round(float(np.mean(host)), 2),
             'host_arrays_ms_std': round(float(np.std(host)), 2),
             'calls': [nd, nh]}
-        _log('r2c %s: device %.3f ms, numpy arrays %.1f ms'
-             % (shape, np.mean(times), np.mean(host)))
+        _log('r2c %s: device %.3f ms (c2r %.3f ms), numpy arrays %.1f ms'
+             % (shape, np.mean(times), np.mean(times_inv), np.mean(host)))
         del rd, cd
         torch.cuda.empty_cache()
     _line('r2c ms per call at 256^3 (janus fftw_python_benchmark_serial)',
@@ -1186,7 +1221,12 @@ def main():
                     help='keep the default plan variants and schedule')
     ap.add_argument('--no-cpu', action='store_true',
                     help='skip the CPU baseline leg')
-    ap.add_argument('--cpu-iters', type=int, default=2)
+    ap.add_argument('--cpu-iters', type=int, default=5,
+                    help='LS iterations of the CPU baseline (C restatement '
+                         'of janus on the same 256^3 workload)')
+    ap.add_argument('--no-configs', action='store_true',
+                    help='skip the batched, differentiable and 512^3 '
+                         'configurations of the default line')
     ap.add_argument('--schedule', default='auto',
                     choices=['auto', 'fused', 'five'],
                     help='iteration schedule (auto: the library default)')
@@ -1210,7 +1250,9 @@ def main():
     ap.add_argument('--per-group', action='store_true',
                     help='batched workload: also time each grid size alone')
     ap.add_argument('--adjoint-reps', type=int, default=2)
-    ap.add_argument('--fd-voxels', type=int, default=20)
+    ap.add_argument('--fd-voxels', type=int, default=0,
+                    help='large workload: > 0 adds a central difference '
+                         'on the mu of phase 1 (two more solves)')
     ap.add_argument('--large-iters', type=int, default=20)
     ap.add_argument('--method', default='cg', choices=['basic', 'cg'],
                     help='solver of the large workload\'s forward and '
@@ -1281,6 +1323,7 @@ def main():
         solver._handle, eps.data_ptr(), E.ctypes.data, args.profile_iters,
         ms.ctypes.data, _lib.current_stream()))
     fused = solver.fused
+    plan_kept = solver.green.plans()
     names = PASS_NAMES_FUSED if fused else PASS_NAMES
     nbytes = pass_bytes((n, n, n), fused)
     dom = int(np.argmax(ms[:5]))
@@ -1301,30 +1344,59 @@ def main():
     cpu = None
     cpu_all = None
     if not args.no_cpu:
-        print('bench: CPU baseline (%d iteration(s), 1 thread)' % args.cpu_iters,
-              file=sys.stderr, flush=True)
-        v, t = cpu_baseline(n, args.kind, phase, table, mat0, args.cpu_iters)
-        cpu = {'value': round(v, 6), 'unit': 'iterations/s', 'cores': 1,
-               'kind': 'port',
-               'sample': '%d LS iteration(s) of the same %d^3 workload with '
-                         'the numpy oracle (pocketfft), %.1f s'
-                         % (args.cpu_iters, n, t)}
-        workers = host_workers()
-        print('bench: CPU baseline (%d iteration(s), %d threads)'
-              % (args.cpu_iters, workers), file=sys.stderr, flush=True)
-        v, t = cpu_baseline_threads(n, args.kind, phase, table, mat0,
-                                    args.cpu_iters, workers)
-        cpu_all = {'value': round(v, 6), 'unit': 'iterations/s',
-                   'cores': workers, 'kind': 'port',
-                   'cpus_available': len(os.sched_getaffinity(0)),
-                   'cpu_model': cpu_model(),
-                   'sample': '%d LS iteration(s) of the same %d^3 workload '
-                             'with the oracle on %d threads (scipy pocketfft '
-                             'workers, slab thread pool), %.1f s'
-                             % (args.cpu_iters, n, workers, t)}
+        sample = ('%d LS iterations of the same %d^3 workload with the C '
+                  'restatement of janus\'s serial path (oracle/janus_serial.c: '
+                  'per-component r2c, janus\'s per-frequency Gamma-hat loop, '
+                  'c2r; own batched Stockham FFT, not FFTW)'
+                  % (args.cpu_iters, n))
+        try:
+            _log('CPU baseline (%d iterations, 1 thread)' % args.cpu_iters)
+            v, t, res = cpu_baseline(n, args.kind, phase, table, mat0,
+                                     args.cpu_iters, 1)
+            cpu = {'value': round(v, 6), 'unit': 'iterations/s', 'cores': 1,
+                   'kind': 'port', 'sample': sample + ', %.1f s' % t,
+                   'residuals': [float(x) for x in res]}
+            workers = host_workers()
+            _log('CPU baseline (%d iterations, %d threads)'
+                 % (args.cpu_iters, workers))
+            v, t, _ = cpu_baseline(n, args.kind, phase, table, mat0,
+                                   args.cpu_iters, workers)
+            cpu_all = {'value': round(v, 6), 'unit': 'iterations/s',
+                       'cores': workers, 'kind': 'port',
+                       'cpus_available': len(os.sched_getaffinity(0)),
+                       'cpu_model': cpu_model(),
+                       'sample': sample + ' on %d OpenMP threads, %.1f s'
+                                 % (workers, t)}
+        except RuntimeError as e:
+            _log('CPU baseline skipped: %s' % e)
 
+    configs = None
+    adjoint_ratio = None
+    if not args.no_configs:
+        # free the 256^3 solver's fields before the larger configurations
+        del solver, eps, r
+        import torch
+        torch.cuda.empty_cache()
+        configs = {}
+        for name, fn in (('differentiable_128', workload_adjoint),
+                         ('batched_mix', workload_batched),
+                         ('large_512', workload_large)):
+            _log('configuration %s' % name)
+            a = argparse.Namespace(**vars(args))
+            a.grid = None
+            t0 = time.perf_counter()
+            configs[name] = fn(a, emit=False)
+            configs[name]['seconds_total'] = round(time.perf_counter() - t0,
+                                                   1)
+            torch.cuda.empty_cache()
+        adjoint_ratio = configs['differentiable_128']['value']
+
+    traffic, traffic_src = read_traffic(
+        n, args.kind, names[dom],
+        16 if plan_kept[PASS_PLAN[names[dom]]] == 0 else 8)
     line = {
-        'metric': 'LS iterations/s at 256^3 FP64 (% HBM roofline)',
+        'metric': 'LS iterations/s at 256^3 FP64 (% HBM roofline); '
+                  'adjoint-grad/forward time',
         'value': round(its, 3),
         'unit': 'iterations/s',
         'n_gpus': 1,
@@ -1344,7 +1416,7 @@ def main():
             'grid': [n, n, n],
             'green': args.kind,
             'schedule': 'fused E+A' if fused else 'five passes',
-            'plans': solver.green.plans(),
+            'plans': plan_kept,
             'autotune_ms': tuned,
             'iteration_bytes_alg': int(iter_bytes),
             'iteration_GBps': round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
@@ -1364,13 +1436,13 @@ def main():
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4),
-            'traffic': read_traffic(
-                n, args.kind, names[dom],
-                16 if solver.green.get_plan(PASS_PLAN[names[dom]]) == 0
-                else 8),
+            'traffic': traffic,
+            'traffic_source': traffic_src,
         },
+        'adjoint_over_forward': adjoint_ratio,
         'cpu_baseline': cpu,
         'cpu_baseline_all_cores': cpu_all,
+        'configs': configs,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mb > 0:

@@ -386,7 +386,7 @@ int jx::green_ensure_device(jx_green *G) {
   if (G->spec) return JX_OK;
   int rc = green_prepare_device(G);
   if (rc) return rc;
-  JX_HIP(hipMalloc((void **)&G->spec, sizeof(double) * 2 * G->sym * G->g.Nh));
+  JX_HIP(hipMalloc((void **)&G->spec, sizeof(double) * spec_doubles(G->g, G->sym)));
   return JX_OK;
 }
 
@@ -407,10 +407,10 @@ static int green_passes(jx_green *G, double *spec, const double *x, double *y,
   int C = G->sym;
   int rc = launch_rows_forward(g, C, x, spec, loc, st, s);
   if (rc) return rc;
-  if (g.dim == 3 && (rc = launch_axis(g, C, 1, -1, spec, spec, nullptr, st, s)))
+  if (g.dim == 3 && (rc = launch_axis(g, C, plain_axis(g), -1, spec, spec, nullptr, st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, spec, spec, &G->dev, st, s))) return rc;
-  if (g.dim == 3 && (rc = launch_axis(g, C, 1, +1, spec, spec, nullptr, st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, spec, spec, &G->dev, st, s))) return rc;
+  if (g.dim == 3 && (rc = launch_axis(g, C, plain_axis(g), +1, spec, spec, nullptr, st, s)))
     return rc;
   return launch_rows_inverse(g, C, spec, y, ls, st, nb, s);
 }
@@ -428,7 +428,7 @@ int jx_green_apply(jx_green *G, const double *x, double *y, void *stream) {
 int jx_green_apply_static(jx_green *G, const double *x, double *y,
                           void *stream) {
   if (!G || !x || !y) return fail(JX_EINVAL, "jx_green_apply_static: NULL argument");
-  const long long need = (long long)sizeof(double) * 2 * G->sym * G->g.Nh;
+  const long long need = (long long)sizeof(double) * spec_doubles(G->g, G->sym);
   if (need > JX_STATIC_SCRATCH_BYTES)
     return fail(JX_EINVAL, "jx_green_apply_static: the grid needs " +
                                std::to_string(need) + " bytes of spectrum "
@@ -725,10 +725,10 @@ int jx::solver_iteration(jx_solver *S, double *eps, double *spec,
   if (first &&
       (rc = launch_rows_forward(g, C, eps, spec, S->local->dev, st, s)))
     return rc;
-  if (g.dim == 3 && (rc = launch_axis(g, C, 1, -1, spec, spec, nullptr, st, s)))
+  if (g.dim == 3 && (rc = launch_axis(g, C, plain_axis(g), -1, spec, spec, nullptr, st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, spec, spec, &G->dev, st, s))) return rc;
-  if (g.dim == 3 && (rc = launch_axis(g, C, 1, +1, spec, spec, nullptr, st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, spec, spec, &G->dev, st, s))) return rc;
+  if (g.dim == 3 && (rc = launch_axis(g, C, plain_axis(g), +1, spec, spec, nullptr, st, s)))
     return rc;
   return launch_rows_update_forward(g, C, spec, eps, ls, S->local->dev, st, nb,
                                     s);
@@ -784,11 +784,11 @@ int time_pass(jx_solver *S, int pass, double *eps, int iters, hipStream_t s,
         rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev, S->st, s);
         break;
       case PASS_AXIS:
-        rc = launch_axis(g, C, g.dim == 3 ? 1 : 0, -1, G->spec, G->spec,
+        rc = launch_axis(g, C, g.dim == 3 ? plain_axis(g) : 0, -1, G->spec, G->spec,
                          nullptr, S->st, s);
         break;
       case PASS_GAMMA:
-        rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s);
+        rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, S->st, s);
         break;
       case PASS_ROWS_INV:
         rc = launch_rows_inverse(g, C, G->spec, eps, ls, S->st, &nb, s);
@@ -1011,11 +1011,11 @@ int jx_solver_profile(jx_solver *S, double *eps, const double *E, int iters,
     (void)hipEventRecord(e[0], s);
     if (!fused) rc = launch_rows_forward(g, C, eps, G->spec, S->local->dev, S->st, s);
     (void)hipEventRecord(e[1], s);
-    if (!rc && g.dim == 3) rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s);
+    if (!rc && g.dim == 3) rc = launch_axis(g, C, plain_axis(g), -1, G->spec, G->spec, nullptr, S->st, s);
     (void)hipEventRecord(e[2], s);
-    if (!rc) rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s);
+    if (!rc) rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, S->st, s);
     (void)hipEventRecord(e[3], s);
-    if (!rc && g.dim == 3) rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s);
+    if (!rc && g.dim == 3) rc = launch_axis(g, C, plain_axis(g), +1, G->spec, G->spec, nullptr, S->st, s);
     (void)hipEventRecord(e[4], s);
     if (!rc)
       rc = fused ? launch_rows_update_forward(g, C, G->spec, eps, ls,

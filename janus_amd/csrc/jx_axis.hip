@@ -13,7 +13,28 @@ struct AxisArgs {
   int L, Ls, T, ntiles;
   int64_t stride_ax, n_outer, stride_outer, n_inner;
   int nh;  // last-axis half length, decodes the inner index in 3D pass C
+  int axis;  // the transformed axis (0, or 1 for a 3D Gamma-hat pass on axis 1)
 };
+
+// Full frequency index of the Gamma-hat pass from the outer index o, the
+// transformed-axis index e and the inner index m: axis 0 -- b = (e, m) in 2D,
+// (e, m / nh, m mod nh) in 3D; axis 1 (3D) -- b = (o, e, m).
+template <int DIM>
+__device__ __forceinline__ void gamma_index(int axis, int64_t o, int e,
+                                            int64_t m, int nh, int *b) {
+  if (DIM == 2) {
+    b[0] = e;
+    b[1] = (int)m;
+  } else if (axis == 0) {
+    b[0] = e;
+    b[1] = (int)(m / nh);
+    b[2] = (int)(m - (int64_t)b[1] * nh);
+  } else {
+    b[0] = (int)o;
+    b[1] = e;
+    b[2] = (int)m;
+  }
+}
 
 template <int C, int MODE>
 __global__ void __launch_bounds__(kThreads)
@@ -54,13 +75,7 @@ __global__ void __launch_bounds__(kThreads)
         int t = v / L, e = v - t * L;
         int64_t m = m0 + t;
         int b[3];
-        b[0] = e;
-        if (DIM == 2) {
-          b[1] = (int)m;
-        } else {
-          b[1] = (int)(m / a.nh);
-          b[2] = (int)(m - (int64_t)b[1] * a.nh);
-        }
+        gamma_index<DIM>(a.axis, o, e, m, a.nh, b);
         double g[NC];
         gamma_hat_moments<DIM>(G, b, g);
         double xr[C], xi[C], yr[C], yi[C];
@@ -96,36 +111,6 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Gamma-hat at frequency (b0 = e, inner index m) applied to the C complex
-// values z[c * zs] (real and imaginary parts separately).
-template <int C>
-__device__ __forceinline__ void gamma_point(const GreenDev &G, int e, int64_t m,
-                                            int nh, double2 *z, int zs) {
-  constexpr int DIM = C == 3 ? 2 : 3;
-  constexpr int NC = C == 3 ? 6 : 21;
-  int b[3];
-  b[0] = e;
-  if (DIM == 2) {
-    b[1] = (int)m;
-  } else {
-    b[1] = (int)(m / nh);
-    b[2] = (int)(m - (int64_t)b[1] * nh);
-  }
-  double g[NC];
-  gamma_hat_moments<DIM>(G, b, g);
-  double xr[C], xi[C], yr[C], yi[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    double2 v = z[c * zs];
-    xr[c] = v.x;
-    xi[c] = v.y;
-  }
-  sym_matvec<C>(g, xr, yr);
-  sym_matvec<C>(g, xi, yi);
-#pragma unroll
-  for (int c = 0; c < C; ++c) z[c * zs] = make_double2(yr[c], yi[c]);
-}
-
 // z <- D Gamma-hat D z for the C complex values z[c * zs] (real and
 // imaginary parts separately), D = diag(-1 on the Mandel components whose
 // bit is set in FLIP).  Negation is exact.
@@ -157,37 +142,39 @@ __device__ __forceinline__ void gamma_apply_flip(const double *g, double2 *z,
 }
 
 // Sign pattern of Gamma-hat(s k) = D Gamma-hat(k) D, D_(ij) = s_i s_j over
-// the Mandel components (3D: xx yy zz yz zx xy; 2D: xx yy xy), for k0
-// negated: zx and xy flip (2D: xy).
+// the Mandel components (3D: xx yy zz yz zx xy; 2D: xx yy xy): for k0
+// negated zx and xy flip (2D: xy); for k1 negated yz and xy flip.
 constexpr int kFlip0_3 = 0x30;
 constexpr int kFlip0_2 = 0x4;
+constexpr int kFlip1_3 = 0x28;
 
-// Gamma-hat at axis-0 frequencies e and L - e of inner index m, evaluated
-// once.  The truncated wave vector and the filtered alias set of L - e are
-// those of e with the axis-0 component negated (and the filtered weights
-// swapped), so Gamma-hat(L - e) = S Gamma-hat(e) S with S = -1 on the
-// Mandel components holding axis 0 once (zx, xy in 3D; xy in 2D) and +1
-// elsewhere.  This holds for every e except 0 and L / 2 (even L), which are
+// Gamma-hat at frequencies e and L - e of the transformed axis (outer
+// index o, inner index m), evaluated once.  The truncated wave vector and
+// the filtered alias set of L - e are those of e with that axis's component
+// negated (and the filtered weights swapped), so Gamma-hat(L - e) =
+// S Gamma-hat(e) S with S = -1 on the Mandel components holding the axis
+// once (axis 0: zx, xy in 3D, xy in 2D; axis 1: yz, xy) and +1 elsewhere
+// (Willot's wave vector negates the other two components instead, the same
+// Gamma0).  This holds for every e except 0 and L / 2 (even L), which are
 // their own mirrors; the caller passes z2 = nullptr for those.  Applied to
 // the C complex values at z0[c * zs] and, when z2 != nullptr, z2[c * zs].
 template <int C>
-__device__ __forceinline__ void gamma_point_pair(const GreenDev &G, int e,
-                                                 int64_t m, int nh, double2 *z0,
+__device__ __forceinline__ void gamma_point_pair(const GreenDev &G, int axis,
+                                                 int64_t o, int e, int64_t m,
+                                                 int nh, double2 *z0,
                                                  double2 *z2, int zs) {
   constexpr int DIM = C == 3 ? 2 : 3;
   constexpr int NC = C == 3 ? 6 : 21;
   int b[3];
-  b[0] = e;
-  if (DIM == 2) {
-    b[1] = (int)m;
-  } else {
-    b[1] = (int)(m / nh);
-    b[2] = (int)(m - (int64_t)b[1] * nh);
-  }
+  gamma_index<DIM>(axis, o, e, m, nh, b);
   double g[NC];
   gamma_hat_moments<DIM>(G, b, g);
   gamma_apply_flip<C, 0>(g, z0, zs);
-  if (z2 != nullptr) gamma_apply_flip<C, DIM == 2 ? kFlip0_2 : kFlip0_3>(g, z2, zs);
+  if (z2 != nullptr) {
+    if (DIM == 2) gamma_apply_flip<C, kFlip0_2>(g, z2, zs);
+    else if (axis == 0) gamma_apply_flip<C, kFlip0_3>(g, z2, zs);
+    else gamma_apply_flip<C, kFlip1_3>(g, z2, zs);
+  }
 }
 
 // Register-resident transform along a strided axis of the spectrum for the
@@ -247,6 +234,9 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
   constexpr int LS = reg_lds_stride(L);
   extern __shared__ double2 smem[];
   const int tid = threadIdx.x;
+  // sequence-fast lanes: a wave instruction reads whole rows of the tile
+  // (thread-fast lanes, free of LDS bank conflicts, measured 10-25% slower
+  // in every axis pass)
   const int s = tid % NSEQ, t = tid / NSEQ;
   const int tp = s / C, c = s - tp * C;
   const int64_t o = blk / a.ntiles;
@@ -282,7 +272,7 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
       int p = w / LH, e = w - p * LH;
       double2 *col = smem + (p * C) * LS;
       double2 *z2 = (e == 0 || 2 * e == L) ? nullptr : col + lp(L - e);
-      gamma_point_pair<C>(G, e, m0 + p, a.nh, col + lp(e), z2, LS);
+      gamma_point_pair<C>(G, a.axis, o, e, m0 + p, a.nh, col + lp(e), z2, LS);
     }
     __syncthreads();
 #pragma unroll
@@ -305,6 +295,19 @@ __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
 }
 
 // ============================================================ launchers
+
+// Strides of the padded 3D spectrum layout (spec_layout): axis 0 steps by
+// whole padded slabs, axis 1 passes take one slab per outer index.
+static void padded_strides(const Grid &g, int ncomp, int axis, AxisArgs &a) {
+  if (g.dim != 3) return;
+  const int64_t slab = g.n[1] * g.nh * 2 * ncomp + spec_layout(g, ncomp).pad;
+  if (axis == 0) {
+    a.stride_ax = slab;
+    a.stride_outer = g.n[0] * slab;
+  } else {
+    a.stride_outer = slab;
+  }
+}
 
 template <int C>
 static void launch_axis_c(int dir, dim3 grid, size_t lds, hipStream_t s,
@@ -372,6 +375,7 @@ static int launch_axis_reg_pn(const Grid &g, int ncomp, int axis, int dir,
   a.L = L;
   a.Ls = reg_lds_stride(L);
   a.nh = (int)g.nh;
+  a.axis = axis;
   int64_t inner = g.nh;
   for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
   int64_t outer = 1;
@@ -380,6 +384,7 @@ static int launch_axis_reg_pn(const Grid &g, int ncomp, int axis, int dir,
   a.n_outer = outer;
   a.stride_ax = inner * 2 * ncomp;
   a.stride_outer = (int64_t)L * inner * 2 * ncomp;
+  padded_strides(g, ncomp, axis, a);
   const int T = NSEQ / ncomp;
   a.T = T;
   a.ntiles = (int)((inner + T - 1) / T);
@@ -432,8 +437,8 @@ int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 hipStream_t s, const Batch *b) {
   if (axis < 0 || axis >= g.dim - 1)
     return fail(JX_EINVAL, "launch_axis: axis out of range");
-  if (dir == 0 && (axis != 0 || green == nullptr || ncomp == 1))
-    return fail(JX_EINVAL, "launch_axis: Gamma-hat pass needs axis 0");
+  if (dir == 0 && (axis != pivot_axis(g) || green == nullptr || ncomp == 1))
+    return fail(JX_EINVAL, "launch_axis: Gamma-hat pass needs the pivot axis");
   FftLen P;
   int rc = get_fft_len((int)g.n[axis], &P);
   if (rc) return rc;
@@ -444,6 +449,7 @@ int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
   a.L = (int)g.n[axis];
   a.Ls = lds_stride(a.L);
   a.nh = (int)g.nh;
+  a.axis = axis;
   // points after `axis` (all inner axes, last one halved) are contiguous
   int64_t inner = g.nh;
   for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
@@ -453,6 +459,7 @@ int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
   a.n_outer = outer;
   a.stride_ax = inner * 2 * ncomp;
   a.stride_outer = (int64_t)a.L * inner * 2 * ncomp;
+  padded_strides(g, ncomp, axis, a);
   int T = std::max(1, 1536 / (a.Ls * ncomp));
   if (T > inner) T = (int)inner;
   a.T = T;

@@ -173,12 +173,12 @@ int enqueue_chunk_iteration(jx_batch *B, const Group &gr, int c0, int n,
       (rc = launch_rows_forward(g, C, nullptr, nullptr, loc, nullptr, s, &b)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, -1, nullptr, nullptr, nullptr, nullptr, s, &b)))
+      (rc = launch_axis(g, C, plain_axis(g), -1, nullptr, nullptr, nullptr, nullptr, s, &b)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, nullptr, nullptr, &G->dev, nullptr, s, &b)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, nullptr, nullptr, &G->dev, nullptr, s, &b)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, +1, nullptr, nullptr, nullptr, nullptr, s, &b)))
+      (rc = launch_axis(g, C, plain_axis(g), +1, nullptr, nullptr, nullptr, nullptr, s, &b)))
     return rc;
   if (gr.fused)
     rc = launch_rows_update_forward(g, C, nullptr, nullptr, ls, loc, nullptr,
@@ -497,7 +497,7 @@ int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
     if (o.eps == eps)
       return fail(JX_EINVAL, "jx_batch_add: each entry needs its own eps");
   int nb = rows_inverse_nblocks(G->g, C);
-  hipError_t he = hipMalloc((void **)&e.spec, sizeof(double) * 2 * C * G->g.Nh);
+  hipError_t he = hipMalloc((void **)&e.spec, sizeof(double) * spec_doubles(G->g, C));
   if (he == hipSuccess) he = hipMalloc((void **)&e.partials, sizeof(double) * nb);
   if (he == hipSuccess)
     he = hipMalloc((void **)&e.res_hist, sizeof(double) * max_iter);

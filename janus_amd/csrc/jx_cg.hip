@@ -33,12 +33,12 @@ int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
                                st, s);
   if (rc) return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), -1, G->spec, G->spec, nullptr, st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, st, s)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), +1, G->spec, G->spec, nullptr, st, s)))
     return rc;
   RowsInverseLs plain;
   return launch_rows_inverse(g, C, G->spec, y, plain, st, nullptr, s);
@@ -329,12 +329,12 @@ int cg_iteration_fused(jx_solver *S, double *x, double wd, double wt,
   int rc = launch_rows_forward_cg(g, C, cg, G->spec, S->local->dev, S->st, s);
   if (rc) return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), -1, G->spec, G->spec, nullptr, S->st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, S->st, s)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), +1, G->spec, G->spec, nullptr, S->st, s)))
     return rc;
   int nb = 0;
   if ((rc = launch_rows_inverse_cg(g, C, G->spec, q, p, S->partials, wd, wt,
@@ -434,12 +434,12 @@ int cg_pol_iteration_fused(jx_solver *S, double *x, hipStream_t s) {
   int rc = launch_rows_forward_cg(g, C, cg, G->spec, none, S->st, s);
   if (rc) return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, -1, G->spec, G->spec, nullptr, S->st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), -1, G->spec, G->spec, nullptr, S->st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, 0, 0, G->spec, G->spec, &G->dev, S->st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, S->st, s)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, 1, +1, G->spec, G->spec, nullptr, S->st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), +1, G->spec, G->spec, nullptr, S->st, s)))
     return rc;
   RowsInverseLs plain;
   if ((rc = launch_rows_inverse(g, C, G->spec, q, plain, S->st, nullptr, s)))

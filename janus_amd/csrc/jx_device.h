@@ -160,7 +160,15 @@ struct RowsArgs {
   int64_t nrows;
   int L, nh, Ls, RB;
   double scale;
+  SpecPad sp;  // spectrum layout (padded 3D slabs)
 };
+
+// Offset in doubles of spectrum row `row` (rowlen doubles per row) in the
+// padded layout of spec_layout().
+__device__ __forceinline__ int64_t spec_row_off(int64_t row, int64_t rowlen,
+                                                const SpecPad &sp) {
+  return row * rowlen + (sp.pad ? (row / sp.slab_rows) * sp.pad : 0);
+}
 
 __device__ __forceinline__ bool solve_done(const SolveState *st) {
   return st != nullptr && *((volatile const int *)&st->done) != 0;
@@ -292,6 +300,7 @@ __device__ __forceinline__ double c0_dot(const double *a, const double *b,
 // count would wait for each round's loads in turn).
 template <int L, int RB, int NT, int LS>
 __device__ __forceinline__ void load_half_spectra6(const double *spec,
+                                                   const SpecPad &lay,
                                                    int64_t row0, int rb,
                                                    int tid, double2 *smem) {
   constexpr int NH = L / 2 + 1;
@@ -306,7 +315,8 @@ __device__ __forceinline__ void load_half_spectra6(const double *spec,
     const int rest = w / 3, r = rest / NH;
     if (w < ITEMS && r < rb) {
       const int cp = w - rest * 3, kk = rest - r * NH;
-      const double *sp = spec + ((row0 + r) * NH + kk) * 12 + 4 * cp;
+      const double *sp =
+          spec + spec_row_off(row0 + r, NH * 12, lay) + kk * 12 + 4 * cp;
       xa[k] = *(const double2 *)sp;
       xb[k] = *(const double2 *)(sp + 2);
     }
@@ -327,6 +337,24 @@ __device__ __forceinline__ void load_half_spectra6(const double *spec,
       }
     }
   }
+}
+
+// Lane order of the register FFT kernels of the last axis: sequence fastest
+// (lane = t * sequences + p) or thread-in-sequence fastest (lane = p * TPP +
+// t: the stage exchanges through LDS touch consecutive 16-byte slots, free
+// of bank conflicts).  JX_ROWS_TFAST 0: always the first, 1: always the
+// second, 2 (default) -- thread-fast for the mixed-radix plans
+// (80, 96, 160: 20- and 12-value plans, where it measured faster, e.g. the
+// 160-point fused pass 0.224 against 0.284 ms at 160^3), sequence-fast for
+// the power-of-two plans (the 8-value 256- and 512-point passes measured
+// slower thread-fast).
+#ifndef JX_ROWS_TFAST
+#define JX_ROWS_TFAST 2
+#endif
+template <class P>
+constexpr bool rows_tfast() {
+  return JX_ROWS_TFAST == 1 ||
+         (JX_ROWS_TFAST == 2 && (P::L & (P::L - 1)) != 0);
 }
 
 // Twiddle source of the last-axis kernels: quarter table in LDS (1) or the

@@ -137,6 +137,44 @@ struct Grid {
 
 Grid make_grid(int dim, const int64_t *shape);
 
+// The strided axis whose pass applies Gamma-hat between its forward and
+// inverse transforms (the pivot, transformed last and inverted first), and
+// in 3D the other strided axis, whose passes only transform.  Either order
+// gives the same operator; the pivot's pass is the heaviest of the
+// iteration, and on axis 1 its tiles stay within one axis-0 slab.
+#ifndef JX_PIVOT_AXIS
+#define JX_PIVOT_AXIS 0
+#endif
+inline int pivot_axis(const Grid &g) { return g.dim == 3 ? JX_PIVOT_AXIS : 0; }
+inline int plain_axis(const Grid &g) { return 1 - pivot_axis(g); }
+
+// Layout of the solver spectra (ncomp complex values per frequency): rows
+// (i0[, i1]) of nh frequencies, C order.  In 3D each i0 slab of n1 rows is
+// followed by `pad` unused doubles when its size is a multiple of 4 KB: the
+// axis-0 transforms stride by whole slabs, and slab strides aligned to 4 KB
+// (256^3: 387 x 8 KB; 512^3: 771 x 16 KB) concentrate their row accesses on
+// few HBM channels -- a probe of the axis-0 pass's access pattern at 256^3
+// moved 3.76 TB/s unpadded and 4.29 TB/s with 8448 bytes of padding
+// (two-point tiles; four-point tiles 4.53 and 4.98).  ncomp = 1 (the FFT
+// API's janus layout) is never padded.
+struct SpecPad {
+  int64_t slab_rows = 1;  // rows per padded slab (n1 in 3D)
+  int64_t pad = 0;        // doubles after each slab
+};
+inline SpecPad spec_layout(const Grid &g, int ncomp) {
+  SpecPad sp;
+  if (g.dim != 3 || ncomp == 1) return sp;
+  sp.slab_rows = g.n[1];
+  const int64_t slab_bytes = g.n[1] * g.nh * ncomp * 16;
+  sp.pad = (slab_bytes % 4096 == 0) ? 8448 / 8 : 0;
+  return sp;
+}
+// Doubles of a spectrum of the grid (padding included).
+inline int64_t spec_doubles(const Grid &g, int ncomp) {
+  const SpecPad sp = spec_layout(g, ncomp);
+  return 2 * ncomp * g.Nh + (sp.pad ? g.n[0] * sp.pad : 0);
+}
+
 // Twiddle / plan cache (per device, shared by all handles).
 int get_fft_len(int L, FftLen *out);
 

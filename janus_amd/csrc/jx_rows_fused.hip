@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
                                              : JX_FUSED8_MIN_WAVES)
                                       : 1)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
-                              double scale, const double2 *W, LsArgs ls,
+                              SpecPad lay, double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st,
                               BatchRef br) {
   constexpr int L = P::L;
@@ -73,12 +73,13 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   const int tid = threadIdx.x;
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
-  const int p = tid % NP, t = tid / NP;
+  const int p = rows_tfast<P>() ? tid / TPP : tid % NP;
+  const int t = rows_tfast<P>() ? tid % TPP : tid / NP;
   const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
 
   // ---- inverse: half spectra into the pair slots (see k_rows_inverse_reg)
   if constexpr (C == 6) {
-    load_half_spectra6<L, RB, NT, LS>(spec, row0, rb, tid, smem);
+    load_half_spectra6<L, RB, NT, LS>(spec, lay, row0, rb, tid, smem);
   } else {
     for (int w = tid; w < RB * NH * C; w += NT) {
       int c = w % C;
@@ -89,9 +90,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
       int part = q & 1;
       double re = 0., im = 0.;
       if (r < rb) {
-        int64_t pt = (row0 + r) * NH + k;
-        re = spec[pt * 2 * C + 2 * c];
-        im = spec[pt * 2 * C + 2 * c + 1];
+        const double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
+        re = sr[2 * c];
+        im = sr[2 * c + 1];
       }
       double2 *Z = smem + (q >> 1) * LS;
       if (k == 0 || 2 * k == L)
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
       double2 zm = Z[lp(k == 0 ? 0 : L - k)];
       double2 xa = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
       double2 xb = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
-      double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+      double *sp = spec + spec_row_off(row0 + r, NH * 12, lay) + k * 12 + 4 * cp;
       *(double2 *)sp = xa;
       *(double2 *)(sp + 2) = xb;
     }
@@ -255,9 +256,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
         re = 0.5 * (zk.y + zm.y);
         im = -0.5 * (zk.x - zm.x);
       }
-      int64_t pt = (row0 + r) * NH + k;
-      spec[pt * 2 * C + 2 * c] = re;
-      spec[pt * 2 * C + 2 * c + 1] = im;
+      double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
+      sr[2 * c] = re;
+      sr[2 * c + 1] = im;
     }
   }
 }
@@ -283,8 +284,8 @@ static int launch_rows_update_forward_reg_lc(const Grid &g, double *spec,
     double scale = 1.0 / (double)g.N;
     auto go = [&](auto kernel) {
       set_lds_attr(kernel, lds);
-      hipLaunchKernelGGL(kernel, grid, block, lds, s, spec, y, g.rows, scale, W,
-                         la, loc, st, br);
+      hipLaunchKernelGGL(kernel, grid, block, lds, s, spec, y, g.rows,
+                         spec_layout(g, C), scale, W, la, loc, st, br);
     };
     const bool mat = local_family(loc) == LF_MATRIX;
     if (la.rhs && mat)

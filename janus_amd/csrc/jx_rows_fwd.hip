@@ -72,9 +72,10 @@ __global__ void __launch_bounds__(kThreads)
       re = 0.5 * (zk.y + zm.y);
       im = -0.5 * (zk.x - zm.x);
     }
-    int64_t pt = (row0 + r) * nh + k;
-    spec[pt * 2 * C + 2 * c] = re;
-    spec[pt * 2 * C + 2 * c + 1] = im;
+    double *sr = spec + spec_row_off(row0 + r, (int64_t)nh * 2 * C, a.sp) +
+                 (int64_t)k * 2 * C;
+    sr[2 * c] = re;
+    sr[2 * c + 1] = im;
   }
 }
 
@@ -138,7 +139,7 @@ __device__ __forceinline__ void cg_update_voxel(const CgDev &cg, int64_t vox,
 template <class P, int C, int NP, int LOCAL, bool CGP>
 __global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_forward_reg(const double *__restrict__ x, double *__restrict__ spec,
-                       int64_t nrows, const double2 *W, LocalDev loc,
+                       int64_t nrows, SpecPad lay, const double2 *W, LocalDev loc,
                        const SolveState *st, BatchRef br, CgDev cg) {
   constexpr int L = P::L;
   const Slot *sl;
@@ -219,7 +220,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     }
   }
   __syncthreads();
-  const int p = tid % NP, t = tid / NP;
+  const int p = rows_tfast<P>() ? tid / reg_tpp<P>() : tid % NP;
+  const int t = rows_tfast<P>() ? tid % reg_tpp<P>() : tid / NP;
   double2 *seq = smem + p * LS;
   double2 v[V];
 #pragma unroll
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
       double2 zm = Z[lp(k == 0 ? 0 : L - k)];
       double2 xa = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
       double2 xb = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
-      double *sp = spec + ((row0 + r) * NH + k) * 12 + 4 * cp;
+      double *sp = spec + spec_row_off(row0 + r, NH * 12, lay) + k * 12 + 4 * cp;
       *(double2 *)sp = xa;
       *(double2 *)(sp + 2) = xb;
     }
@@ -267,9 +269,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
       re = 0.5 * (zk.y + zm.y);
       im = -0.5 * (zk.x - zm.x);
     }
-    int64_t pt = (row0 + r) * NH + k;
-    spec[pt * 2 * C + 2 * c] = re;
-    spec[pt * 2 * C + 2 * c + 1] = im;
+    double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
+    sr[2 * c] = re;
+    sr[2 * c + 1] = im;
   }
 }
 
@@ -293,8 +295,8 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
   const CgDev cgd = cg ? *cg : CgDev();
   auto go = [&](auto kernel) {
     set_lds_attr(kernel, lds);
-    hipLaunchKernelGGL(kernel, grid, block, lds, s, x, spec, g.rows, W, loc, st,
-                       br, cgd);
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, x, spec, g.rows,
+                       spec_layout(g, C), W, loc, st, br, cgd);
   };
   // a batch always applies the local operator of its slots (all of one
   // family: the first slot's, which the caller passes as loc)
@@ -365,6 +367,7 @@ int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
   a.Ls = lds_stride(a.L);
   a.RB = rows_per_block(a.L, ncomp);
   a.scale = 1.0;
+  a.sp = spec_layout(g, ncomp);
   size_t lds = rows_lds_bytes(a.L, ncomp, a.RB);
   if ((rc = check_lds(lds, "rows_forward"))) return rc;
   int64_t nb = (g.rows + a.RB - 1) / a.RB;

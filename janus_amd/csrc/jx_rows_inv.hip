@@ -37,9 +37,10 @@ __global__ void __launch_bounds__(kThreads)
     int r = rest / nh;
     int q = r * C + c;
     int part = q & 1;
-    int64_t pt = (row0 + r) * nh + k;
-    double re = spec[pt * 2 * C + 2 * c];
-    double im = spec[pt * 2 * C + 2 * c + 1];
+    const double *sr = spec + spec_row_off(row0 + r, (int64_t)nh * 2 * C, a.sp) +
+                       (int64_t)k * 2 * C;
+    double re = sr[2 * c];
+    double im = sr[2 * c + 1];
     double2 *Z = bufA + (q >> 1) * a.Ls;
     bool lonely = (!part) && (q + 1 == Q);
     if (k == 0 || 2 * k == L) {
@@ -132,7 +133,7 @@ __device__ __forceinline__ double cg_qp_voxel(const LsArgs &ls, double *q,
 template <class P, int C, int NP, int MODE>
 __global__ void __launch_bounds__(NP * reg_tpp<P>())
     k_rows_inverse_reg(const double *__restrict__ spec, double *y,
-                       int64_t nrows, double scale, const double2 *W,
+                       int64_t nrows, SpecPad lay, double scale, const double2 *W,
                        LsArgs ls, const SolveState *st, BatchRef br) {
   constexpr int L = P::L;
   const Slot *sl;
@@ -162,7 +163,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   const int tid = threadIdx.x;
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);
-  const int p = tid % NP, t = tid / NP;
+  const int p = rows_tfast<P>() ? tid / TPP : tid % NP;
+  const int t = rows_tfast<P>() ? tid % TPP : tid / NP;
   __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
   const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
   // half spectra into the pair slots: slot k <- Xa[k], slot L-k <- Xb[k]
@@ -170,7 +172,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   // Nyquist slots <- (Re Xa, Re Xb), their imaginary parts dropped as
   // FFTW's and pocketfft's c2r do
   if constexpr (C == 6) {
-    load_half_spectra6<L, RB, NT, LS>(spec, row0, rb, tid, smem);
+    load_half_spectra6<L, RB, NT, LS>(spec, lay, row0, rb, tid, smem);
   } else
   for (int w = tid; w < RB * NH * C; w += NT) {
     int c = w % C;
@@ -181,9 +183,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     int part = q & 1;
     double re = 0., im = 0.;
     if (r < rb) {
-      int64_t pt = (row0 + r) * NH + k;
-      re = spec[pt * 2 * C + 2 * c];
-      im = spec[pt * 2 * C + 2 * c + 1];
+      const double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
+      re = sr[2 * c];
+      im = sr[2 * c + 1];
     }
     double2 *Z = smem + (q >> 1) * LS;
     if (k == 0 || 2 * k == L)
@@ -325,11 +327,11 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
       if (la.rhs) {
         set_lds_attr(k_rows_inverse_reg<P, C, NP, 2>, lds);
         hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 2>), grid, block,
-                           lds, s, spec, y, g.rows, scale, W, la, st, br);
+                           lds, s, spec, y, g.rows, spec_layout(g, C), scale, W, la, st, br);
       } else {
         set_lds_attr(k_rows_inverse_reg<P, C, NP, 1>, lds);
         hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 1>), grid, block,
-                           lds, s, spec, y, g.rows, scale, W, la, st, br);
+                           lds, s, spec, y, g.rows, spec_layout(g, C), scale, W, la, st, br);
       }
     } else {
       return fail(JX_EINVAL, "fixed point needs sym components");
@@ -337,7 +339,7 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
   } else {
     set_lds_attr(k_rows_inverse_reg<P, C, NP, 0>, lds);
     hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 0>), grid, block, lds,
-                       s, spec, y, g.rows, scale, W, la, st, br);
+                       s, spec, y, g.rows, spec_layout(g, C), scale, W, la, st, br);
   }
   JX_HIP(hipGetLastError());
   return JX_OK;
@@ -367,7 +369,7 @@ static int launch_rows_inverse_cg_lc(const Grid &g, const double *spec,
     double scale = 1.0 / (double)g.N;
     set_lds_attr(k_rows_inverse_reg<P, C, NP, 3>, lds);
     hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 3>), dim3((unsigned)nb),
-                       dim3(NP * reg_tpp<P>()), lds, s, spec, q, g.rows, scale,
+                       dim3(NP * reg_tpp<P>()), lds, s, spec, q, g.rows, spec_layout(g, C), scale,
                        W, la, st, batch_ref(nullptr, nb));
     JX_HIP(hipGetLastError());
     return JX_OK;
@@ -422,6 +424,7 @@ int launch_rows_inverse(const Grid &g, int ncomp, const double *spec, double *y,
   a.Ls = lds_stride(a.L);
   a.RB = rows_per_block(a.L, ncomp);
   a.scale = 1.0 / (double)g.N;
+  a.sp = spec_layout(g, ncomp);
   size_t lds = rows_lds_bytes(a.L, ncomp, a.RB);
   if ((rc = check_lds(lds, "rows_inverse"))) return rc;
   int64_t nb = (g.rows + a.RB - 1) / a.RB;

@@ -640,7 +640,7 @@ def cg_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400, h=1.0,
 
 
 def cg_polarization_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400,
-                           h=1.0, res_scale=None, mats=None):
+                           h=1.0, res_scale=None, mats=None, dot=None):
     """Conjugate gradients on the polarization form, as
     examples/square_inclusion.py:17-52, 77-85 runs PETSc CG (KSP 'cg', PC
     'none', rtol): (L + Gamma0) tau = E, where the local operator L (sph,
@@ -648,7 +648,10 @@ def cg_polarization_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400,
     the Euclidean inner product from tau = 0.  The residual after
     iteration k is ||E - A tau_k|| / (sqrt(N) ||E||) (PETSc's relative
     residual; ``res_scale`` replaces the normalization).  Stops after the
-    first iteration below tol.  Returns (tau, eps = L tau, residuals)."""
+    first iteration below tol.  ``dot(a, b)`` replaces the Euclidean dot
+    product (default: numpy's pairwise sum of a * b); other summation orders
+    give the rounding spread of the scheme.  Returns (tau, eps = L tau,
+    residuals)."""
     E = np.asarray(E, dtype=np.float64)
     shape = _grid_shape(sph, mats)
     local = _local(sph, dev, mats)
@@ -661,24 +664,28 @@ def cg_polarization_scheme(sph, dev, E, kind, g0, nu0, tol=0.0, max_iter=400,
     def op(v):
         return local(v) + green_apply(v, kind, g0, nu0, h)
 
+    if dot is None:
+        def dot(a, b):
+            return np.sum(a * b)
+
     r = np.empty(shape + (E.size,), dtype=np.float64)
     r[...] = E
     x = np.zeros_like(r)
     p = r.copy()
-    rr = np.sum(r * r)
+    rr = dot(r, r)
     res = []
     if rr == 0.:
         return x, local(x), np.zeros(0)
     for _ in range(max_iter):
         q = op(p)
-        alpha = rr / np.sum(p * q)
+        alpha = rr / dot(p, q)
         x = x + alpha * p
         r = r - alpha * q
         res_k = normalization * np.linalg.norm(r)
         res.append(res_k)
         if res_k < tol:
             break
-        rr_new = np.sum(r * r)
+        rr_new = dot(r, r)
         beta = rr_new / rr
         p = r + beta * p
         rr = rr_new

@@ -618,6 +618,36 @@ def test_cg_polarization_equals_strain_form(kind):
                                rtol=0, atol=1e-12 * np.abs(tau).max())
 
 
+def test_cg_polarization_rounding_floor():
+    """The polarization form's conditioning sets a floor under any parity
+    bar.  At the GPU test's 3D configuration (16^3 spheres, filtered
+    operator, C0 just below the matrix: g0 = 0.9 against g = 1, so L =
+    (C - C0)^-1 is large), two float64 runs that differ only in the summation
+    order of the Euclidean dot products -- numpy's pairwise sum and a plain
+    sequential sum, equally exact -- end more than 1e-12 apart in tau after
+    their 80-odd iterations (and less than 1e-9).  Matching janus's
+    restatement to 1e-12 would need the very same summation order, which a
+    parallel reduction does not have."""
+    from janus_amd import microstructure as ms
+    g0 = 0.9
+    phase = ms.spheres(16, 3.0, 0.2, seed=3)
+    inv = np.array([_inverse_iso(g, nu, g0, 0.3, 3)
+                    for g, nu in ((1.0, 0.3), (10.0, 0.2))])
+    E = np.random.default_rng(16).standard_normal(6)
+
+    def sequential(a, b):
+        return float(np.cumsum((a * b).ravel())[-1])
+
+    runs = [jo.cg_polarization_scheme(inv[phase, 0], inv[phase, 1], E,
+                                      'filtered', g0, 0.3, tol=1e-10,
+                                      max_iter=500, dot=d)
+            for d in (None, sequential)]
+    (tau_a, _, res_a), (tau_b, _, res_b) = runs
+    assert res_a[-1] < 1e-10 and res_b[-1] < 1e-10
+    spread = np.abs(tau_a - tau_b).max() / np.abs(tau_a).max()
+    assert 1e-12 < spread < 1e-9, spread
+
+
 def test_c0_inner_matches_matrix():
     rng = np.random.default_rng(3)
     for dim, sym in ((2, 3), (3, 6)):
