@@ -1253,7 +1253,10 @@ def main():
     ap.add_argument('--fd-voxels', type=int, default=0,
                     help='large workload: > 0 adds a central difference '
                          'on the mu of phase 1 (two more solves)')
-    ap.add_argument('--large-iters', type=int, default=20)
+    ap.add_argument('--large-iters', type=int, default=60,
+                    help='large workload: iterations of the timed 6-load '
+                         'batch (its first iteration also runs the unfused '
+                         'pass A)')
     ap.add_argument('--method', default='cg', choices=['basic', 'cg'],
                     help='solver of the large workload\'s forward and '
                          'adjoint solves')
