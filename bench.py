@@ -1398,8 +1398,8 @@ def main():
         n, args.kind, names[dom],
         16 if plan_kept[PASS_PLAN[names[dom]]] == 0 else 8)
     line = {
-        'metric': 'LS iterations/s at 256^3 FP64 (% HBM roofline); '
-                  'adjoint-grad/forward time',
+        'metric': 'LS iterations/s at %d^3 FP64 (%% HBM roofline); '
+                  'adjoint-grad/forward time' % n,
         'value': round(its, 3),
         'unit': 'iterations/s',
         'n_gpus': 1,

@@ -17,6 +17,9 @@
  * sym = 3 (2D, Mandel order xx, yy, sqrt2 xy) or 6 (3D, xx, yy, zz, sqrt2 yz,
  * sqrt2 zx, sqrt2 xy) -- janus/mandelvoigt.py:26-40.
  *
+ * Device arrays passed to the compute calls must be 16-byte aligned (any
+ * hipMalloc or torch allocation is); a misaligned one gets JX_EINVAL.
+ *
  * Status codes: JX_OK (0) or a negative JX_E* value; jx_last_error() returns
  * a thread-local message for the last failure of the calling thread.
  * Handles are not thread safe (they own scratch memory), as in janus.
@@ -87,8 +90,9 @@ int jx_green_apply(jx_green *green, const double *x, double *y, void *stream);
  * kernels and results; the spectrum scratch is a file-scope static device
  * array of 64 MiB that each GPU holds until the process ends, instead of
  * the handle's own buffer.  The call makes no hipMalloc, hipFree or other
- * allocating call: the grid's spectrum, 16 sym (n0 n1 [n2 -- the last axis
- * halved] + 1) bytes, must fit the 64 MiB (3D up to about 100^3, 2D up to
+ * allocating call: the grid's spectrum, 16 sym n0 n1 [n2] / n_last (n_last / 2
+ * + 1) bytes (in 3D plus 8448 bytes after each n0 slab whose size is a
+ * multiple of 4 KB), must fit the 64 MiB (3D up to about 100^3, 2D up to
  * 1024^2), else JX_EINVAL; the handle must have been created with a GPU
  * present and this device current (its tables and FFT plans are made by
  * jx_green_create then), else JX_EINVAL.  Every call on one GPU shares the

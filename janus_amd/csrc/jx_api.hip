@@ -207,6 +207,8 @@ int jx_fft_destroy(jx_fft *f) {
 
 int jx_fft_r2c(jx_fft *f, const double *r, double *c, void *stream) {
   if (!f || !r || !c) return fail(JX_EINVAL, "jx_fft_r2c: NULL argument");
+  JX_ALIGNED(r, "jx_fft_r2c");
+  JX_ALIGNED(c, "jx_fft_r2c");
   hipStream_t s = (hipStream_t)stream;
   LocalDev none;
   int rc = launch_rows_forward(f->g, 1, r, c, none, nullptr, s);
@@ -218,6 +220,8 @@ int jx_fft_r2c(jx_fft *f, const double *r, double *c, void *stream) {
 
 int jx_fft_c2r(jx_fft *f, const double *c, double *r, void *stream) {
   if (!f || !r || !c) return fail(JX_EINVAL, "jx_fft_c2r: NULL argument");
+  JX_ALIGNED(r, "jx_fft_c2r");
+  JX_ALIGNED(c, "jx_fft_c2r");
   hipStream_t s = (hipStream_t)stream;
   int rc;
   const double *src = c;
@@ -417,6 +421,8 @@ static int green_passes(jx_green *G, double *spec, const double *x, double *y,
 
 int jx_green_apply(jx_green *G, const double *x, double *y, void *stream) {
   if (!G || !x || !y) return fail(JX_EINVAL, "jx_green_apply: NULL argument");
+  JX_ALIGNED(x, "jx_green_apply");
+  JX_ALIGNED(y, "jx_green_apply");
   int rc = green_ensure_device(G);
   if (rc) return rc;
   LocalDev none;
@@ -428,6 +434,8 @@ int jx_green_apply(jx_green *G, const double *x, double *y, void *stream) {
 int jx_green_apply_static(jx_green *G, const double *x, double *y,
                           void *stream) {
   if (!G || !x || !y) return fail(JX_EINVAL, "jx_green_apply_static: NULL argument");
+  JX_ALIGNED(x, "jx_green_apply_static");
+  JX_ALIGNED(y, "jx_green_apply_static");
   const long long need = (long long)sizeof(double) * spec_doubles(G->g, G->sym);
   if (need > JX_STATIC_SCRATCH_BYTES)
     return fail(JX_EINVAL, "jx_green_apply_static: the grid needs " +
@@ -620,12 +628,16 @@ int jx_local_destroy(jx_local *l) {
 
 int jx_local_apply(jx_local *l, const double *x, double *y, void *stream) {
   if (!l || !x || !y) return fail(JX_EINVAL, "jx_local_apply: NULL argument");
+  JX_ALIGNED(x, "jx_local_apply");
+  JX_ALIGNED(y, "jx_local_apply");
   return launch_local_apply(l->g, l->sym, l->dev, x, y, (hipStream_t)stream);
 }
 
 int jx_local_apply_transpose(jx_local *l, const double *x, double *y,
                              void *stream) {
   if (!l || !x || !y) return fail(JX_EINVAL, "jx_local_apply_transpose: NULL argument");
+  JX_ALIGNED(x, "jx_local_apply_transpose");
+  JX_ALIGNED(y, "jx_local_apply_transpose");
   return launch_local_apply(l->g, l->sym, l->dev, x, y, (hipStream_t)stream, true);
 }
 
@@ -646,6 +658,7 @@ int jx_block_linear_apply(int dim, const int64_t *shape, int m, int n,
 int jx_local_means(jx_local *l, const double *x, double *out_tau, double *out_eps,
                    void *stream) {
   if (!l || !x) return fail(JX_EINVAL, "jx_local_means: NULL argument");
+  JX_ALIGNED(x, "jx_local_means");
   hipStream_t s = (hipStream_t)stream;
   int C = l->sym;
   double *partials = l->partials;
@@ -874,6 +887,8 @@ int jx_solver_solve(jx_solver *S, double *eps, const double *E, const double *rh
                     double tol, int max_iter, double res_scale, double *res_hist,
                     int *iters, int *status, void *stream) {
   if (!S || !eps || !E) return fail(JX_EINVAL, "jx_solver_solve: NULL argument");
+  JX_ALIGNED(eps, "jx_solver_solve");
+  JX_ALIGNED(rhs, "jx_solver_solve");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   hipStream_t s = (hipStream_t)stream;
   jx_green *G = S->green;
@@ -953,6 +968,8 @@ int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
   if (rc) return rc;
   if (!eps || !mu || !dsph || !ddev)
     return fail(JX_EINVAL, "jx_adjoint_contract: NULL argument");
+  JX_ALIGNED(eps, "jx_adjoint_contract");
+  JX_ALIGNED(mu, "jx_adjoint_contract");
   Grid g = make_grid(dim, shape);
   return launch_adjoint_contract(g, dim == 2 ? 3 : 6, eps, mu, dsph, ddev,
                                  (hipStream_t)stream);

@@ -475,6 +475,8 @@ int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
                  int *index) {
   if (!B || !S || !eps || !E)
     return fail(JX_EINVAL, "jx_batch_add: NULL argument");
+  JX_ALIGNED(eps, "jx_batch_add");
+  JX_ALIGNED(rhs, "jx_batch_add");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   const jx_green *G = S->green;
   const int C = G->sym;

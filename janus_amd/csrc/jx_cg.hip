@@ -501,6 +501,8 @@ int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E,
                        double res_scale, double *res_hist, int *iters,
                        int *status, void *stream) {
   if (!S || !eps || !E) return fail(JX_EINVAL, "jx_solver_solve_cg: NULL argument");
+  JX_ALIGNED(eps, "jx_solver_solve_cg");
+  JX_ALIGNED(rhs, "jx_solver_solve_cg");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   if (!S->local->symmetric)
     return fail(JX_EINVAL, "conjugate gradients need a symmetric local operator");
@@ -561,6 +563,8 @@ int jx_solver_solve_cg_polarization(jx_solver *S, double *tau, double *eps,
                                     int *iters, int *status, void *stream) {
   if (!S || !tau || !eps || !E)
     return fail(JX_EINVAL, "jx_solver_solve_cg_polarization: NULL argument");
+  JX_ALIGNED(tau, "jx_solver_solve_cg_polarization");
+  JX_ALIGNED(eps, "jx_solver_solve_cg_polarization");
   if (tau == eps) return fail(JX_EINVAL, "tau and eps must not overlap");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   if (!S->local->symmetric)

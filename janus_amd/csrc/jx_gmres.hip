@@ -359,6 +359,8 @@ int jx_solver_solve_gmres(jx_solver *S, double *eps, const double *E,
                           int *iters, int *status, void *stream) {
   if (!S || !eps || !E)
     return fail(JX_EINVAL, "jx_solver_solve_gmres: NULL argument");
+  JX_ALIGNED(eps, "jx_solver_solve_gmres");
+  JX_ALIGNED(rhs, "jx_solver_solve_gmres");
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   if (restart < 1 || restart > 256)
     return fail(JX_EINVAL, "restart must be in [1, 256]");

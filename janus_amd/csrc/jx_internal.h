@@ -16,6 +16,16 @@ void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 
+// Device arrays crossing the C ABI must be 16-byte aligned: the kernels
+// move complex values and component pairs with 16-byte loads and stores.
+#define JX_ALIGNED(p, what)                                             \
+  do {                                                                  \
+    if ((p) != nullptr && (reinterpret_cast<uintptr_t>(p) & 15))        \
+      return jx::fail(JX_EINVAL, std::string(what) +                    \
+                                     ": device arrays must be 16-byte " \
+                                     "aligned");                        \
+  } while (0)
+
 #define JX_HIP(call)                                  \
   do {                                                \
     hipError_t jx_e_ = (call);                        \

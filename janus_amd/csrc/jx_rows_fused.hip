@@ -91,8 +91,11 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
       double re = 0., im = 0.;
       if (r < rb) {
         const double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
-        re = sr[2 * c];
-        im = sr[2 * c + 1];
+        {
+          const double2 z = *(const double2 *)(sr + 2 * c);
+          re = z.x;
+          im = z.y;
+        }
       }
       double2 *Z = smem + (q >> 1) * LS;
       if (k == 0 || 2 * k == L)
@@ -257,8 +260,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
         im = -0.5 * (zk.x - zm.x);
       }
       double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
-      sr[2 * c] = re;
-      sr[2 * c + 1] = im;
+      *(double2 *)(sr + 2 * c) = make_double2(re, im);
     }
   }
 }

@@ -74,8 +74,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     double *sr = spec + spec_row_off(row0 + r, (int64_t)nh * 2 * C, a.sp) +
                  (int64_t)k * 2 * C;
-    sr[2 * c] = re;
-    sr[2 * c + 1] = im;
+    *(double2 *)(sr + 2 * c) = make_double2(re, im);
   }
 }
 
@@ -270,8 +269,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
       im = -0.5 * (zk.x - zm.x);
     }
     double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
-    sr[2 * c] = re;
-    sr[2 * c + 1] = im;
+    *(double2 *)(sr + 2 * c) = make_double2(re, im);
   }
 }
 

@@ -39,8 +39,8 @@ __global__ void __launch_bounds__(kThreads)
     int part = q & 1;
     const double *sr = spec + spec_row_off(row0 + r, (int64_t)nh * 2 * C, a.sp) +
                        (int64_t)k * 2 * C;
-    double re = sr[2 * c];
-    double im = sr[2 * c + 1];
+    const double2 z = *(const double2 *)(sr + 2 * c);
+    double re = z.x, im = z.y;
     double2 *Z = bufA + (q >> 1) * a.Ls;
     bool lonely = (!part) && (q + 1 == Q);
     if (k == 0 || 2 * k == L) {
@@ -184,8 +184,11 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     double re = 0., im = 0.;
     if (r < rb) {
       const double *sr = spec + spec_row_off(row0 + r, NH * 2 * C, lay) + k * 2 * C;
-      re = sr[2 * c];
-      im = sr[2 * c + 1];
+      {
+        const double2 z = *(const double2 *)(sr + 2 * c);
+        re = z.x;
+        im = z.y;
+      }
     }
     double2 *Z = smem + (q >> 1) * LS;
     if (k == 0 || 2 * k == L)

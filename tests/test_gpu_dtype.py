@@ -77,3 +77,28 @@ def test_wrong_dtype_refused(dtype):
     # the refused calls launched nothing and left the operators usable
     y = green.apply(good)
     assert float(y.abs().max()) == 0.0
+
+
+def test_misaligned_device_arrays_refused():
+    """A float64 view starting 8 bytes into an allocation is refused by the
+    C ABI (the kernels use 16-byte accesses) before any launch."""
+    import torch
+    t = fft.create_real((16, 12, 10))
+    n = 16 * 12 * 10
+    base = torch.zeros(n + 1, dtype=torch.float64, device='cuda')
+    r = base[1:].view(16, 12, 10)
+    assert r.data_ptr() % 16 == 8
+    with pytest.raises(ValueError, match='aligned'):
+        t.r2c(r)
+    green, local = _ops()
+    nf = 8 * 6 * 10 * 6
+    buf = torch.zeros(nf + 1, dtype=torch.float64, device='cuda')
+    x = buf[1:].view(SHAPE + (6,))
+    with pytest.raises(ValueError, match='aligned'):
+        green.apply(x, torch.zeros(SHAPE + (6,), dtype=torch.float64,
+                                   device='cuda'))
+    with pytest.raises(ValueError, match='aligned'):
+        LSSolver(green, local).solve(np.array([0., 0., 0., 0., 0., 1.]),
+                                     eps=x)
+    torch.cuda.synchronize()
+    assert float(buf.abs().max()) == 0.0
