@@ -265,6 +265,15 @@ def kernel_pass(name):
     """Bench pass name of a kernel symbol (None for other kernels)."""
     if 'k_stream_copy' in name:
         return 'stream_copy'
+    head = name.split('(')[0].replace(' ', '')
+    if '>,1,' in head:  # one component: the janus.fft.serial transforms
+        if 'k_rows_forward_reg' in head:
+            return 'fft_rows_r2c'
+        if 'k_rows_inverse_reg' in head:
+            return 'fft_rows_c2r'
+        if 'k_axis_reg' in head:
+            return 'fft_axis_fwd' if head.endswith(',-1>') else 'fft_axis_inv'
+        return None
     if 'k_rows_update_forward_reg' in name:
         return 'EA_rows_c2r_update_local_r2c'
     if 'k_rows_forward_reg' in name:
@@ -1170,6 +1179,23 @@ def workload_fft(args):
             times_inv.append(1e3 * (time.perf_counter() - t0))
         roundtrip = float((back - rd).abs().max())
         del back
+        # GPU time per call with calls queued back to back (no host
+        # synchronization between them), against the bytes the three passes
+        # move: the last-axis pass reads N and writes Nh complex values, each
+        # strided pass reads and writes Nh complex values
+        reps = 4 * nd
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        _sync()
+        ev0.record()
+        for _ in range(reps):
+            t.r2c(rd, cd)
+        ev1.record()
+        _sync()
+        ms_queued = ev0.elapsed_time(ev1) / reps
+        ncell = int(np.prod(shape))
+        nhalf = ncell // shape[-1] * (shape[-1] // 2 + 1)
+        pass_bytes = 8 * ncell + 16 * nhalf + (len(shape) - 1) * 32 * nhalf
         c = np.empty(t.oshape)
         t.r2c(r, c)
         host = []
@@ -1187,17 +1213,27 @@ def workload_fft(args):
             'device_c2r_GBps_in_plus_out': round(
                 nbytes / (np.mean(times_inv) * 1e-3) / 1e9, 1),
             'c2r_r2c_roundtrip_max_abs_err': roundtrip,
+            'device_ms_queued': round(ms_queued, 4),
+            'pass_bytes': pass_bytes,
+            'pass_GBps_queued': round(pass_bytes / (ms_queued * 1e-3) / 1e9,
+                                      1),
+            'pass_frac_of_8TBps_queued': round(
+                pass_bytes / (ms_queued * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
             'host_arrays_ms_mean': round(float(np.mean(host)), 2),
             'host_arrays_ms_std': round(float(np.std(host)), 2),
             'calls': [nd, nh]}
-        _log('r2c %s: device %.3f ms (c2r %.3f ms), numpy arrays %.1f ms'
-             % (shape, np.mean(times), np.mean(times_inv), np.mean(host)))
+        _log('r2c %s: device %.3f ms (c2r %.3f ms; queued %.3f ms, %.0f '
+             'GB/s on the three passes), numpy arrays %.1f ms'
+             % (shape, np.mean(times), np.mean(times_inv), ms_queued,
+                pass_bytes / (ms_queued * 1e-3) / 1e9, np.mean(host)))
         del rd, cd
         torch.cuda.empty_cache()
+    copy = stream_copy_gbps()
     _line('r2c ms per call at 256^3 (janus fftw_python_benchmark_serial)',
           rows['256^3']['device_ms_mean'], 'ms',
           {'workload': 'r2c of uniform(-1, 1) fields, seed 20140121, '
-                       '128^3 / 256^3 / 512^3', 'shapes': rows},
+                       '128^3 / 256^3 / 512^3', 'shapes': rows,
+           'stream_copy_GBps': round(copy, 1)},
           higher=False)
 
 
