@@ -1237,9 +1237,64 @@ def workload_fft(args):
           higher=False)
 
 
+def workload_patterns(args):
+    """Memory rate of the strided passes' access pattern with no arithmetic
+    (jx_tile_probe): the 6-component padded spectrum of a 256^3 and a 512^3
+    grid copied along axis 0 (the Gamma-hat pass) and axis 1 (passes B and
+    D) in tiles of 1, 2 and 4 points, read + write bytes over hipEvent time,
+    beside the stream copy."""
+    import ctypes
+    import torch
+    from janus_amd import _lib
+    lib = _lib.lib()
+    stream = _lib.current_stream()
+    table = {}
+    for n in (256, 512):
+        shape = (ctypes.c_int64 * 3)(n, n, n)
+        nbytes = ctypes.c_int64()
+        _lib.check(lib.jx_tile_probe_bytes(shape, ctypes.byref(nbytes)))
+        a = torch.ones(nbytes.value // 8, dtype=torch.float64, device='cuda')
+        b = torch.zeros_like(a)
+        moved = 2 * 96 * n * n * (n // 2 + 1)
+        for axis in (0, 1):
+            for pts in (1, 2, 4):
+                def run():
+                    _lib.check(lib.jx_tile_probe(shape, axis, pts, a.data_ptr(),
+                                                 b.data_ptr(), nbytes.value,
+                                                 stream))
+                run()
+                _sync()
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev1 = torch.cuda.Event(enable_timing=True)
+                reps = 10
+                ev0.record()
+                for _ in range(reps):
+                    run()
+                ev1.record()
+                _sync()
+                ms = ev0.elapsed_time(ev1) / reps
+                key = '%d^3 axis %d, %d-point tiles' % (n, axis, pts)
+                table[key] = {'ms': round(ms, 4),
+                              'GBps': round(moved / (ms * 1e-3) / 1e9, 1)}
+                _log('%s: %.4f ms, %.0f GB/s' % (key, ms,
+                                                  moved / (ms * 1e-3) / 1e9))
+        # every spectrum value copied (the slab padding stays zero)
+        table['%d^3 all values copied' % n] = (
+            float(b.sum()) == 12. * n * n * (n // 2 + 1))
+        del a, b
+        torch.cuda.empty_cache()
+    copy = stream_copy_gbps()
+    _line('strided-pass access pattern without arithmetic, 256^3 axis 0 '
+          '2-point tiles', table['256^3 axis 0, 2-point tiles']['GBps'],
+          'GB/s', {'workload': 'jx_tile_probe: padded 6-component spectrum, '
+                               'read + write bytes over hipEvent time',
+                   'table': table, 'stream_copy_GBps': round(copy, 1)})
+
+
 WORKLOADS = {'plain2d': workload_plain2d, 'stiffness': workload_stiffness,
              'batched': workload_batched, 'adjoint': workload_adjoint,
-             'large': workload_large, 'fft': workload_fft}
+             'large': workload_large, 'fft': workload_fft,
+             'patterns': workload_patterns}
 
 
 def main():

@@ -347,6 +347,17 @@ int jx_adjoint_outer(int dim, const int64_t *shape, const double *eps,
  * for the calibration of traffic counters. */
 int jx_stream_copy(const void *src, void *dst, int64_t bytes, void *stream);
 
+/* Measurement helper (no janus counterpart): the memory traffic of the
+ * strided transform passes without their arithmetic.  Copies the spectrum of
+ * the 3D grid `shape` (6 components, the solvers' padded layout) from src to
+ * dst along `axis` (0 or 1; that axis has 256 or 512 points) in tiles of
+ * `points` points (1, 2 or 4), in the passes' tile and lane order.  src and
+ * dst are 16-byte aligned and hold `bytes` each, at least the layout's size
+ * from jx_tile_probe_bytes. */
+int jx_tile_probe_bytes(const int64_t *shape, int64_t *bytes);
+int jx_tile_probe(const int64_t *shape, int axis, int points, const void *src,
+                  void *dst, int64_t bytes, void *stream);
+
 /* Measurement helper (no janus counterpart): runs `iters` fixed-point
  * iterations exactly as jx_solver_solve does (tol ignored) after one untimed
  * iteration, and records a hipEvent before and after every pass on

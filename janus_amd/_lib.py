@@ -77,6 +77,9 @@ SIGNATURES = [
     ('jx_adjoint_outer', ctypes.c_int,
      [ctypes.c_int, _c_int64_p, _VP, _VP, _VP, _VP]),
     ('jx_stream_copy', ctypes.c_int, [_VP, _VP, ctypes.c_int64, _VP]),
+    ('jx_tile_probe_bytes', ctypes.c_int, [_c_int64_p, _c_int64_p]),
+    ('jx_tile_probe', ctypes.c_int,
+     [_c_int64_p, ctypes.c_int, ctypes.c_int, _VP, _VP, ctypes.c_int64, _VP]),
     ('jx_solver_set_fused', ctypes.c_int, [_VP, ctypes.c_int]),
     ('jx_solver_get_fused', ctypes.c_int, [_VP]),
     ('jx_solver_profile', ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]),

@@ -966,6 +966,28 @@ int jx_stream_copy(const void *src, void *dst, int64_t bytes, void *stream) {
   return launch_stream_copy(src, dst, bytes / 16, (hipStream_t)stream);
 }
 
+int jx_tile_probe_bytes(const int64_t *shape, int64_t *bytes) {
+  int rc = check_shape(3, shape);
+  if (rc) return rc;
+  if (!bytes) return fail(JX_EINVAL, "jx_tile_probe_bytes: bytes is NULL");
+  *bytes = (int64_t)sizeof(double) * spec_doubles(make_grid(3, shape), 6);
+  return JX_OK;
+}
+
+int jx_tile_probe(const int64_t *shape, int axis, int points, const void *src,
+                  void *dst, int64_t bytes, void *stream) {
+  int rc = check_shape(3, shape);
+  if (rc) return rc;
+  if (!src || !dst) return fail(JX_EINVAL, "jx_tile_probe: NULL argument");
+  JX_ALIGNED(src, "jx_tile_probe");
+  JX_ALIGNED(dst, "jx_tile_probe");
+  const Grid g = make_grid(3, shape);
+  if (bytes < (int64_t)sizeof(double) * spec_doubles(g, 6))
+    return fail(JX_EINVAL, "jx_tile_probe: buffers smaller than the spectrum layout");
+  return tile_probe(g, axis, points, (const double *)src, (double *)dst,
+                    (hipStream_t)stream);
+}
+
 int jx_adjoint_contract(int dim, const int64_t *shape, const double *eps,
                         const double *mu, double *dsph, double *ddev,
                         void *stream) {

@@ -432,6 +432,89 @@ static int launch_axis_reg(const Grid &g, int ncomp, int axis, int dir,
   });
 }
 
+// ------------------------------------------------- access-pattern probe
+// The memory traffic of a strided pass without its transforms: every block
+// copies the L rows of one tile of T points (6 complex components each,
+// 96 B per point and row) of the solver's padded 3D spectrum layout along
+// `axis`, in the passes' tile order (xcd_tile) and lane order (consecutive
+// lanes read consecutive 16-byte values of a tile row).  Its rate is what
+// the memory system gives that pattern with no arithmetic in the way.
+template <int T, int V>
+__global__ void __launch_bounds__(6 * T * 16)
+    k_tile_probe(const double2 *in, double2 *out,
+                 int64_t step, int64_t outer_step, int64_t ntiles,
+                 int64_t n_inner) {
+  constexpr int NS = 6 * T;
+  const int tid = threadIdx.x, s = tid % NS, t = tid / NS;
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t o = blk / ntiles;
+  const int64_t m0 = (blk - o * ntiles) * T;
+  if (m0 + s / 6 >= n_inner) return;
+  const double2 *src = in + o * outer_step + m0 * 6 + s;
+  double2 *dst = out + o * outer_step + m0 * 6 + s;
+  // every row of the tile is loaded before the first store, as in the passes
+  double vx[V], vy[V];
+#pragma unroll
+  for (int r = 0; r < V; ++r) {
+    const double2 z = src[(int64_t)(t + 16 * r) * step];
+    vx[r] = z.x;
+    vy[r] = z.y;
+  }
+#pragma unroll
+  for (int r = 0; r < V; ++r)
+    dst[(int64_t)(t + 16 * r) * step] = make_double2(vx[r], vy[r]);
+}
+
+template <int T>
+static int launch_tile_probe(int L, dim3 grid, hipStream_t s, const double *in,
+                             double *out, const AxisArgs &a) {
+  dim3 block(6 * T * 16);
+  const double2 *i2 = (const double2 *)in;
+  double2 *o2 = (double2 *)out;
+  if (L == 256)
+    hipLaunchKernelGGL((k_tile_probe<T, 16>), grid, block, 0, s, i2, o2,
+                       a.stride_ax / 2, a.stride_outer / 2,
+                       (int64_t)a.ntiles, a.n_inner);
+  else
+    hipLaunchKernelGGL((k_tile_probe<T, 32>), grid, block, 0, s, i2, o2,
+                       a.stride_ax / 2, a.stride_outer / 2,
+                       (int64_t)a.ntiles, a.n_inner);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+int tile_probe(const Grid &g, int axis, int points, const double *src,
+               double *dst, hipStream_t s) {
+  if (g.dim != 3) return fail(JX_EINVAL, "jx_tile_probe: the grid must be 3D");
+  if (axis != 0 && axis != 1)
+    return fail(JX_EINVAL, "jx_tile_probe: axis must be 0 or 1");
+  const int L = (int)g.n[axis];
+  if (L != 256 && L != 512)
+    return fail(JX_EINVAL, "jx_tile_probe: the probed axis must have 256 or 512 points");
+  if (points != 1 && points != 2 && points != 4)
+    return fail(JX_EINVAL, "jx_tile_probe: points must be 1, 2 or 4");
+  AxisArgs a;
+  a.L = L;
+  a.Ls = 0;
+  a.nh = (int)g.nh;
+  a.axis = axis;
+  int64_t inner = g.nh;
+  for (int d = axis + 1; d < g.dim - 1; ++d) inner *= g.n[d];
+  int64_t outer = 1;
+  for (int d = 0; d < axis; ++d) outer *= g.n[d];
+  a.n_inner = inner;
+  a.n_outer = outer;
+  a.stride_ax = inner * 2 * 6;
+  a.stride_outer = (int64_t)L * inner * 2 * 6;
+  padded_strides(g, 6, axis, a);
+  a.T = points;
+  a.ntiles = (int)((inner + points - 1) / points);
+  dim3 grid((unsigned)(outer * a.ntiles));
+  if (points == 1) return launch_tile_probe<1>(L, grid, s, src, dst, a);
+  if (points == 2) return launch_tile_probe<2>(L, grid, s, src, dst, a);
+  return launch_tile_probe<4>(L, grid, s, src, dst, a);
+}
+
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 double *out, const GreenDev *green, const SolveState *st,
                 hipStream_t s, const Batch *b) {

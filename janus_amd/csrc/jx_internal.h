@@ -207,6 +207,10 @@ int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,
 int launch_axis(const Grid &g, int ncomp, int axis, int dir, const double *in,
                 double *out, const GreenDev *green, const SolveState *st,
                 hipStream_t s, const Batch *b = nullptr);
+// Copy of a 3D 6-component padded spectrum along axis 0 or 1 in the strided
+// passes' tile and lane order, without the transforms (jx_tile_probe).
+int tile_probe(const Grid &g, int axis, int points, const double *src,
+               double *dst, hipStream_t s);
 // Scalars of a conjugate-gradient solve, kept on the device.
 struct CgScalars {
   double rr;     // <r, r> in the C0 inner product

@@ -2,6 +2,13 @@
 workloads, through checks that hold at any size plus the oracle where it
 finishes in seconds.
 
+* Stiffness as stated (128^3 Boolean spheres r = 8, vf 0.2, contrast 50,
+  filtered Gamma0, C0 = the matrix, where the basic scheme diverges): the
+  six loads by conjugate gradients and by GMRES(20) give 6x6 effective
+  stiffnesses that are symmetric and equal to each other to 1e-10; the
+  first CG iterates equal the oracle's CG; the e_xy field has mean strain E
+  and satisfies the Lippmann-Schwinger equation with the separate
+  operators.
 * Batched (grids {64, 80, 96, 128, 160}^3, Boolean spheres r = 8,
   contrast logU[2, 100], filtered Gamma0, six Mandel unit loads each): ten
   of the configuration's 32 draws -- the lowest and the highest contrast of
@@ -55,6 +62,65 @@ def _two_phase(phase, contrast):
                      fft.create_real(phase.shape))
     return LSSolver(green, BlockDiagonalOperator3D.from_phases(phase, table)), \
         table, mat0
+
+
+# ---------------------------------------------------------------- stiffness
+def test_stiffness_128_as_stated():
+    import torch
+    n = 128
+    phase = ms.spheres(n, 8.0, 0.2, seed=2014)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(50.0, 0.25, 3)
+    mat0 = material.create(1.0, 0.3, 3)
+    table = np.array([delta_stiffness(m, mat0) for m in (mat_m, mat_i)])
+    green = filtered(mat0.green_operator(), phase.shape, 1.0,
+                     fft.create_real(phase.shape))
+    local = BlockDiagonalOperator3D.from_phases(phase, table)
+    S = LSSolver(green, local)
+    E = _unit(5)
+    first = S.solve_cg(E, tol=0.0, max_iter=2)
+    eps_ref, res_ref = jo.cg_scheme(table[phase, 0], table[phase, 1], E,
+                                    'filtered', mat0.g, mat0.nu, tol=0.0,
+                                    max_iter=2)
+    np.testing.assert_allclose(first.residuals, res_ref, rtol=1e-10)
+    err = float(np.abs(first.eps.cpu().numpy() - eps_ref).max() /
+                np.abs(eps_ref).max())
+    print('stiffness 128^3: 2 CG iterates vs oracle %.3g' % err, flush=True)
+    assert err < 1e-12, err
+    del eps_ref
+    C0 = reference_stiffness(mat0)
+    C = {}
+    for method in ('cg', 'gmres'):
+        Cm = np.zeros((6, 6))
+        for j in range(6):
+            if method == 'cg':
+                r = S.solve_cg(_unit(j), tol=1e-10, max_iter=2000)
+            else:
+                r = S.solve_gmres(_unit(j), tol=1e-10, max_iter=2000,
+                                  restart=20)
+            assert r.status == 'converged', (method, j, r.status)
+            tau, _ = local.means(r.eps)
+            Cm[:, j] = C0.dot(_unit(j)) + tau
+            if method == 'cg' and j == 5:
+                eps = r.eps
+        C[method] = Cm
+        scale = np.abs(Cm).max()
+        asym = np.abs(Cm - Cm.T).max() / scale
+        print('stiffness 128^3 %s: C_1212 %.12f, asymmetry %.3g'
+              % (method, 0.5 * Cm[5, 5], asym), flush=True)
+        assert asym < 1e-10, (method, asym)
+    diff = np.abs(C['cg'] - C['gmres']).max() / np.abs(C['cg']).max()
+    print('stiffness 128^3: CG vs GMRES C_eff %.3g' % diff, flush=True)
+    assert diff < 1e-10, diff
+    # the e_xy field: mean strain and the equation with the separate operators
+    Ed = torch.from_numpy(E).cuda()
+    assert float((eps.reshape(-1, 6).mean(dim=0) - Ed).abs().max()) < 1e-13
+    tau = torch.empty_like(eps)
+    local.apply(eps, tau)
+    g = torch.empty_like(eps)
+    green.apply(tau, g)
+    res = float(torch.linalg.vector_norm(eps + g - Ed)) / np.sqrt(n ** 3)
+    assert res < 2e-10, res
 
 
 # ------------------------------------------------------------------ batched
