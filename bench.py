@@ -672,6 +672,16 @@ def iteration_bytes(shape, fused, sym=6, m=1):
     return ((2 if fused else 3) * 8 * sym + m) * N + 8 * 16 * sym * Nh
 
 
+def cg_iteration_bytes(shape, sym=6, m=1):
+    """Algorithmic HBM bytes of one fused conjugate-gradient iteration on a
+    3D grid: 768 Nh for the four spectrum passes, and per voxel pass A
+    (r, p, x read, p, x written, phase id), pass E (p read, q written) and
+    the vector kernel (r, q read, r written): (10 * 8 sym + m) N."""
+    N = int(np.prod(shape))
+    Nh = int(np.prod(shape[:-1])) * (shape[-1] // 2 + 1)
+    return (10 * 8 * sym + m) * N + 8 * 16 * sym * Nh
+
+
 def workload_batched(args, emit=True):
     """32 seeded microstructures, n ~ U{64, 80, 96, 128, 160}, Boolean
     spheres r = 8, vf ~ U[0.1, 0.35], inclusion/matrix contrast ~
@@ -791,6 +801,52 @@ def workload_batched(args, emit=True):
               for g in batch.groups()]
     del eps_seq, eps_b, batch
     torch.cuda.empty_cache()
+    # the same 192 right-hand sides by conjugate gradients: sequential
+    # solve_cg against one batch of CG entries
+    cg = {}
+    if not args.no_batched_cg:
+        for S in solvers[:1]:
+            S.solve_cg(_unit(6, 0), tol=0., max_iter=2)  # warm-up
+        eps_cs = fields()
+        _sync()
+        t0 = time.perf_counter()
+        seq_cg = []
+        for (i, j), e in zip(rhs, eps_cs):
+            r = solvers[i].solve_cg(_unit(6, j), tol=1e-10,
+                                    max_iter=args.max_iter, eps=e)
+            seq_cg.append((r.iterations, r.status))
+        _sync()
+        t_seq_cg = time.perf_counter() - t0
+        eps_cb = fields()
+        bcg = BatchSolver(iters_per_launch=args.iters_per_launch,
+                          graph=args.graph, autotune=True)
+        for (i, j), e in zip(rhs, eps_cb):
+            bcg.add(solvers[i], _unit(6, j), tol=1e-10,
+                    max_iter=args.max_iter, eps=e, method='cg')
+        _sync()
+        t0 = time.perf_counter()
+        res_cg = bcg.run()
+        _sync()
+        t_b_cg = time.perf_counter() - t0
+        eq_cg = sum(bool(torch.equal(a, b)) for a, b in zip(eps_cs, eps_cb))
+        same_cg = all(r.iterations == q[0] for r, q in zip(res_cg, seq_cg))
+        its_cg = [q[0] for q in seq_cg]
+        _log('CG: sequential %.3f s, batched %.3f s (%d iterations); '
+             'batched CG vs sequential fixed point %.2fx'
+             % (t_seq_cg, t_b_cg, sum(its_cg), t_seq / t_b_cg))
+        del eps_cs, eps_cb, bcg
+        torch.cuda.empty_cache()
+        cg = {'seconds_sequential': round(t_seq_cg, 4),
+              'seconds_batched': round(t_b_cg, 4),
+              'speedup': round(t_seq_cg / t_b_cg, 3),
+              'speedup_vs_sequential_fixed_point': round(t_seq / t_b_cg, 3),
+              'iterations_total': int(sum(its_cg)),
+              'converged': sum(q[1] == 'converged' for q in seq_cg),
+              'entries_bitwise_equal': eq_cg,
+              'same_iteration_counts': same_cg,
+              'bytes_alg_total': int(sum(
+                  it * cg_iteration_bytes(solvers[i].shape[:-1])
+                  for (i, j), it in zip(rhs, its_cg)))}
     copy_gbps = stream_copy_gbps()
     total_bytes = sum(it * iteration_bytes(solvers[i].shape[:-1],
                                            solvers[i].fused)
@@ -816,6 +872,14 @@ def workload_batched(args, emit=True):
         'converged': conv,
         'iterations_total': int(sum(its)),
     }
+    if cg:
+        cg['bound_seconds_copy_bw'] = round(
+            cg['bytes_alg_total'] / (copy_gbps * 1e9), 4)
+        cg['speedup_bound_copy_bw'] = round(
+            cg['seconds_sequential'] / cg['bound_seconds_copy_bw'], 3)
+        cg['batched_over_bound'] = round(
+            cg['seconds_batched'] / cg['bound_seconds_copy_bw'], 3)
+        summary['cg'] = cg
     if emit:
         _line('batched vs sequential plain solves, %d right-hand sides (%d '
               'microstructures x 6 loads) at 1e-10' % (len(rhs), len(solvers)),
@@ -1340,6 +1404,9 @@ def main():
                     help='Arnoldi steps per GMRES cycle (stiffness workload)')
     ap.add_argument('--per-group', action='store_true',
                     help='batched workload: also time each grid size alone')
+    ap.add_argument('--no-batched-cg', action='store_true',
+                    help='batched workload: skip the conjugate-gradient '
+                         'comparison')
     ap.add_argument('--adjoint-reps', type=int, default=2)
     ap.add_argument('--fd-voxels', type=int, default=0,
                     help='large workload: > 0 adds a central difference '

@@ -290,6 +290,17 @@ int jx_batch_destroy(jx_batch *batch);
 int jx_batch_add(jx_batch *batch, jx_solver *solver, double *eps,
                  const double *E, const double *rhs, double tol, int max_iter,
                  double res_scale, int *index);
+/* The same entry solved by conjugate gradients: the arguments, the result
+ * (eps = E + x), the iterates, residual history and iteration count of
+ * jx_solver_solve_cg (janus drives CG from its examples,
+ * examples/square_inclusion.py:40-47, 71-86).  Needs a symmetric local
+ * operator (else JX_EINVAL) and register FFT lengths on every axis (else
+ * JX_EUNSUPPORTED).  Also allocates the entry's r, p, q fields, partial
+ * sums and scalars.  Entries of both methods can share a batch; they never
+ * share a launch. */
+int jx_batch_add_cg(jx_batch *batch, jx_solver *solver, double *eps,
+                    const double *E, const double *rhs, double tol,
+                    int max_iter, double res_scale, int *index);
 /* Groups in order of first appearance: grid (dim, shape[3], 1 for unused
  * axes), entry count, entries per launch (after jx_batch_run: the width
  * used) and whether the group's launches are batched (1) or per entry. */

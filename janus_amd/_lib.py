@@ -100,6 +100,9 @@ SIGNATURES = [
     ('jx_batch_add', ctypes.c_int,
      [_VP, _VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
       _VP]),
+    ('jx_batch_add_cg', ctypes.c_int,
+     [_VP, _VP, _VP, _VP, _VP, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+      _VP]),
     ('jx_batch_ngroups', ctypes.c_int, [_VP, _VP]),
     ('jx_batch_group', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]),
     ('jx_batch_tuning', ctypes.c_int,

@@ -19,6 +19,10 @@ different grids -- and runs them together through ``jx_batch_run``:
 
 Each entry stops after the first iteration whose residual is below its
 ``tol``, exactly as :meth:`LSSolver.solve` would, with the same iterates.
+Entries added with ``method='cg'`` run :meth:`LSSolver.solve_cg` instead
+(conjugate gradients on the strain form, symmetric local operators, register
+FFT lengths on every axis), with its iterates; they share launches with
+other CG entries of their grid only.
 """
 
 import ctypes
@@ -55,11 +59,15 @@ class BatchSolver:
             self._handle = None
 
     def add(self, solver, E, tol=1e-10, max_iter=200, eps=None, rhs=None,
-            res_scale=None):
-        """Add one right-hand side; arguments as :meth:`LSSolver.solve`.
+            res_scale=None, method='basic'):
+        """Add one right-hand side; arguments as :meth:`LSSolver.solve`,
+        ``method`` 'basic' (the fixed point) or 'cg' (as
+        :meth:`LSSolver.solve_cg`).
 
         Returns the entry index.  ``eps`` (default: E everywhere) is the
         entry's own GPU field, overwritten by the last iterate."""
+        if method not in ('basic', 'cg'):
+            raise ValueError("method must be 'basic' or 'cg'")
         t = _device.torch()
         E = np.ascontiguousarray(E, dtype=np.float64)
         if E.shape != (solver.sym,):
@@ -79,7 +87,9 @@ class BatchSolver:
             raise ValueError('rhs must be a contiguous float64 GPU field of '
                              'shape {0}'.format(solver.shape))
         index = ctypes.c_int(-1)
-        _lib.check(_lib.lib().jx_batch_add(
+        add = (_lib.lib().jx_batch_add_cg if method == 'cg'
+               else _lib.lib().jx_batch_add)
+        _lib.check(add(
             self._handle, solver._handle, eps.data_ptr(), E.ctypes.data,
             _lib.ptr(rhs), float(tol), int(max_iter),
             float(res_scale or 0.0), ctypes.byref(index)))
@@ -151,9 +161,10 @@ class BatchSolver:
 
 
 def effective_stiffness_batched(solvers, mats0, tol=1e-10, max_iter=200,
-                                **schedule):
+                                method='basic', **schedule):
     """Full effective stiffness of every solver (all sym Mandel unit loads)
-    in one batch.  Returns ([C_eff], [[iterations]], [[status]], batch)."""
+    in one batch, by the fixed point (method 'basic') or conjugate gradients
+    ('cg').  Returns ([C_eff], [[iterations]], [[status]], batch)."""
     from .solver import reference_stiffness
     batch = BatchSolver(**schedule)
     index = []
@@ -162,7 +173,8 @@ def effective_stiffness_batched(solvers, mats0, tol=1e-10, max_iter=200,
         for j in range(S.sym):
             E = np.zeros(S.sym)
             E[j] = 1.0
-            row.append(batch.add(S, E, tol=tol, max_iter=max_iter))
+            row.append(batch.add(S, E, tol=tol, max_iter=max_iter,
+                                 method=method))
         index.append(row)
     res = batch.run()
     Cs, its, sts = [], [], []

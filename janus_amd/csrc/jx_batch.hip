@@ -19,6 +19,12 @@
 // once per replay.  Each entry runs exactly the kernels and
 // arithmetic of jx_solver_solve, and its converged blocks skip their work,
 // so the iterates and iteration counts are those of the sequential solves.
+//
+// Conjugate-gradient entries (jx_batch_add_cg) run jx_solver_solve_cg: each
+// starts and finishes on its own (the fill, the first matvec and residual;
+// the final eps = E + x), and every iteration of a chunk is the fused CG
+// iteration of the single solve with the chunk's entries in each launch,
+// their fields, scalars and C0 weights in their slots.
 
 #include <algorithm>
 #include <chrono>
@@ -45,6 +51,12 @@ struct Entry {
   double *spec = nullptr, *partials = nullptr, *res_hist = nullptr;
   int slot = -1;  // index of its solve state in the last run
   int iters = 0, status = 0;
+  // conjugate gradients: r, p, q fields, vector partial sums, scalars and
+  // the C0 inner product weights
+  bool cg = false;
+  double *cg_buf = nullptr, *cg_partials = nullptr;
+  CgScalars *cg_cs = nullptr;
+  double wd = 0., wt = 0.;
 };
 
 struct Group {
@@ -54,6 +66,7 @@ struct Group {
   int act_off = 0;           // their first position in the table
   bool batched = false;      // register kernels take the whole chunk
   bool fused = false;
+  bool cg = false;           // conjugate-gradient entries
   int width = 0;             // entries per launch (0 = all)
   bool user_width = false;
   // autotuning of the last run: candidate widths, ms per entry-iteration
@@ -87,11 +100,11 @@ struct jx_batch {
 namespace {
 
 using Key = std::tuple<int, int64_t, int64_t, int64_t, int, uint64_t, bool,
-                       bool, int, int>;
+                       bool, int, int, bool>;
 
 // Entries share launches when they have the same grid, kind, h (the axis
-// tables), rhs use, schedule, plan variants (the rounding) and family of
-// local operator (the kernel instantiation).
+// tables), rhs use, schedule, plan variants (the rounding), family of
+// local operator (the kernel instantiation) and method.
 Key entry_key(const Entry &e) {
   const jx_green *G = e.S->green;
   uint64_t hb = 0;
@@ -100,7 +113,8 @@ Key entry_key(const Entry &e) {
   int plans = 0;
   for (int p = 0; p < kPlanPasses; ++p) plans |= G->g.plan[p] << p;
   return Key(G->g.dim, G->g.n[0], G->g.n[1], G->g.n[2], G->dev.kind, hb,
-             e.rhs != nullptr, fused, plans, local_family(e.S->local->dev));
+             e.rhs != nullptr, fused && !e.cg, plans,
+             local_family(e.S->local->dev), e.cg);
 }
 
 void build_groups(jx_batch *B) {
@@ -115,6 +129,7 @@ void build_groups(jx_batch *B) {
       const jx_green *G = B->ents[i].S->green;
       gr.batched = batchable(G->g, G->sym);
       gr.fused = std::get<7>(k);
+      gr.cg = std::get<10>(k);
       B->groups.push_back(gr);
     }
     B->groups[it->second].members.push_back(i);
@@ -142,6 +157,13 @@ int chunk_width(const Group &gr, int width, int n) {
 int enqueue_chunk_iteration(jx_batch *B, const Group &gr, int c0, int n,
                             bool first, hipStream_t s) {
   int rc;
+  if (gr.cg) {
+    const Entry &e0 = B->ents[gr.act[c0]];
+    Batch b;
+    b.tab = B->tab + gr.act_off + c0;
+    b.n = n;
+    return cg_iteration_batched(e0.S->green, e0.S->local->dev, b, s);
+  }
   if (!gr.batched) {
     for (int m = c0; m < c0 + n; ++m) {
       Entry &e = B->ents[gr.act[m]];
@@ -290,7 +312,29 @@ void free_entry(Entry &e) {
   if (e.spec) (void)hipFree(e.spec);
   if (e.partials) (void)hipFree(e.partials);
   if (e.res_hist) (void)hipFree(e.res_hist);
-  e.spec = e.partials = e.res_hist = nullptr;
+  if (e.cg_buf) (void)hipFree(e.cg_buf);
+  if (e.cg_partials) (void)hipFree(e.cg_partials);
+  if (e.cg_cs) (void)hipFree(e.cg_cs);
+  e.spec = e.partials = e.res_hist = e.cg_buf = e.cg_partials = nullptr;
+  e.cg_cs = nullptr;
+}
+
+// The work buffers of a conjugate-gradient entry (its solve state in the
+// run's table).
+CgWork entry_cg_work(const jx_batch *B, const Entry &e) {
+  const Grid &g = e.S->green->g;
+  const int64_t F = (int64_t)e.S->green->sym * g.N;
+  CgWork w;
+  w.r = e.cg_buf;
+  w.p = e.cg_buf + F;
+  w.q = e.cg_buf + 2 * F;
+  w.spec = e.spec;
+  w.rows_partials = e.partials;
+  w.vec_partials = e.cg_partials;
+  w.cs = e.cg_cs;
+  w.st = B->st + e.slot;
+  w.res_hist = e.res_hist;
+  return w;
 }
 
 void free_run_state(jx_batch *B) {
@@ -339,6 +383,16 @@ int run(jx_batch *B, hipStream_t caller) {
       sl.daux[1] = D.daux2;
       sl.daux[2] = D.daux3;
       sl.daux[3] = D.daux4;
+      if (e.cg) {
+        const CgWork w = entry_cg_work(B, e);
+        sl.cg_r = w.r;
+        sl.cg_p = w.p;
+        sl.cg_q = w.q;
+        sl.cg_cs = e.cg_cs;
+        sl.cg_partials = e.cg_partials;
+        sl.cg_wd = e.wd;
+        sl.cg_wt = e.wt;
+      }
       SolveState &st = B->st_host[e.slot];
       st.iter = 0;
       st.done = 0;
@@ -352,6 +406,12 @@ int run(jx_batch *B, hipStream_t caller) {
                         hipMemcpyHostToDevice, s));
   int rc = upload_table(B, s);
   if (rc) return rc;
+  // conjugate-gradient entries start one by one (x = 0, r = p = rhs -
+  // Gamma0[dC : E], the first residual, which stops a homogeneous medium)
+  for (const Entry &e : B->ents)
+    if (e.cg &&
+        (rc = cg_start(e.S, entry_cg_work(B, e), e.eps, e.rhs, e.wd, e.wt, s)))
+      return rc;
   B->replays = 0;
   B->captures = 0;
   auto t0 = std::chrono::steady_clock::now();
@@ -441,6 +501,11 @@ int run(jx_batch *B, hipStream_t caller) {
     ++B->replays;
   }
   drop_graph(B);
+  // eps = E + x + alpha p for the conjugate-gradient entries
+  for (const Entry &e : B->ents)
+    if (e.cg && (rc = cg_finish(e.S, entry_cg_work(B, e), e.eps, e.wd, e.wt, s)))
+      return rc;
+  JX_HIP(hipStreamSynchronize(s));
   B->ms = std::chrono::duration<double, std::milli>(
               std::chrono::steady_clock::now() - t0).count();
   for (Entry &e : B->ents) {
@@ -470,9 +535,9 @@ int jx_batch_destroy(jx_batch *B) {
   return JX_OK;
 }
 
-int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
-                 const double *rhs, double tol, int max_iter, double res_scale,
-                 int *index) {
+static int batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
+                     const double *rhs, double tol, int max_iter,
+                     double res_scale, int *index, bool cg) {
   if (!B || !S || !eps || !E)
     return fail(JX_EINVAL, "jx_batch_add: NULL argument");
   JX_ALIGNED(eps, "jx_batch_add");
@@ -480,8 +545,18 @@ int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
   if (max_iter < 1) return fail(JX_EINVAL, "max_iter must be >= 1");
   const jx_green *G = S->green;
   const int C = G->sym;
+  if (cg) {
+    if (!S->local->symmetric)
+      return fail(JX_EINVAL, "conjugate gradients need a symmetric local operator");
+    if (!cg_batchable(G->g, C))
+      return fail(JX_EUNSUPPORTED, "batched conjugate gradients need register "
+                                   "FFT lengths on every axis (use "
+                                   "jx_solver_solve_cg)");
+  }
   Entry e;
   e.S = S;
+  e.cg = cg;
+  if (cg) cg_weights(G, &e.wd, &e.wt);
   e.eps = eps;
   e.rhs = rhs;
   for (int c = 0; c < C; ++c) e.E[c] = E[c];
@@ -503,6 +578,13 @@ int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
   if (he == hipSuccess) he = hipMalloc((void **)&e.partials, sizeof(double) * nb);
   if (he == hipSuccess)
     he = hipMalloc((void **)&e.res_hist, sizeof(double) * max_iter);
+  if (cg && he == hipSuccess)
+    he = hipMalloc((void **)&e.cg_buf, sizeof(double) * 3 * C * G->g.N);
+  if (cg && he == hipSuccess)
+    he = hipMalloc((void **)&e.cg_partials,
+                   sizeof(double) * cg_vec_partials_doubles());
+  if (cg && he == hipSuccess)
+    he = hipMalloc((void **)&e.cg_cs, sizeof(CgScalars));
   if (he != hipSuccess) {
     free_entry(e);
     return hip_fail(he, "jx_batch_add");
@@ -510,6 +592,18 @@ int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
   B->ents.push_back(e);
   if (index) *index = (int)B->ents.size() - 1;
   return JX_OK;
+}
+
+int jx_batch_add(jx_batch *B, jx_solver *S, double *eps, const double *E,
+                 const double *rhs, double tol, int max_iter, double res_scale,
+                 int *index) {
+  return batch_add(B, S, eps, E, rhs, tol, max_iter, res_scale, index, false);
+}
+
+int jx_batch_add_cg(jx_batch *B, jx_solver *S, double *eps, const double *E,
+                    const double *rhs, double tol, int max_iter,
+                    double res_scale, int *index) {
+  return batch_add(B, S, eps, E, rhs, tol, max_iter, res_scale, index, true);
 }
 
 int jx_batch_ngroups(jx_batch *B, int *ngroups) {

@@ -25,23 +25,24 @@
 namespace jx {
 
 int ls_matvec(jx_solver *S, const double *x, double *y, const SolveState *st,
-              hipStream_t s, const LocalDev *loc) {
+              hipStream_t s, const LocalDev *loc, double *spec) {
   jx_green *G = S->green;
   const Grid &g = G->g;
   const int C = G->sym;
-  int rc = launch_rows_forward(g, C, x, G->spec, loc ? *loc : S->local->dev,
+  if (!spec) spec = G->spec;
+  int rc = launch_rows_forward(g, C, x, spec, loc ? *loc : S->local->dev,
                                st, s);
   if (rc) return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, plain_axis(g), -1, G->spec, G->spec, nullptr, st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), -1, spec, spec, nullptr, st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, spec, spec, &G->dev, st, s)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, plain_axis(g), +1, G->spec, G->spec, nullptr, st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), +1, spec, spec, nullptr, st, s)))
     return rc;
   RowsInverseLs plain;
-  return launch_rows_inverse(g, C, G->spec, y, plain, st, nullptr, s);
+  return launch_rows_inverse(g, C, spec, y, plain, st, nullptr, s);
 }
 
 namespace {
@@ -62,12 +63,31 @@ constexpr int kCgBlocks = 4096;  // blocks of the vector kernels (partials)
 #endif
 constexpr int kScalarThreads = 1024;
 
+// Batched (tab != nullptr): block e nb + b works as block b of entry e's
+// solve, whose fields, scalars, weights, state and partial sums come from
+// its slot, so that every entry gets the single solve's partial sums.
 template <int C, int MODE>
 __global__ void __launch_bounds__(kThreads)
     k_cg_vec(double *x, double *r, double *p, double *q, int64_t N, double wd,
              double wt, const CgScalars *cs, const SolveState *st,
-             double *partials, const double *rhs, LocalDev loc) {
+             double *partials, const double *rhs, LocalDev loc,
+             const Slot *tab, int nb) {
 #pragma clang fp contract(off)
+  int blk = blockIdx.x, nblk = gridDim.x;
+  if (tab) {
+    const Slot *sl = tab + blk / nb;
+    blk -= (blk / nb) * nb;
+    nblk = nb;
+    x = sl->eps;
+    r = sl->cg_r;
+    p = sl->cg_p;
+    q = sl->cg_q;
+    cs = sl->cg_cs;
+    st = sl->st;
+    partials = sl->cg_partials;
+    wd = sl->cg_wd;
+    wt = sl->cg_wt;
+  }
   // the fill and the final sum run whatever the state
   constexpr bool FINAL =
       MODE == CG_FINAL || MODE == CG_FINAL_P || MODE == CG_STEP_X;
@@ -79,8 +99,8 @@ __global__ void __launch_bounds__(kThreads)
                            ? cs->alpha
                            : 0.;
   const double beta = MODE == CG_P ? cs->beta : 0.;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < N;
-       v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t v = (int64_t)blk * blockDim.x + threadIdx.x; v < N;
+       v += (int64_t)nblk * blockDim.x) {
     double e[C];
     if constexpr (MODE == CG_FILL) {
 #pragma unroll
@@ -213,8 +233,8 @@ __global__ void __launch_bounds__(kThreads)
     __syncthreads();
     double s1 = block_sum(acc1, red);
     if (threadIdx.x == 0) {
-      partials[2 * blockIdx.x] = s0;
-      partials[2 * blockIdx.x + 1] = s1;
+      partials[2 * blk] = s0;
+      partials[2 * blk + 1] = s1;
     }
   }
 }
@@ -232,10 +252,21 @@ __device__ double sum_partials(const double *partials, int nb, int stride,
 // Scalars from the partial sums: PH_INIT and PH_BETA read two columns
 // (stride 2), PH_ALPHA the first column (stride 2 from the vector kernel,
 // 1 from the fused pass E).
+// Batched (tab != nullptr): block e serves entry e, from its slot; the
+// partial sums are the pass-E ones (slot partials) or, with vec_sums, the
+// vector kernel's.
 template <int PHASE>
 __global__ void __launch_bounds__(kScalarThreads)
     k_cg_scalar(const double *partials, int nb, int stride, CgScalars *cs,
-                SolveState *st, double *res_hist) {
+                SolveState *st, double *res_hist, const Slot *tab,
+                int vec_sums) {
+  if (tab) {
+    const Slot *sl = tab + blockIdx.x;
+    partials = vec_sums ? sl->cg_partials : sl->partials;
+    cs = sl->cg_cs;
+    st = sl->st;
+    res_hist = sl->res_hist;
+  }
   if (solve_done(st)) return;
   __shared__ double red[kScalarThreads / 64];
   double s0 = sum_partials(partials, nb, stride, 0, red);
@@ -275,7 +306,19 @@ int vec(double *x, double *r, double *p, double *q, const Grid &g, double wd,
         double *partials, hipStream_t s, const double *rhs = nullptr,
         const LocalDev &loc = LocalDev()) {
   hipLaunchKernelGGL((k_cg_vec<C, MODE>), dim3(kCgBlocks), dim3(kThreads), 0,
-                     s, x, r, p, q, g.N, wd, wt, cs, st, partials, rhs, loc);
+                     s, x, r, p, q, g.N, wd, wt, cs, st, partials, rhs, loc,
+                     nullptr, kCgBlocks);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+// The vector kernel over the entries of a batch (kCgBlocks blocks each).
+template <int C, int MODE>
+int vec_batched(const Grid &g, const Batch &b, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_vec<C, MODE>), dim3(kCgBlocks * b.n),
+                     dim3(kThreads), 0, s, nullptr, nullptr, nullptr, nullptr,
+                     g.N, 0., 0., nullptr, nullptr, nullptr, nullptr,
+                     LocalDev(), b.tab, kCgBlocks);
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
@@ -285,29 +328,39 @@ int scalar(const double *partials, CgScalars *cs, SolveState *st,
            double *res_hist, hipStream_t s, int nb = kCgBlocks,
            int stride = 2) {
   hipLaunchKernelGGL((k_cg_scalar<PHASE>), dim3(1), dim3(kScalarThreads), 0, s,
-                     partials, nb, stride, cs, st, res_hist);
+                     partials, nb, stride, cs, st, res_hist, nullptr, 0);
+  JX_HIP(hipGetLastError());
+  return JX_OK;
+}
+
+template <int PHASE>
+int scalar_batched(const Batch &b, int nb, int stride, int vec_sums,
+                   hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_scalar<PHASE>), dim3(b.n), dim3(kScalarThreads), 0,
+                     s, nullptr, nb, stride, nullptr, nullptr, nullptr, b.tab,
+                     vec_sums);
   JX_HIP(hipGetLastError());
   return JX_OK;
 }
 
 template <int C>
-int cg_iteration(jx_solver *S, double *x, double wd, double wt, hipStream_t s) {
+int cg_iteration(jx_solver *S, const CgWork &w, double *x, double wd,
+                 double wt, hipStream_t s) {
   const Grid &g = S->green->g;
-  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
   int rc;
-  if ((rc = ls_matvec(S, p, q, S->st, s))) return rc;
-  if ((rc = vec<C, CG_QP>(x, r, p, q, g, wd, wt, S->cg_s, S->st,
-                          S->cg_partials, s)))
+  if ((rc = ls_matvec(S, w.p, w.q, w.st, s, nullptr, w.spec))) return rc;
+  if ((rc = vec<C, CG_QP>(x, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                          w.vec_partials, s)))
     return rc;
-  if ((rc = scalar<PH_ALPHA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+  if ((rc = scalar<PH_ALPHA>(w.vec_partials, w.cs, w.st, w.res_hist, s)))
     return rc;
-  if ((rc = vec<C, CG_XR>(x, r, p, q, g, wd, wt, S->cg_s, S->st,
-                          S->cg_partials, s)))
+  if ((rc = vec<C, CG_XR>(x, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                          w.vec_partials, s)))
     return rc;
-  if ((rc = scalar<PH_BETA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
+  if ((rc = scalar<PH_BETA>(w.vec_partials, w.cs, w.st, w.res_hist, s)))
     return rc;
-  return vec<C, CG_P>(x, r, p, q, g, wd, wt, S->cg_s, S->st, S->cg_partials,
-                      s);
+  return vec<C, CG_P>(x, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                      w.vec_partials, s);
 }
 
 // One iteration with the vector updates folded into the Green operator's
@@ -316,62 +369,94 @@ int cg_iteration(jx_solver *S, double *x, double wd, double wt, hipStream_t s) {
 // G[dC : p] with the block sums of p : C0 : q; then alpha, r -= alpha q
 // with its sums, beta: 768 Nh + 481 N bytes instead of 768 Nh + 673 N in 3D.
 template <int C>
-int cg_iteration_fused(jx_solver *S, double *x, double wd, double wt,
-                       hipStream_t s) {
+int cg_iteration_fused(jx_solver *S, const CgWork &w, double *x, double wd,
+                       double wt, hipStream_t s) {
   jx_green *G = S->green;
   const Grid &g = G->g;
-  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
   CgDev cg;
-  cg.r = r;
-  cg.p = p;
+  cg.r = w.r;
+  cg.p = w.p;
   cg.x = x;
-  cg.cs = S->cg_s;
-  int rc = launch_rows_forward_cg(g, C, cg, G->spec, S->local->dev, S->st, s);
+  cg.cs = w.cs;
+  int rc = launch_rows_forward_cg(g, C, cg, w.spec, S->local->dev, w.st, s);
   if (rc) return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, plain_axis(g), -1, G->spec, G->spec, nullptr, S->st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), -1, w.spec, w.spec, nullptr, w.st, s)))
     return rc;
-  if ((rc = launch_axis(g, C, pivot_axis(g), 0, G->spec, G->spec, &G->dev, S->st, s)))
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, w.spec, w.spec, &G->dev, w.st, s)))
     return rc;
   if (g.dim == 3 &&
-      (rc = launch_axis(g, C, plain_axis(g), +1, G->spec, G->spec, nullptr, S->st, s)))
+      (rc = launch_axis(g, C, plain_axis(g), +1, w.spec, w.spec, nullptr, w.st, s)))
     return rc;
   int nb = 0;
-  if ((rc = launch_rows_inverse_cg(g, C, G->spec, q, p, S->partials, wd, wt,
-                                   S->st, &nb, s)))
+  if ((rc = launch_rows_inverse_cg(g, C, w.spec, w.q, w.p, w.rows_partials,
+                                   wd, wt, w.st, &nb, s)))
     return rc;
-  if ((rc = scalar<PH_ALPHA>(S->partials, S->cg_s, S->st, S->res_dev, s, nb,
+  if ((rc = scalar<PH_ALPHA>(w.rows_partials, w.cs, w.st, w.res_hist, s, nb,
                              1)))
     return rc;
-  if ((rc = vec<C, CG_R>(x, r, p, q, g, wd, wt, S->cg_s, S->st,
-                         S->cg_partials, s)))
+  if ((rc = vec<C, CG_R>(x, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                         w.vec_partials, s)))
     return rc;
-  return scalar<PH_BETA>(S->cg_partials, S->cg_s, S->st, S->res_dev, s);
+  return scalar<PH_BETA>(w.vec_partials, w.cs, w.st, w.res_hist, s);
+}
+
+// q = G[dC : E], then r = p = rhs - q, x = 0 (x is eps), and the first
+// residual.
+template <int C>
+int cg_start_c(jx_solver *S, const CgWork &w, double *eps, const double *rhs,
+               double wd, double wt, hipStream_t s) {
+  const Grid &g = S->green->g;
+  int rc;
+  if ((rc = vec<C, CG_FILL>(eps, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                            w.vec_partials, s)))
+    return rc;
+  if ((rc = ls_matvec(S, eps, w.q, w.st, s, nullptr, w.spec))) return rc;
+  if ((rc = vec<C, CG_INIT>(eps, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                            w.vec_partials, s, rhs)))
+    return rc;
+  return scalar<PH_INIT>(w.vec_partials, w.cs, w.st, w.res_hist, s);
+}
+
+// eps = E + x (+ alpha p, the fused iterations' last x step)
+template <int C>
+int cg_finish_c(jx_solver *S, const CgWork &w, double *eps, double wd,
+                double wt, bool fused, hipStream_t s) {
+  const Grid &g = S->green->g;
+  if (fused)
+    return vec<C, CG_FINAL_P>(eps, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                              w.vec_partials, s);
+  return vec<C, CG_FINAL>(eps, w.r, w.p, w.q, g, wd, wt, w.cs, w.st,
+                          w.vec_partials, s);
+}
+
+bool cg_fused_grid(const Grid &g, int C) {
+  return JX_CG_FUSED && rows_cg_fusable(g, C);
 }
 
 template <int C>
 int cg_solve(jx_solver *S, double *eps, const double *rhs, int max_iter,
              double wd, double wt, hipStream_t s) {
   const Grid &g = S->green->g;
-  double *r = S->cg_buf, *p = r + C * g.N, *q = p + C * g.N;
-  int rc;
-  // q = G[dC : E], then r = p = rhs - q, x = 0 (x is eps)
-  if ((rc = vec<C, CG_FILL>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
-                            S->cg_partials, s)))
-    return rc;
-  if ((rc = ls_matvec(S, eps, q, S->st, s))) return rc;
-  if ((rc = vec<C, CG_INIT>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
-                            S->cg_partials, s, rhs)))
-    return rc;
-  if ((rc = scalar<PH_INIT>(S->cg_partials, S->cg_s, S->st, S->res_dev, s)))
-    return rc;
-  const bool fused = JX_CG_FUSED && rows_cg_fusable(g, C);
+  CgWork w;
+  w.r = S->cg_buf;
+  w.p = w.r + C * g.N;
+  w.q = w.p + C * g.N;
+  w.spec = S->green->spec;
+  w.rows_partials = S->partials;
+  w.vec_partials = S->cg_partials;
+  w.cs = S->cg_s;
+  w.st = S->st;
+  w.res_hist = S->res_dev;
+  int rc = cg_start_c<C>(S, w, eps, rhs, wd, wt, s);
+  if (rc) return rc;
+  const bool fused = cg_fused_grid(g, C);
   int enqueued = 0, chunk = 4;
   while (true) {
     int n = std::min(chunk, max_iter - enqueued);
     for (int i = 0; i < n; ++i) {
-      rc = fused ? cg_iteration_fused<C>(S, eps, wd, wt, s)
-                 : cg_iteration<C>(S, eps, wd, wt, s);
+      rc = fused ? cg_iteration_fused<C>(S, w, eps, wd, wt, s)
+                 : cg_iteration<C>(S, w, eps, wd, wt, s);
       if (rc) return rc;
     }
     enqueued += n;
@@ -381,12 +466,7 @@ int cg_solve(jx_solver *S, double *eps, const double *rhs, int max_iter,
     if (S->st_host->done || enqueued >= max_iter) break;
     chunk = std::min(chunk * 2, 64);
   }
-  // eps = E + x (+ alpha p, the fused iterations' last x step)
-  if (fused)
-    return vec<C, CG_FINAL_P>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
-                              S->cg_partials, s);
-  return vec<C, CG_FINAL>(eps, r, p, q, g, wd, wt, S->cg_s, S->st,
-                          S->cg_partials, s);
+  return cg_finish_c<C>(S, w, eps, wd, wt, fused, s);
 }
 
 // Polarization form (examples/square_inclusion.py:17-52, 77-85): with the
@@ -490,6 +570,68 @@ int cg_pol_solve(jx_solver *S, double *tau, double *eps, int max_iter,
 
 }  // namespace
 
+int64_t cg_vec_partials_doubles() { return 2 * (int64_t)kCgBlocks; }
+
+void cg_weights(const jx_green *G, double *wd, double *wt) {
+  // C0 inner product: 2 g0 (a . b) + (k0 - 2 g0 / dim) tr a tr b
+  const double g0 = G->g0, nu0 = G->nu0;
+  const int dim = G->g.dim;
+  const double k0 = dim == 2 ? g0 / (1. - 2. * nu0)
+                             : 2. / 3. * (1. + nu0) / (1. - 2. * nu0) * g0;
+  *wd = 2. * g0;
+  *wt = k0 - 2. * g0 / dim;
+}
+
+int cg_start(jx_solver *S, const CgWork &w, double *eps, const double *rhs,
+             double wd, double wt, hipStream_t s) {
+  return S->green->sym == 3 ? cg_start_c<3>(S, w, eps, rhs, wd, wt, s)
+                            : cg_start_c<6>(S, w, eps, rhs, wd, wt, s);
+}
+
+int cg_finish(jx_solver *S, const CgWork &w, double *eps, double wd,
+              double wt, hipStream_t s) {
+  const bool fused = cg_fused_grid(S->green->g, S->green->sym);
+  return S->green->sym == 3
+             ? cg_finish_c<3>(S, w, eps, wd, wt, fused, s)
+             : cg_finish_c<6>(S, w, eps, wd, wt, fused, s);
+}
+
+bool cg_batchable(const Grid &g, int C) {
+  return JX_CG_FUSED && batchable(g, C) && rows_cg_fusable(g, C);
+}
+
+int cg_iteration_batched(const jx_green *G, const LocalDev &loc,
+                         const Batch &b, hipStream_t s) {
+  const Grid &g = G->g;
+  const int C = G->sym;
+  if (!cg_batchable(g, C))
+    return fail(JX_EUNSUPPORTED, "batched conjugate gradients need register "
+                                 "FFT lengths on every axis");
+  const CgDev none;
+  int rc = launch_rows_forward_cg(g, C, none, nullptr, loc, nullptr, s, &b);
+  if (rc) return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, plain_axis(g), -1, nullptr, nullptr, nullptr,
+                        nullptr, s, &b)))
+    return rc;
+  if ((rc = launch_axis(g, C, pivot_axis(g), 0, nullptr, nullptr, &G->dev,
+                        nullptr, s, &b)))
+    return rc;
+  if (g.dim == 3 &&
+      (rc = launch_axis(g, C, plain_axis(g), +1, nullptr, nullptr, nullptr,
+                        nullptr, s, &b)))
+    return rc;
+  int nb = 0;
+  if ((rc = launch_rows_inverse_cg(g, C, nullptr, nullptr, nullptr, nullptr,
+                                   0., 0., nullptr, &nb, s, &b)))
+    return rc;
+  if ((rc = scalar_batched<PH_ALPHA>(b, nb, 1, 0, s))) return rc;
+  if ((rc = C == 3 ? vec_batched<3, CG_R>(g, b, s)
+                   : vec_batched<6, CG_R>(g, b, s)))
+    return rc;
+  return scalar_batched<PH_BETA>(b, kCgBlocks, 2, 1, s);
+}
+
 }  // namespace jx
 
 using namespace jx;
@@ -539,11 +681,8 @@ int jx_solver_solve_cg(jx_solver *S, double *eps, const double *E,
   *S->st_host = init;
   JX_HIP(hipMemcpyAsync(S->st, S->st_host, sizeof(SolveState),
                         hipMemcpyHostToDevice, s));
-  // C0 inner product weights: 2 g0 (a . b) + (k0 - 2 g0 / dim) tr a tr b
-  const double g0 = G->g0, nu0 = G->nu0;
-  const double k0 = dim == 2 ? g0 / (1. - 2. * nu0)
-                             : 2. / 3. * (1. + nu0) / (1. - 2. * nu0) * g0;
-  const double wd = 2. * g0, wt = k0 - 2. * g0 / dim;
+  double wd = 0., wt = 0.;
+  cg_weights(G, &wd, &wt);
   int rc = C == 3 ? cg_solve<3>(S, eps, rhs, max_iter, wd, wt, s)
                   : cg_solve<6>(S, eps, rhs, max_iter, wd, wt, s);
   if (rc) return rc;

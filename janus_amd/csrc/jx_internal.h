@@ -106,6 +106,7 @@ struct SolveState {
 // consecutive entries of a device table of slots, all on the same grid;
 // each block finds its entry from its index and takes every per-entry
 // pointer from the slot.
+struct CgScalars;
 struct Slot {
   double *eps = nullptr;        // iterate, real field (rows, L, C)
   double *spec = nullptr;       // spectrum (rows, nh, C, 2)
@@ -115,6 +116,13 @@ struct Slot {
   SolveState *st = nullptr;
   LocalDev loc;
   double daux[4] = {};          // Green operator constants of C0
+  // conjugate-gradient entries: residual, direction and A p fields, their
+  // scalars, the vector kernel's partial sums and the C0 inner product
+  // weights (eps holds the fluctuation x while the solve runs)
+  double *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr;
+  CgScalars *cg_cs = nullptr;
+  double *cg_partials = nullptr;
+  double cg_wd = 0., cg_wt = 0.;
 };
 
 struct Batch {
@@ -246,15 +254,19 @@ int rows_inverse_nblocks(const Grid &g, int ncomp);
 // Conjugate-gradient variants of passes A and E for the register lengths
 // (C = 3 or 6): pass A updates p and x (CgDev) and transforms dC : p; pass
 // E writes q = p + c2r(spec) / N and the block sums of p : C0 : q
-// (wd p . q + wt tr p tr q per voxel) into partials[block].
+// (wd p . q + wt tr p tr q per voxel) into partials[block].  Batched: the
+// fields, scalars, weights and state of each entry come from its slot (the
+// CgDev, spectrum, q, p, partials and weights arguments are then unused).
 bool rows_cg_fusable(const Grid &g, int ncomp);
 int launch_rows_forward_cg(const Grid &g, int ncomp, const CgDev &cg,
                            double *spec, const LocalDev &loc,
-                           const SolveState *st, hipStream_t s);
+                           const SolveState *st, hipStream_t s,
+                           const Batch *b = nullptr);
 int launch_rows_inverse_cg(const Grid &g, int ncomp, const double *spec,
                            double *q, const double *p, double *partials,
                            double wd, double wt, const SolveState *st,
-                           int *nblocks_out, hipStream_t s);
+                           int *nblocks_out, hipStream_t s,
+                           const Batch *b = nullptr);
 // Fused pass E of iteration k + pass A of iteration k + 1 (register path).
 bool rows_fusable(const Grid &g, int ncomp);
 int launch_rows_update_forward(const Grid &g, int ncomp, double *spec,

@@ -148,6 +148,12 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
     spec = sl->spec;
     loc = sl->loc;
     st = sl->st;
+    if constexpr (CGP) {
+      cg.r = sl->cg_r;
+      cg.p = sl->cg_p;
+      cg.x = sl->eps;
+      cg.cs = sl->cg_cs;
+    }
   }
   x = global_ptr(x);
   spec = global_ptr(spec);
@@ -339,14 +345,14 @@ bool rows_cg_fusable(const Grid &g, int ncomp) {
 
 int launch_rows_forward_cg(const Grid &g, int ncomp, const CgDev &cg,
                            double *spec, const LocalDev &loc,
-                           const SolveState *st, hipStream_t s) {
+                           const SolveState *st, hipStream_t s,
+                           const Batch *b) {
   if (!rows_cg_fusable(g, ncomp))
     return fail(JX_EUNSUPPORTED, "rows_forward_cg needs a register FFT length");
   FftLen P;
   int rc = get_fft_len((int)g.L, &P);
   if (rc) return rc;
-  return rows_forward_reg(g, ncomp, nullptr, spec, loc, st, P.W, s, nullptr,
-                          &cg);
+  return rows_forward_reg(g, ncomp, nullptr, spec, loc, st, P.W, s, b, &cg);
 }
 
 int launch_rows_forward(const Grid &g, int ncomp, const double *x, double *spec,

@@ -140,10 +140,15 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   const int64_t blk = batch_block(br, sl);
   if (sl) {
     spec = sl->spec;
-    y = sl->eps;
+    y = MODE == 3 ? sl->cg_q : sl->eps;
     ls.rhs = sl->rhs;
     ls.partials = sl->partials;
     st = sl->st;
+    if constexpr (MODE == 3) {
+      ls.cgp = sl->cg_p;
+      ls.wd = sl->cg_wd;
+      ls.wt = sl->cg_wt;
+    }
   }
   spec = global_ptr(spec);
   y = global_ptr(y);
@@ -359,7 +364,8 @@ template <class P, int C>
 static int launch_rows_inverse_cg_lc(const Grid &g, const double *spec,
                                      double *q, const LsArgs &la,
                                      const SolveState *st, const double2 *W,
-                                     hipStream_t s, int *nblocks_out) {
+                                     hipStream_t s, int *nblocks_out,
+                                     const Batch *b) {
   if constexpr (C == 3 || C == 6) {
     constexpr int L = P::L;
     constexpr int NP = reg_np<P>();
@@ -371,9 +377,10 @@ static int launch_rows_inverse_cg_lc(const Grid &g, const double *spec,
     if (nblocks_out) *nblocks_out = (int)nb;
     double scale = 1.0 / (double)g.N;
     set_lds_attr(k_rows_inverse_reg<P, C, NP, 3>, lds);
-    hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 3>), dim3((unsigned)nb),
+    hipLaunchKernelGGL((k_rows_inverse_reg<P, C, NP, 3>),
+                       dim3((unsigned)(nb * batch_count(b))),
                        dim3(NP * reg_tpp<P>()), lds, s, spec, q, g.rows, spec_layout(g, C), scale,
-                       W, la, st, batch_ref(nullptr, nb));
+                       W, la, st, batch_ref(b, nb));
     JX_HIP(hipGetLastError());
     return JX_OK;
   } else {
@@ -384,7 +391,7 @@ static int launch_rows_inverse_cg_lc(const Grid &g, const double *spec,
 int launch_rows_inverse_cg(const Grid &g, int ncomp, const double *spec,
                            double *q, const double *p, double *partials,
                            double wd, double wt, const SolveState *st,
-                           int *nblocks_out, hipStream_t s) {
+                           int *nblocks_out, hipStream_t s, const Batch *b) {
   if (!rows_cg_fusable(g, ncomp))
     return fail(JX_EUNSUPPORTED, "rows_inverse_cg needs a register FFT length");
   FftLen P;
@@ -397,7 +404,7 @@ int launch_rows_inverse_cg(const Grid &g, int ncomp, const double *spec,
   la.wt = wt;
   const int C = ncomp;
   JX_WITH_PLAN_C((int)g.L, g.plan[PASS_ROWS_INV], launch_rows_inverse_cg_lc,
-                 g, spec, q, la, st, P.W, s, nblocks_out);
+                 g, spec, q, la, st, P.W, s, nblocks_out, b);
 }
 
 int rows_inverse_nblocks(const Grid &g, int ncomp) {

@@ -1,0 +1,190 @@
+"""Batched conjugate gradients (jx_batch_add_cg, BatchSolver.add(...,
+method='cg')) vs the same solves by LSSolver.solve_cg one after another.
+
+A CG entry runs the kernels, block decomposition and summation order of
+solve_cg (its start and finish on its own, every fused iteration with the
+other entries of its chunk in each launch), so its field, residual history,
+iteration count and status must equal the sequential solve's bit for bit
+-- with entries of different reference media (their own C0 weights) in
+one launch, whatever the width, graph capture or autotuning, and beside
+fixed-point entries of the same grid.  solve_cg itself is pinned against
+the oracle's CG in test_gpu_cg.py."""
+
+import numpy as np
+import pytest
+
+import janus_amd.fft.serial as fft
+import janus_amd.material.elastic.linear.isotropic as material
+from janus_amd import microstructure as ms
+from janus_amd.batch import BatchSolver, effective_stiffness_batched
+from janus_amd.green import filtered, truncated
+from janus_amd.operators import BlockDiagonalOperator2D, \
+    BlockDiagonalOperator3D
+from janus_amd.solver import LSSolver, delta_stiffness, effective_stiffness
+
+pytestmark = pytest.mark.gpu
+
+
+def make_solver(phase, contrast, kind, g0=None):
+    dim = phase.ndim
+    mat_m = material.create(1.0, 0.3, dim)
+    mat_i = material.create(contrast, 0.25, dim)
+    mat0 = material.create(0.5 * (1.0 + contrast) if g0 is None else g0, 0.3,
+                           dim)
+    table = np.array([delta_stiffness(m, mat0) for m in (mat_m, mat_i)])
+    cls = BlockDiagonalOperator2D if dim == 2 else BlockDiagonalOperator3D
+    local = cls.from_phases(phase, table)
+    n = phase.shape
+    green = {'filtered': filtered, 'truncated': truncated}[kind](
+        mat0.green_operator(), n, 1.0, fft.create_real(n))
+    return LSSolver(green, local), mat0
+
+
+def field_solver(shape, seed):
+    import torch
+    rho = np.random.default_rng(seed).random(shape)
+    g = 1.0 + 4.0 * rho ** 3
+    mat0 = material.create(1.0, 0.3, 3)
+    k = 2. / 3. * 1.3 / 0.4 * g
+    sph = torch.from_numpy(3 * (k - mat0.k)).cuda()
+    dev = torch.from_numpy(2. * (g - mat0.g)).cuda()
+    S = LSSolver(filtered(mat0.green_operator(), shape, 1.0,
+                          fft.create_real(shape)),
+                 BlockDiagonalOperator3D.from_fields(sph, dev))
+    S._keep = (sph, dev)
+    return S
+
+
+def unit(sym, j):
+    E = np.zeros(sym)
+    E[j] = 1.0
+    return E
+
+
+@pytest.fixture(scope='module')
+def cases():
+    """(solver, E, tol, max_iter, rhs, method): two 32^3 filtered
+    microstructures with different reference media sharing CG launches (one
+    with C0 = the matrix, where the fixed point diverges), a truncated 32^3,
+    a field local operator on 16x32x64, a 2D 80x64 grid, a rhs field, a
+    max_iter stop, and fixed-point entries on the first grid."""
+    import torch
+    sA, _ = make_solver(ms.spheres(32, 4.0, 0.2, seed=3), 10.0, 'filtered')
+    sM, _ = make_solver(ms.spheres(32, 3.0, 0.3, seed=4), 40.0, 'filtered',
+                        g0=1.0)
+    sB, _ = make_solver(ms.spheres(32, 4.0, 0.15, seed=5), 6.0, 'truncated')
+    sC = field_solver((16, 32, 64), 7)
+    s2, _ = make_solver(ms.spheres(80, 6.0, 0.2, seed=8, dim=2)[:, :64]
+                        .copy(), 8.0, 'filtered')
+    out = []
+    for j in range(6):
+        out.append((sA, unit(6, j), 1e-10, 500, None, 'cg'))
+    for j in (0, 5):
+        out.append((sM, unit(6, j), 1e-10, 500, None, 'cg'))
+    for j in (1, 3):
+        out.append((sB, unit(6, j), 1e-10, 500, None, 'cg'))
+    for j in (0, 5):
+        out.append((sC, unit(6, j), 1e-10, 500, None, 'cg'))
+    for j in range(3):
+        out.append((s2, unit(3, j), 1e-10, 500, None, 'cg'))
+    rhs = torch.from_numpy(np.random.default_rng(10).standard_normal(
+        sA.shape) * 0.1).cuda()
+    out.append((sA, unit(6, 2), 1e-10, 500, rhs, 'cg'))
+    out.append((sM, unit(6, 1), 1e-10, 5, None, 'cg'))
+    for j in (0, 4):
+        out.append((sA, unit(6, j), 1e-10, 1000, None, 'basic'))
+    return out
+
+
+@pytest.fixture(scope='module')
+def sequential(cases):
+    res = []
+    for S, E, tol, mi, rhs, method in cases:
+        solve = S.solve_cg if method == 'cg' else S.solve
+        r = solve(E, tol=tol, max_iter=mi, rhs=rhs)
+        res.append((r.eps.cpu().numpy(), r.residuals.copy(), r.iterations,
+                    r.status))
+    return res
+
+
+SCHEDULES = [
+    dict(iters_per_launch=8, graph=False, autotune=False),
+    dict(iters_per_launch=3, graph=True, autotune=False),
+    dict(iters_per_launch=5, graph=False, autotune=True),
+]
+
+
+@pytest.mark.parametrize('schedule', SCHEDULES)
+def test_batched_cg_matches_sequential_bitwise(cases, sequential, schedule):
+    batch = BatchSolver(**schedule)
+    for S, E, tol, mi, rhs, method in cases:
+        batch.add(S, E, tol=tol, max_iter=mi, rhs=rhs, method=method)
+    res = batch.run()
+    for k, ((eps, hist, its, status), r) in enumerate(zip(sequential, res)):
+        assert r.iterations == its, k
+        assert r.status == status, k
+        np.testing.assert_array_equal(r.residuals, hist)
+        np.testing.assert_array_equal(r.eps.cpu().numpy(), eps)
+    # the C0 = matrix entries converge (the fixed point would diverge)
+    assert res[6].status == 'converged' and res[7].status == 'converged'
+    assert res[-3].status == 'max_iter' and res[-3].iterations == 5
+
+
+def test_cg_and_fixed_point_entries_never_share_a_group(cases):
+    batch = BatchSolver()
+    for S, E, tol, mi, rhs, method in cases:
+        batch.add(S, E, tol=tol, max_iter=mi, rhs=rhs, method=method)
+    sizes = sorted(size for shape, size, width, batched in batch.groups()
+                   if shape == (32, 32, 32))
+    # filtered CG (sA, sM), CG with rhs, truncated CG, fixed point
+    assert sizes == [1, 2, 2, 9]
+
+
+def test_widths_give_identical_results(cases, sequential):
+    batch = BatchSolver(iters_per_launch=4, graph=False)
+    for S, E, tol, mi, rhs, method in cases:
+        batch.add(S, E, tol=tol, max_iter=mi, rhs=rhs, method=method)
+    for g in range(len(batch.groups())):
+        batch.set_width(g, 1 + g % 3)
+    res = batch.run()
+    for (eps, hist, its, status), r in zip(sequential, res):
+        assert r.iterations == its
+        np.testing.assert_array_equal(r.eps.cpu().numpy(), eps)
+
+
+def test_effective_stiffness_batched_cg_matches_sequential():
+    solvers, mats0 = [], []
+    for seed, n, c in ((1, 32, 50.0), (2, 16, 20.0)):
+        S, m0 = make_solver(ms.spheres(n, 3.0, 0.2, seed=seed), c,
+                            'filtered', g0=1.0)
+        solvers.append(S)
+        mats0.append(m0)
+    Cs, its, sts, batch = effective_stiffness_batched(
+        solvers, mats0, tol=1e-10, max_iter=500, method='cg', graph=False)
+    for S, m0, C, it, st in zip(solvers, mats0, Cs, its, sts):
+        Cseq, its_seq, _ = effective_stiffness(S, m0, tol=1e-10, max_iter=500,
+                                               method='cg')
+        assert it == its_seq
+        assert all(x == 'converged' for x in st)
+        np.testing.assert_array_equal(C, Cseq)
+
+
+def test_batched_cg_refusals():
+    import torch
+    # a grid without register FFT lengths on every axis
+    S, _ = make_solver(ms.spheres(30, 3.0, 0.2, seed=9)[:, :20, :24].copy(),
+                       3.0, 'truncated')
+    with pytest.raises(NotImplementedError):
+        BatchSolver().add(S, unit(6, 0), method='cg')
+    # a non-symmetric local operator
+    shape = (16, 16, 16)
+    a = torch.zeros(shape + (6, 6), dtype=torch.float64, device='cuda')
+    a[..., 0, 1] = 1.0
+    m0 = material.create(1.0, 0.3, 3)
+    T = LSSolver(filtered(m0.green_operator(), shape, 1.0,
+                          fft.create_real(shape)),
+                 BlockDiagonalOperator3D.from_matrix_field(a))
+    with pytest.raises(ValueError, match='symmetric'):
+        BatchSolver().add(T, unit(6, 0), method='cg')
+    with pytest.raises(ValueError, match='method'):
+        BatchSolver().add(T, unit(6, 0), method='gmres')
