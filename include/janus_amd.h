@@ -291,8 +291,9 @@ int jx_batch_add(jx_batch *batch, jx_solver *solver, double *eps,
                  const double *E, const double *rhs, double tol, int max_iter,
                  double res_scale, int *index);
 /* The same entry solved by conjugate gradients: the arguments, the result
- * (eps = E + x), the iterates, residual history and iteration count of
- * jx_solver_solve_cg (janus drives CG from its examples,
+ * (eps = E + x; the content of eps on entry is not used, so a second run
+ * solves again from E), the iterates, residual history and iteration count
+ * of jx_solver_solve_cg (janus drives CG from its examples,
  * examples/square_inclusion.py:40-47, 71-86).  Needs a symmetric local
  * operator (else JX_EINVAL) and register FFT lengths on every axis (else
  * JX_EUNSUPPORTED).  Also allocates the entry's r, p, q fields, partial

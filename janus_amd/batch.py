@@ -65,7 +65,9 @@ class BatchSolver:
         :meth:`LSSolver.solve_cg`).
 
         Returns the entry index.  ``eps`` (default: E everywhere) is the
-        entry's own GPU field, overwritten by the last iterate."""
+        entry's own GPU field, overwritten by the last iterate; a fixed-point
+        entry starts from its content (a second run continues), a CG entry
+        from E, as :meth:`LSSolver.solve_cg`."""
         if method not in ('basic', 'cg'):
             raise ValueError("method must be 'basic' or 'cg'")
         t = _device.torch()

@@ -12,10 +12,12 @@ finishes in seconds.
 * Batched (grids {64, 80, 96, 128, 160}^3, Boolean spheres r = 8,
   contrast logU[2, 100], filtered Gamma0, six Mandel unit loads each): ten
   of the configuration's 32 draws -- the lowest and the highest contrast of
-  every grid size, up to 73 -- as one batch of 60 right-hand sides.  Every
-  batched entry equals its sequential solve bit for bit (iterates,
-  residual histories, iteration counts); one entry per grid size equals
-  the oracle's basic scheme (square_basic.py:83-92) for its first iterates.
+  every grid size, up to 73 -- as one batch of 60 right-hand sides, by the
+  fixed point and by conjugate gradients.  Every batched entry equals its
+  sequential solve (LSSolver.solve, LSSolver.solve_cg) bit for bit
+  (iterates, residual histories, iteration counts); one entry per grid size
+  equals the oracle's basic scheme (square_basic.py:83-92) for its first
+  iterates.
 * Differentiable (128^3 smoothed density rho, mu = 1 + 19 rho^3, nu = 0.3,
   E = e_xy, L = (C_1212 - 3)^2, C_1212 from janus's post-process
   <sigma> = C0 : E + <tau>, square_basic.py:95-97): the gradients of
@@ -176,6 +178,32 @@ def test_batched_configuration_bitwise_and_oracle():
     print('batched configuration: %d entries, %d-%d iterations, widths %s'
           % (len(rhs), min(its), max(its), [g[2] for g in batch.groups()]),
           flush=True)
+    del batch, bat_eps, seq_eps
+    torch.cuda.empty_cache()
+
+    # the same right-hand sides by conjugate gradients, batched against
+    # sequential solve_cg
+    seq_eps = fields()
+    seq = [built[i][0].solve_cg(_unit(j), tol=1e-10, max_iter=max_iter,
+                                eps=e)
+           for (i, j), e in zip(rhs, seq_eps)]
+    assert all(r.status == 'converged' for r in seq)
+    batch = BatchSolver(iters_per_launch=8, graph=False, autotune=True)
+    bat_eps = fields()
+    for (i, j), e in zip(rhs, bat_eps):
+        batch.add(built[i][0], _unit(j), tol=1e-10, max_iter=max_iter, eps=e,
+                  method='cg')
+    res = batch.run()
+    for k, (r, s) in enumerate(zip(res, seq)):
+        assert r.iterations == s.iterations, (rhs[k], r.iterations,
+                                              s.iterations)
+        assert r.status == s.status
+        assert np.array_equal(r.residuals, s.residuals), rhs[k]
+        assert torch.equal(bat_eps[k], seq_eps[k]), rhs[k]
+    its_cg = [s.iterations for s in seq]
+    print('batched configuration, CG: %d-%d iterations (fixed point %d-%d)'
+          % (min(its_cg), max(its_cg), min(its), max(its)), flush=True)
+    assert sum(its_cg) < sum(its)
     del batch, bat_eps, seq_eps
     torch.cuda.empty_cache()
 
