@@ -738,7 +738,7 @@ def workload_batched(args, emit=True):
     eps_b = fields()
     batch = BatchSolver(iters_per_launch=args.iters_per_launch,
                         graph=args.graph,
-                        autotune=True)
+                        autotune=True, streams=args.batch_streams)
     for (i, j), e in zip(rhs, eps_b):
         batch.add(solvers[i], _unit(6, j), tol=1e-10, max_iter=args.max_iter,
                   eps=e)
@@ -819,7 +819,8 @@ def workload_batched(args, emit=True):
         t_seq_cg = time.perf_counter() - t0
         eps_cb = fields()
         bcg = BatchSolver(iters_per_launch=args.iters_per_launch,
-                          graph=args.graph, autotune=True)
+                          graph=args.graph, autotune=True,
+                          streams=args.batch_streams)
         for (i, j), e in zip(rhs, eps_cb):
             bcg.add(solvers[i], _unit(6, j), tol=1e-10,
                     max_iter=args.max_iter, eps=e, method='cg')
@@ -871,6 +872,7 @@ def workload_batched(args, emit=True):
         'same_iteration_counts': same_iters,
         'converged': conv,
         'iterations_total': int(sum(its)),
+        'streams': args.batch_streams,
     }
     if cg:
         cg['bound_seconds_copy_bw'] = round(
@@ -895,6 +897,7 @@ def workload_batched(args, emit=True):
                   'iterations_min_max': [int(min(its)), int(max(its))],
                   'replays': replays, 'graph_captures': captures,
                   'iters_per_launch': args.iters_per_launch,
+                  'streams': args.batch_streams,
                   'groups': groups,
                   'per_grid': per_grid,
                   'parity': {'entries_bitwise_equal': equal,
@@ -1397,6 +1400,9 @@ def main():
     ap.add_argument('--batch-count', type=int, default=32,
                     help='microstructures of the batched workload')
     ap.add_argument('--iters-per-launch', type=int, default=8)
+    ap.add_argument('--batch-streams', type=int, default=2,
+                    help='batched workload: streams the groups of different '
+                         'grids run on at once (enqueued replays)')
     ap.add_argument('--graph', action='store_true',
                     help='batched workload: replay captured hipGraphs '
                          '(default: enqueue the launches, 2.5%% faster here)')

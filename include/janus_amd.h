@@ -322,6 +322,14 @@ int jx_batch_set_width(jx_batch *batch, int group, int width);
  * entry-iteration run is kept (default 0: whole groups). */
 int jx_batch_set_schedule(jx_batch *batch, int iters_per_launch, int graph,
                           int autotune);
+/* Concurrent groups (no janus counterpart): nstreams in [1, 8] (default 1).
+ * Above 1, each enqueued replay (graph = 0) after the autotuning ones forks
+ * the groups onto up to nstreams streams (groups balanced by running
+ * entries times voxels) and joins them back, so that the launches of small
+ * grids run beside those of large ones.  Captured graphs (graph = 1) keep
+ * one stream.  Groups share no fields: every entry's iterates are
+ * unchanged. */
+int jx_batch_set_streams(jx_batch *batch, int nstreams);
 /* Runs every entry to its stop on an internal stream after synchronizing
  * `stream`; returns when all have stopped.  A batch may be run again: each
  * entry restarts from the content of its eps. */

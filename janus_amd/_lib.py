@@ -110,6 +110,7 @@ SIGNATURES = [
     ('jx_batch_set_width', ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     ('jx_batch_set_schedule', ctypes.c_int,
      [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ('jx_batch_set_streams', ctypes.c_int, [_VP, ctypes.c_int]),
     ('jx_batch_run', ctypes.c_int, [_VP, _VP]),
     ('jx_batch_result', ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP]),
     ('jx_batch_stats', ctypes.c_int, [_VP, _VP, _VP, _VP]),

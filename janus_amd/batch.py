@@ -15,7 +15,10 @@ different grids -- and runs them together through ``jx_batch_run``:
   captured hipGraph, captured again only when a chunk has finished;
 * with ``autotune=True`` the first replays time the candidate widths (all,
   12, 6, 3, 2, 1 entries per launch) of every group and keep the one with
-  the least time per entry-iteration.
+  the least time per entry-iteration;
+* with ``streams > 1`` and ``graph=False`` the groups of different grids
+  run side by side on that many streams, so that small grids' launches fill
+  what large ones leave of the GPU.
 
 Each entry stops after the first iteration whose residual is below its
 ``tol``, exactly as :meth:`LSSolver.solve` would, with the same iterates.
@@ -41,9 +44,13 @@ class BatchSolver:
             per replay).
         graph: replay a captured hipGraph (True) or enqueue the launches.
         autotune: pick each group's width by timing the first replays.
+        streams: with ``graph=False``, groups of different grids run on up
+            to this many streams at once (1: one after another); captured
+            graphs keep one stream.
     """
 
-    def __init__(self, iters_per_launch=8, graph=True, autotune=False):
+    def __init__(self, iters_per_launch=8, graph=True, autotune=False,
+                 streams=1):
         _lib.require_gpu()
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().jx_batch_create(ctypes.byref(h)))
@@ -51,6 +58,7 @@ class BatchSolver:
         self._entries = []
         _lib.check(_lib.lib().jx_batch_set_schedule(
             h, int(iters_per_launch), int(bool(graph)), int(bool(autotune))))
+        _lib.check(_lib.lib().jx_batch_set_streams(h, int(streams)))
 
     def __del__(self):
         h = getattr(self, '_handle', None)

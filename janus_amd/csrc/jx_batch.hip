@@ -92,6 +92,13 @@ struct jx_batch {
   int cap = 0;
   hipStream_t stream = nullptr;
   hipGraphExec_t exec = nullptr;
+  // concurrent groups: the groups' launches of a replay go to `nstreams`
+  // streams (side streams forked from and joined back into `stream`)
+  int nstreams = 1;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> side_done;
+  hipEvent_t fork = nullptr;
+  std::vector<int> group_stream;  // stream of each group, < nstreams
   // statistics of the last run
   double ms = 0;
   int replays = 0, captures = 0;
@@ -240,6 +247,74 @@ int enqueue_group(jx_batch *B, const Group &gr, int width, int iters,
   return JX_OK;
 }
 
+// Side streams and events for nstreams concurrent group streams (default
+// priority, non-blocking).
+int ensure_side_streams(jx_batch *B) {
+  const int ns = B->nstreams;
+  if (!B->fork) JX_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+  while ((int)B->side.size() < ns) {
+    hipStream_t st = nullptr;
+    JX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    B->side.push_back(st);
+    hipEvent_t ev = nullptr;
+    JX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    B->side_done.push_back(ev);
+  }
+  return JX_OK;
+}
+
+// Groups to streams, longest first onto the least loaded stream, the load
+// of a group being its running entries times their voxels (the bytes of
+// one iteration of each).
+void assign_streams(jx_batch *B) {
+  const int ng = (int)B->groups.size();
+  const int ns = std::max(1, std::min(B->nstreams, ng));
+  std::vector<std::pair<double, int>> order;
+  for (int g = 0; g < ng; ++g) {
+    const Group &gr = B->groups[g];
+    double w = 0.;
+    if (!gr.act.empty()) {
+      const Grid &G = B->ents[gr.act[0]].S->green->g;
+      w = (double)G.N * (double)gr.act.size();
+    }
+    order.push_back({-w, g});
+  }
+  std::sort(order.begin(), order.end());
+  std::vector<double> load(ns, 0.);
+  B->group_stream.assign(ng, 0);
+  for (const auto &o : order) {
+    int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    B->group_stream[o.second] = j;
+    load[j] -= o.first;
+  }
+}
+
+// One replay of every group: on s alone, or (concurrent) forked to the side
+// streams (each group on its assigned stream, its chunks in order) and
+// joined back into s.  Groups share no fields, so the order of their
+// launches changes no result.
+int enqueue_replay(jx_batch *B, int iters, hipStream_t s, bool concurrent) {
+  int rc = JX_OK;
+  const int ng = (int)B->groups.size();
+  if (!concurrent || B->nstreams <= 1 || ng <= 1) {
+    for (const Group &gr : B->groups)
+      if ((rc = enqueue_group(B, gr, gr.width, iters, false, s))) return rc;
+    return JX_OK;
+  }
+  const int ns = std::min(B->nstreams, ng);
+  JX_HIP(hipEventRecord(B->fork, s));
+  for (int j = 0; j < ns; ++j) JX_HIP(hipStreamWaitEvent(B->side[j], B->fork, 0));
+  for (int g = 0; g < ng && !rc; ++g)
+    rc = enqueue_group(B, B->groups[g], B->groups[g].width, iters, false,
+                       B->side[B->group_stream[g]]);
+  // join even after a failed enqueue, so that s waits for what was queued
+  for (int j = 0; j < ns; ++j) {
+    JX_HIP(hipEventRecord(B->side_done[j], B->side[j]));
+    JX_HIP(hipStreamWaitEvent(s, B->side_done[j], 0));
+  }
+  return rc;
+}
+
 // Device table of the running entries, group after group.
 int upload_table(jx_batch *B, hipStream_t s) {
   std::vector<Slot> tab;
@@ -253,6 +328,7 @@ int upload_table(jx_batch *B, hipStream_t s) {
       }
   }
   B->built = (int)tab.size();
+  assign_streams(B);
   if (!tab.empty())
     JX_HIP(hipMemcpyAsync(B->tab, tab.data(), sizeof(Slot) * tab.size(),
                           hipMemcpyHostToDevice, s));
@@ -277,11 +353,9 @@ void drop_graph(jx_batch *B) {
 int capture(jx_batch *B, hipStream_t s) {
   drop_graph(B);
   JX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-  int rc = JX_OK;
-  for (const Group &gr : B->groups) {
-    rc = enqueue_group(B, gr, gr.width, B->iters_per_launch, false, s);
-    if (rc) break;
-  }
+  // captured on s alone: a capture forked onto the side streams aborted the
+  // process with conjugate-gradient entries, for a cause not found
+  int rc = enqueue_replay(B, B->iters_per_launch, s, false);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s, &graph);
   if (rc) {
@@ -362,6 +436,10 @@ int run(jx_batch *B, hipStream_t caller) {
   drop_graph(B);
   if (!B->stream)
     JX_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+  if (B->nstreams > 1) {
+    int rcs = ensure_side_streams(B);
+    if (rcs) return rcs;
+  }
   hipStream_t s = B->stream;
   JX_HIP(hipStreamSynchronize(caller));
 
@@ -493,9 +571,9 @@ int run(jx_batch *B, hipStream_t caller) {
     if (B->graph) {
       if (!B->exec && (rc = capture(B, s))) return rc;
       JX_HIP(hipGraphLaunch(B->exec, s));
-    } else {
-      for (const Group &gr : B->groups)
-        if ((rc = enqueue_group(B, gr, gr.width, K, false, s))) return rc;
+    } else if ((rc = enqueue_replay(B, K, s, true))) {
+      (void)hipStreamSynchronize(s);
+      return rc;
     }
     if ((rc = read_states(B, s))) return rc;
     ++B->replays;
@@ -531,6 +609,9 @@ int jx_batch_destroy(jx_batch *B) {
   free_run_state(B);
   for (Entry &e : B->ents) free_entry(e);
   if (B->stream) (void)hipStreamDestroy(B->stream);
+  for (hipStream_t st : B->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : B->side_done) (void)hipEventDestroy(ev);
+  if (B->fork) (void)hipEventDestroy(B->fork);
   delete B;
   return JX_OK;
 }
@@ -669,11 +750,20 @@ int jx_batch_set_schedule(jx_batch *B, int iters_per_launch, int graph,
   return JX_OK;
 }
 
+int jx_batch_set_streams(jx_batch *B, int nstreams) {
+  if (!B) return fail(JX_EINVAL, "jx_batch_set_streams: NULL batch");
+  if (nstreams < 1 || nstreams > 8)
+    return fail(JX_EINVAL, "nstreams must be in [1, 8]");
+  B->nstreams = nstreams;
+  return JX_OK;
+}
+
 int jx_batch_run(jx_batch *B, void *stream) {
   if (!B) return fail(JX_EINVAL, "jx_batch_run: NULL batch");
   int rc = run(B, (hipStream_t)stream);
   if (rc) {
     drop_graph(B);
+    for (hipStream_t st : B->side) (void)hipStreamSynchronize(st);
     if (B->stream) (void)hipStreamSynchronize(B->stream);
   }
   return rc;

@@ -118,6 +118,10 @@ SCHEDULES = [
     dict(iters_per_launch=8, graph=True, autotune=False),
     dict(iters_per_launch=3, graph=False, autotune=False),
     dict(iters_per_launch=5, graph=True, autotune=True),
+    # groups forked onto concurrent streams (captured and enqueued)
+    dict(iters_per_launch=8, graph=True, autotune=False, streams=4),
+    dict(iters_per_launch=3, graph=False, autotune=False, streams=3),
+    dict(iters_per_launch=5, graph=False, autotune=True, streams=8),
 ]
 
 
