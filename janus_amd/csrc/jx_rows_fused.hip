@@ -68,7 +68,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   constexpr int NT = NP * TPP;
   constexpr int RB = 2 * NP / C;
   extern __shared__ double2 smem[];
-  __shared__ double red[4];
+  // one partial sum per wave (384-thread blocks: 512- and 1024-point plans)
+  __shared__ double red[(NT + 63) / 64];
   __shared__ double2 twl[JX_ROWS_TQ ? L / 4 : 1];
   const int tid = threadIdx.x;
   const int64_t row0 = blk * RB;

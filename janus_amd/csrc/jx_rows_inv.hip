@@ -164,7 +164,8 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
   constexpr int RB = 2 * NP / C;
   constexpr bool LS_UPDATE = MODE == 1 || MODE == 2;
   extern __shared__ double2 smem[];
-  __shared__ double red[4];
+  // one partial sum per wave (384-thread blocks: 512- and 1024-point plans)
+  __shared__ double red[(NT + 63) / 64];
   const int tid = threadIdx.x;
   const int64_t row0 = blk * RB;
   const int rb = (int)min((int64_t)RB, nrows - row0);

@@ -206,6 +206,35 @@ def test_plan_variants_match_oracle(variant):
         assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12
 
 
+@pytest.mark.parametrize('n, variant', [((6, 10, 512), 1),
+                                        ((4, 6, 1024), 0)])
+def test_six_wave_rows_blocks_residuals_match_oracle(n, variant):
+    """The last-axis passes of the 8-value 512-point plan and of the
+    1024-point plan run 384 threads (six waves) per block: their residual
+    partial sums (one per wave, then per block) against the oracle's
+    residual history, fused and five-pass, iterate for iterate."""
+    phase = np.random.default_rng(n[2]).integers(0, 2, size=n).astype(
+        np.uint8)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(10.0, 0.25, 3)
+    mat0 = material.create(5.5, 0.3, 3)
+    E = np.array([0.3, -0.1, 0.2, 0.0, 0.5, 1.0])
+    eps_ref, res_ref = None, None
+    for fused in (True, False):
+        solver, table = build(phase, [mat_m, mat_i], mat0, 'filtered')
+        for p in solver.green.PLAN_PASSES:
+            solver.green.set_plan(p, variant)
+        solver.fused = fused
+        r = solver.solve(E, tol=0., max_iter=6)
+        if eps_ref is None:
+            eps_ref, res_ref = jo.basic_scheme(
+                table[phase, 0], table[phase, 1], E, 'filtered', mat0.g,
+                mat0.nu, tol=0., max_iter=6)
+        np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-10)
+        eps = r.eps.cpu().numpy()
+        assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12
+
+
 def test_autotune_keeps_results():
     n = 64
     phase = ms.spheres(n, 4.0, 0.2, seed=6)
