@@ -397,13 +397,27 @@ constexpr int reg_nseq() {
   return reg_block_threads<P>() / reg_tpp<P>();
 }
 
+// Pair sequences per block of the last-axis (rows) kernels.  Plans with 64
+// threads per transform (512 points with 8 values, 1024 points) take three
+// pairs, 192 threads and one 6-component row per block: at 512^3 the fused
+// update pass went from 6.06 to 5.39 ms against six pairs (384 threads, 52
+// KB of LDS, three blocks per CU).  The others take reg_np.
+#ifndef JX_ROWS_NP64
+#define JX_ROWS_NP64 3
+#endif
+template <class P>
+constexpr int rows_np() {
+  return (reg_tpp<P>() >= 64 && JX_ROWS_NP64 > 0) ? JX_ROWS_NP64
+                                                  : reg_np<P>();
+}
+
 // Rows per block of the register last-axis kernels with plan variant k
 // (0 = no register FFT for L).
 static inline int reg_rows_per_block(int L, int C, int k) {
   if (!reg_fft_length(L) || (C != 1 && C != 3 && C != 6)) return 0;
   int np = 0;
   with_plan(L, k, [&](auto plan) -> int {
-    np = reg_np<decltype(plan)>();
+    np = rows_np<decltype(plan)>();
     return 0;
   });
   return 2 * np / C;

@@ -27,14 +27,23 @@ __global__ void __launch_bounds__(kFinalizeThreads)
   // the summation order is fixed, so the result is deterministic
   double acc[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   const int stride = kFinalizeThreads * 8;
-  // four strides per unrolled step: 32 loads in flight per thread
+  // whole strides without bounds checks (four per unrolled step: 32 loads
+  // in flight per thread), then the checked tail; accumulator u sums the
+  // same partials in the same order either way
+  int i0 = 0;
 #pragma unroll 4
-  for (int i0 = 0; i0 < nb; i0 += stride) {
+  for (; i0 + stride <= nb; i0 += stride) {
+    double x[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      int i = i0 + u * kFinalizeThreads + threadIdx.x;
-      if (i < nb) acc[u] += partials[i];
-    }
+    for (int u = 0; u < 8; ++u)
+      x[u] = partials[i0 + u * kFinalizeThreads + threadIdx.x];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += x[u];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    int i = i0 + u * kFinalizeThreads + threadIdx.x;
+    if (i < nb) acc[u] += partials[i];
   }
   double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
                ((acc[4] + acc[5]) + (acc[6] + acc[7]));

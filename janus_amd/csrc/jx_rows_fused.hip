@@ -276,7 +276,7 @@ static int launch_rows_update_forward_reg_lc(const Grid &g, double *spec,
                                              const Batch *b) {
   constexpr int L = P::L;
   if constexpr (C == 3 || C == 6) {
-    constexpr int NP = reg_np<P>();
+    constexpr int NP = rows_np<P>();
     constexpr int RB = 2 * NP / C;
     size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
     int rc = check_lds(lds, "rows_update_forward");

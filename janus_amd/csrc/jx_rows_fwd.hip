@@ -288,7 +288,7 @@ static int launch_rows_forward_reg_lc(const Grid &g, const double *x,
                                       hipStream_t s, const Batch *b,
                                       const CgDev *cg) {
   constexpr int L = P::L;
-  constexpr int NP = reg_np<P>();
+  constexpr int NP = rows_np<P>();
   constexpr int RB = 2 * NP / C;
   size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
   int rc = check_lds(lds, "rows_forward");

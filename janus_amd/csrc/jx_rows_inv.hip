@@ -322,7 +322,7 @@ static int launch_rows_inverse_reg_lc(const Grid &g, const double *spec,
                                       const SolveState *st, const double2 *W,
                                       hipStream_t s, const Batch *b) {
   constexpr int L = P::L;
-  constexpr int NP = reg_np<P>();
+  constexpr int NP = rows_np<P>();
   constexpr int RB = 2 * NP / C;
   size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
   int rc = check_lds(lds, "rows_inverse");
@@ -369,7 +369,7 @@ static int launch_rows_inverse_cg_lc(const Grid &g, const double *spec,
                                      const Batch *b) {
   if constexpr (C == 3 || C == 6) {
     constexpr int L = P::L;
-    constexpr int NP = reg_np<P>();
+    constexpr int NP = rows_np<P>();
     constexpr int RB = 2 * NP / C;
     size_t lds = (size_t)NP * reg_lds_stride(L) * sizeof(double2);
     int rc = check_lds(lds, "rows_inverse_cg");

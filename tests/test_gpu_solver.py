@@ -208,11 +208,12 @@ def test_plan_variants_match_oracle(variant):
 
 @pytest.mark.parametrize('n, variant', [((6, 10, 512), 1),
                                         ((4, 6, 1024), 0)])
-def test_six_wave_rows_blocks_residuals_match_oracle(n, variant):
+def test_64_thread_transform_rows_residuals_match_oracle(n, variant):
     """The last-axis passes of the 8-value 512-point plan and of the
-    1024-point plan run 384 threads (six waves) per block: their residual
-    partial sums (one per wave, then per block) against the oracle's
-    residual history, fused and five-pass, iterate for iterate."""
+    1024-point plan (64 threads per transform, one 6-component row per
+    block): their residual partial sums (one per wave, then per block)
+    against the oracle's residual history, fused and five-pass, iterate for
+    iterate."""
     phase = np.random.default_rng(n[2]).integers(0, 2, size=n).astype(
         np.uint8)
     mat_m = material.create(1.0, 0.3, 3)
