@@ -14,18 +14,21 @@ namespace jx {
 // iterate never travels back from memory.  MODE 1: E only; 2: E + rhs.
 // Minimum waves per SIMD requested for the fused update pass: 3 (about 70
 // registers spilled to scratch) measured faster than 2 at 256^3.
+#ifndef JX_FUSED_PREFETCH
+#define JX_FUSED_PREFETCH 1
+#endif
 #ifndef JX_FUSED_MIN_WAVES
 #define JX_FUSED_MIN_WAVES 3
 #endif
 // Same for the 8-value plans (1: the compiler's choice), and for the
 // 8-value 3D isotropic-family kernels without a field right-hand side,
-// which fit 5 waves per SIMD without spilling (6 blocks of 26 KB LDS per CU
-// instead of 5; the other 8-value kernels would spill there).
+// which fit 4 waves per SIMD with the prefetched iterate (5 without it, and
+// the prefetch at 5 spills).
 #ifndef JX_FUSED8_MIN_WAVES
 #define JX_FUSED8_MIN_WAVES 1
 #endif
 #ifndef JX_FUSED8_ISO_MIN_WAVES
-#define JX_FUSED8_ISO_MIN_WAVES 5
+#define JX_FUSED8_ISO_MIN_WAVES (JX_FUSED_PREFETCH ? 4 : 5)
 #endif
 
 template <class P, int C, int NP, int MODE, int LF>
@@ -77,6 +80,20 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   const int p = rows_tfast<P>() ? tid / TPP : tid % NP;
   const int t = rows_tfast<P>() ? tid % TPP : tid / NP;
   const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
+  // 8-value plans, 6 components: the previous iterate is loaded before the
+  // inverse transform, so that its latency overlaps the spectrum loads and
+  // the transform (256^3: 0.654 against 0.670 ms, at 4 waves per SIMD
+  // instead of 5)
+  constexpr bool PREFETCH = JX_FUSED_PREFETCH && C == 6 && V == 8;
+  double2 prev[PREFETCH ? V : 1];
+  if constexpr (PREFETCH) {
+    const int r = p / 3, c0 = 2 * (p - r * 3);
+    if (r < rb) {
+      const double2 *yr = (const double2 *)(y + (row0 + r) * L * 6 + c0);
+#pragma unroll
+      for (int i = 0; i < V; ++i) prev[i] = yr[(int64_t)reg_out_pos<P>(t, i) * 3];
+    }
+  }
 
   // ---- inverse: half spectra into the pair slots (see k_rows_inverse_reg)
   if constexpr (C == 6) {
@@ -147,7 +164,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
           nv = make_double2(ls_new_rhs(e0, h.x, v[i].x, scale),
                             ls_new_rhs(e1, h.y, v[i].y, scale));
         }
-        double2 o = yr[off];
+        double2 o;
+        if constexpr (PREFETCH) o = prev[i];
+        else o = yr[off];
         double dx = nv.x - o.x, dy = nv.y - o.y;
         acc += dx * dx;
         acc += dy * dy;
