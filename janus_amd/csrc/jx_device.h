@@ -184,6 +184,16 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + slot;
 }
 
+// xcd_tile with each XCD walking its range from the end: a pass that reads
+// what the previous pass wrote last (still in the Infinity Cache) first.
+__device__ __forceinline__ int64_t xcd_tile_rev(int64_t b, int64_t nb) {
+  const int64_t nx = 8;
+  int64_t q = nb / nx, r = nb - q * nx;
+  int64_t x = b - (b / nx) * nx, slot = b / nx;
+  int64_t count = q + (x < r ? 1 : 0);
+  return x * q + (x < r ? x : r) + (count - 1 - slot);
+}
+
 // Stage the twiddle entries a register kernel reads into LDS (none when the
 // global table is used).  The caller synchronizes before the first use.
 template <int L, int NT, bool TQ>
@@ -212,9 +222,11 @@ struct BatchRef {
 
 // Index of the block within its entry (XCD-aware order over the whole
 // launch); *sl is the entry's slot in a batched launch, else nullptr.
+template <bool REV = false>
 __device__ __forceinline__ int64_t batch_block(const BatchRef &br,
                                                const Slot *&sl) {
-  int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  int64_t blk = REV ? xcd_tile_rev(blockIdx.x, gridDim.x)
+                    : xcd_tile(blockIdx.x, gridDim.x);
   sl = nullptr;
   if (br.tab) {
     const int64_t e = blk / br.nbe;

@@ -17,6 +17,15 @@ namespace jx {
 #ifndef JX_FUSED_PREFETCH
 #define JX_FUSED_PREFETCH 1
 #endif
+// Rows in reverse order within each XCD's range (xcd_tile_rev): the fused
+// pass then starts on the slabs the inverse axis-1 pass wrote last, which
+// are still in the 256 MB Infinity Cache, and ends on the slabs the next
+// forward axis-1 pass reads first (256^3: EA 0.651 -> 0.631 ms and B 0.312
+// -> 0.297 ms, 595 -> 607 it/s; 512^3 unchanged, its spectrum being 25
+// times the cache).
+#ifndef JX_FUSED_REVERSE
+#define JX_FUSED_REVERSE 1
+#endif
 #ifndef JX_FUSED_MIN_WAVES
 #define JX_FUSED_MIN_WAVES 3
 #endif
@@ -45,7 +54,9 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
                               BatchRef br) {
   constexpr int L = P::L;
   const Slot *sl;
-  const int64_t blk = batch_block(br, sl);
+  // each XCD walks its rows from the end of its range, where the inverse
+  // axis-1 pass (which walks the same slab range forwards) wrote last
+  const int64_t blk = batch_block<JX_FUSED_REVERSE != 0>(br, sl);
   if (sl) {
     spec = sl->spec;
     y = sl->eps;
