@@ -125,6 +125,14 @@ __device__ __forceinline__ double cg_qp_voxel(const LsArgs &ls, double *q,
   return c0_dot<C>(pv, qv, ls.wd, ls.wt);
 }
 
+// Rows in reverse order within each XCD's range (xcd_tile_rev), as the
+// fused pass: this pass follows the inverse axis-1 pass (256^3: Green apply
+// 1.699 -> 1.690 ms, five-pass iteration 1.858 -> 1.842 ms, CG iteration
+// 3.122 -> 3.111 ms).
+#ifndef JX_ROWS_INV_REVERSE
+#define JX_ROWS_INV_REVERSE 1
+#endif
+
 // Register-FFT variant for the register lengths.  MODE 0: y = c2r(spec);
 // 1: y <- E - c2r(spec) with residual partials; 2: y <- (E + rhs) -
 // c2r(spec) with residual partials; 3 (conjugate gradients): y <- p +
@@ -137,7 +145,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>())
                        LsArgs ls, const SolveState *st, BatchRef br) {
   constexpr int L = P::L;
   const Slot *sl;
-  const int64_t blk = batch_block(br, sl);
+  const int64_t blk = batch_block<JX_ROWS_INV_REVERSE != 0>(br, sl);
   if (sl) {
     spec = sl->spec;
     y = MODE == 3 ? sl->cg_q : sl->eps;
