@@ -356,10 +356,11 @@ __device__ __forceinline__ void load_half_spectra6(const double *spec,
 // t: the stage exchanges through LDS touch consecutive 16-byte slots, free
 // of bank conflicts).  JX_ROWS_TFAST 0: always the first, 1: always the
 // second, 2 (default) -- thread-fast for the mixed-radix plans
-// (80, 96, 160: 20- and 12-value plans, where it measured faster, e.g. the
-// 160-point fused pass 0.224 against 0.284 ms at 160^3), sequence-fast for
-// the power-of-two plans (the 8-value 256- and 512-point passes measured
-// slower thread-fast).
+// (80, 96, 160, where it measured faster: the 160-point fused pass 0.224
+// against 0.284 ms at 160^3 with 20 values, 0.199 against 0.242 ms with 10;
+// the 10-value 80-point one 0.027 against 0.051 ms at 80^3; the 6-value
+// 96-point one within 2%), sequence-fast for the power-of-two plans (the
+// 8-value 256- and 512-point passes measured slower thread-fast).
 #ifndef JX_ROWS_TFAST
 #define JX_ROWS_TFAST 2
 #endif

@@ -9,7 +9,8 @@
 //   16 (16; 16)   32 (16; 16 2)   64 (16; 16 4) | (8; 8 8)
 //   128 (16; 16 8) | (8; 8 4 4)   256 (16; 16 16) | (8; 8 8 4)
 //   512 (16; 16 16 2) | (8; 8 8 8)   1024 (16; 16 16 4)
-//   80 (20; 5 4 4)   96 (12; 6 4 4)   160 (20; 10 4 4)
+//   80 (20; 5 4 4) | (10; 10 2 2 2)   96 (12; 6 4 4) | (6; 6 2 2 2 2)
+//   160 (20; 10 4 4) | (10; 10 2 2 2 2)
 // Variant 1 holds half the values per thread: fewer registers and more
 // waves per SIMD for one more LDS exchange; each pass picks its variant.
 // The first stage reads x[t + TPP (i / R0) + (i mod R0) L / R0] and the last
@@ -23,18 +24,25 @@
 
 namespace jx {
 
-// A plan: length L, V values per thread, n stages of radices r0, r1, r2.
-template <int L_, int V_, int N_, int R0_, int R1_, int R2_>
+// A plan: length L, V values per thread, n stages of radices r0 .. r4
+// (unused ones 1); rl is the last stage's radix.
+template <int L_, int V_, int N_, int R0_, int R1_, int R2_, int R3_ = 1,
+          int R4_ = 1>
 struct Plan {
-  static constexpr int L = L_, V = V_, n = N_, r0 = R0_, r1 = R1_, r2 = R2_;
+  static constexpr int L = L_, V = V_, n = N_, r0 = R0_, r1 = R1_, r2 = R2_,
+                       r3 = R3_, r4 = R4_;
+  static constexpr int radix(int s) {
+    return s == 0 ? r0 : s == 1 ? r1 : s == 2 ? r2 : s == 3 ? r3 : r4;
+  }
+  static constexpr int rl = radix(N_ - 1);
 };
 
 template <int L, int K>
 struct RegPlan;
-#define JX_REG_PLAN(LL, KK, VV, N, R0, R1, R2)                              \
+#define JX_REG_PLAN(LL, KK, VV, N, ...)                                     \
   template <>                                                              \
   struct RegPlan<LL, KK> {                                                 \
-    using type = Plan<LL, VV, N, R0, R1, R2>;                              \
+    using type = Plan<LL, VV, N, __VA_ARGS__>;                             \
   };
 JX_REG_PLAN(16, 0, 16, 1, 16, 1, 1)
 JX_REG_PLAN(32, 0, 16, 2, 16, 2, 1)
@@ -50,6 +58,14 @@ JX_REG_PLAN(1024, 0, 16, 3, 16, 16, 4)
 JX_REG_PLAN(80, 0, 20, 3, 5, 4, 4)
 JX_REG_PLAN(96, 0, 12, 3, 6, 4, 4)
 JX_REG_PLAN(160, 0, 20, 3, 10, 4, 4)
+// Ten values per thread for the lengths with a factor 5: half the registers
+// and half the LDS per wave of the 20-value plans (a 20-value plan holds
+// 20 KB of LDS per wave, which caps its blocks at 6 waves per CU), for two
+// more LDS exchanges of radix 2.
+JX_REG_PLAN(80, 1, 10, 4, 10, 2, 2, 2)
+JX_REG_PLAN(160, 1, 10, 5, 10, 2, 2, 2, 2)
+// Six values per thread for 96 points, the same trade.
+JX_REG_PLAN(96, 1, 6, 5, 6, 2, 2, 2, 2)
 #undef JX_REG_PLAN
 
 // Lengths the register kernels are instantiated for.
@@ -61,7 +77,8 @@ constexpr bool reg_fft_length(int L) {
 
 // Plan variants compiled for a length.
 constexpr int reg_variants(int L) {
-  return (L == 64 || L == 128 || L == 256 || L == 512) ? 2 : 1;
+  return (L == 64 || L == 128 || L == 256 || L == 512 || L == 80 ||
+          L == 96 || L == 160) ? 2 : 1;
 }
 
 // Values held per thread, threads per transform.
@@ -303,7 +320,7 @@ template <class P>
 __device__ __forceinline__ int reg_out_pos(int t, int i) {
   constexpr int L = P::L;
   constexpr int TPP = reg_tpp<P>();
-  constexpr int RL = P::n == 1 ? P::r0 : (P::n == 2 ? P::r1 : P::r2);
+  constexpr int RL = P::rl;
   int q = i / RL, r = i - q * RL;
   return t + TPP * q + r * (L / RL);
 }
@@ -322,23 +339,27 @@ __device__ __forceinline__ int reg_in_pos(int t, int i) {
 // v[i] = X[reg_out_pos(t, i)].  `seq` is this sequence's LDS scratch; the
 // caller's block must reach the barriers inside (every thread of the block
 // calls this function the same number of times).
+// Stages S .. n-1 of a transform; NSP is the product of the radices of the
+// stages before S, whose outputs are in registers.
+template <class P, int S, int NSP, int DIR, bool TQ>
+__device__ __forceinline__ void reg_fft_stages(double2 *v, int t, double2 *seq,
+                                               const double2 *W) {
+  if constexpr (S < P::n) {
+    constexpr int RP = P::radix(S - 1), R = P::radix(S);
+    if constexpr (S >= 2) __syncthreads();
+    reg_store_stage<P, RP, NSP / RP>(v, t, seq);
+    __syncthreads();
+    reg_load_stage<P, R>(v, t, seq);
+    reg_stage<P, R, NSP, DIR, TQ>(v, t, W);
+    reg_fft_stages<P, S + 1, NSP * R, DIR, TQ>(v, t, seq, W);
+  }
+}
+
 template <class P, int DIR, bool TQ>
 __device__ __forceinline__ void reg_fft(double2 *v, int t, double2 *seq,
                                         const double2 *W) {
   reg_stage<P, P::r0, 1, DIR, TQ>(v, t, W);
-  if constexpr (P::n >= 2) {
-    reg_store_stage<P, P::r0, 1>(v, t, seq);
-    __syncthreads();
-    reg_load_stage<P, P::r1>(v, t, seq);
-    reg_stage<P, P::r1, P::r0, DIR, TQ>(v, t, W);
-  }
-  if constexpr (P::n >= 3) {
-    __syncthreads();
-    reg_store_stage<P, P::r1, P::r0>(v, t, seq);
-    __syncthreads();
-    reg_load_stage<P, P::r2>(v, t, seq);
-    reg_stage<P, P::r2, P::r0 * P::r1, DIR, TQ>(v, t, W);
-  }
+  reg_fft_stages<P, 1, P::r0, DIR, TQ>(v, t, seq, W);
 }
 
 // Calls f(P{}) with the plan type of variant k of length L (variant 0 when

@@ -39,6 +39,14 @@ namespace jx {
 #ifndef JX_FUSED8_ISO_MIN_WAVES
 #define JX_FUSED8_ISO_MIN_WAVES (JX_FUSED_PREFETCH ? 4 : 5)
 #endif
+// The 10-value 160-point plan loads the previous iterate early too (below).
+#ifndef JX_FUSED10_PREFETCH
+#define JX_FUSED10_PREFETCH 1
+#endif
+// The other plans (6, 10, 12 and 20 values): the compiler's choice.
+#ifndef JX_FUSEDX_MIN_WAVES
+#define JX_FUSEDX_MIN_WAVES 1
+#endif
 
 template <class P, int C, int NP, int MODE, int LF>
 __global__ void __launch_bounds__(NP * reg_tpp<P>(),
@@ -47,7 +55,7 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
                                       ? (C == 6 && LF == LF_ISO && MODE == 1
                                              ? JX_FUSED8_ISO_MIN_WAVES
                                              : JX_FUSED8_MIN_WAVES)
-                                      : 1)
+                                      : JX_FUSEDX_MIN_WAVES)
     k_rows_update_forward_reg(double *spec, double *y, int64_t nrows,
                               SpecPad lay, double scale, const double2 *W, LsArgs ls,
                               LocalDev loc, const SolveState *st,
@@ -91,11 +99,17 @@ __global__ void __launch_bounds__(NP * reg_tpp<P>(),
   const int p = rows_tfast<P>() ? tid / TPP : tid % NP;
   const int t = rows_tfast<P>() ? tid % TPP : tid / NP;
   const double2 *Wt = stage_twiddles<L, NT, JX_ROWS_TQ>(W, twl);
-  // 8-value plans, 6 components: the previous iterate is loaded before the
-  // inverse transform, so that its latency overlaps the spectrum loads and
-  // the transform (256^3: 0.654 against 0.670 ms, at 4 waves per SIMD
-  // instead of 5)
-  constexpr bool PREFETCH = JX_FUSED_PREFETCH && C == 6 && V == 8;
+  // 8-value plans and the 10-value 160-point plan, 6 components: the
+  // previous iterate is loaded before the inverse transform, so that its
+  // latency overlaps the spectrum loads and the transform (256^3: 0.654
+  // against 0.670 ms, at 4 waves per SIMD instead of 5; 160^3: 0.198
+  // against 0.232 ms, which makes the 10-value plan faster than the
+  // 20-value one's 0.218; with it the 10-value 80-point plan measured
+  // 0.0286 against 0.0264 ms and the 6-value 96-point one 0.0411 against
+  // 0.0387 ms)
+  constexpr bool PREFETCH =
+      C == 6 && ((JX_FUSED_PREFETCH && V == 8) ||
+                 (JX_FUSED10_PREFETCH && V == 10 && L == 160));
   double2 prev[PREFETCH ? V : 1];
   if constexpr (PREFETCH) {
     const int r = p / 3, c0 = 2 * (p - r * 3);

@@ -110,6 +110,30 @@ def test_cg_homogeneous_medium_returns_E():
                                   np.tile(E, (n ** 3, 1)))
 
 
+@pytest.mark.parametrize('n', [(3, 5, 160), (5, 3, 80), (80, 16, 160),
+                               (96, 3, 96)])
+def test_cg_radix2_tail_plans_match_oracle(n):
+    """CG's own last-axis passes on the 10-value plans of the 80- and
+    160-point transforms and the 6-value plan of the 96-point one, against
+    the oracle iterate for iterate."""
+    phase = np.random.default_rng(sum(n) + 1).integers(0, 2, size=n).astype(
+        np.uint8)
+    mat_m = material.create(1.0, 0.3, 3)
+    mat_i = material.create(20.0, 0.25, 3)
+    mat0 = material.create(1.0, 0.3, 3)
+    solver, table = build(phase, [mat_m, mat_i], mat0, 'filtered')
+    for p in solver.green.PLAN_PASSES:
+        solver.green.set_plan(p, 1)
+    E = np.random.default_rng(5).standard_normal(6)
+    r = solver.solve_cg(E, tol=0., max_iter=5)
+    eps_ref, res_ref = jo.cg_scheme(table[phase, 0], table[phase, 1], E,
+                                    'filtered', 1.0, 0.3, tol=0., max_iter=5)
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-9, atol=1e-15)
+    eps = r.eps.cpu().numpy()
+    err = np.abs(eps - eps_ref).max() / np.abs(eps_ref).max()
+    assert err < 1e-12, err
+
+
 def test_cg_plan_variants_agree():
     phase = ms.spheres(64, 5.0, 0.2, seed=9)
     mat_m = material.create(1.0, 0.3, 3)

@@ -65,11 +65,15 @@ def test_apply_vs_oracle(kind, n, h):
 
 @pytest.mark.parametrize('kind', ['truncated', 'filtered', 'willot'])
 @pytest.mark.parametrize('n', [(64, 128, 256), (8, 16, 512), (512, 12),
-                               (128, 64), (256, 10, 64)])
+                               (128, 64), (256, 10, 64), (160, 80, 64),
+                               (16, 160, 80), (16, 80, 160), (160, 80),
+                               (80, 160), (96, 96, 32), (16, 32, 96),
+                               (96, 96)])
 @pytest.mark.parametrize('variant', [0, 1])
 def test_apply_plan_variants_vs_oracle(kind, n, variant):
     """Every pass on plan variant 0 or 1 (8 values per thread for lengths
-    64 ... 512): the same operator up to rounding."""
+    64 ... 512, 10 for 80 and 160, 6 for 96, each length in the Gamma-hat,
+    axis-1 and last-axis passes): the same operator up to rounding."""
     rng = np.random.default_rng(zlib.crc32(repr((kind, n, 'v')).encode()))
     dim = len(n)
     sym = 3 if dim == 2 else 6

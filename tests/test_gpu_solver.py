@@ -286,3 +286,36 @@ def test_eight_value_fused_pass_odd_rows_matches_oracle(n, with_rhs):
     np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-10)
     eps = r.eps.cpu().numpy()
     assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12
+
+
+@pytest.mark.parametrize('n', [(3, 5, 160), (5, 3, 80), (3, 5, 96),
+                               (160, 80, 16), (80, 160, 16), (96, 96, 16)])
+@pytest.mark.parametrize('fused, with_rhs', [(True, False), (True, True),
+                                             (False, False)])
+def test_radix2_tail_plans_match_oracle(n, fused, with_rhs):
+    """The 10-value plans of the 80- and 160-point transforms and the
+    6-value plan of the 96-point one (a radix-10 or radix-6 stage, then
+    three or four of radix 2) in the last-axis, axis-1 and Gamma-hat
+    passes, fused and five-pass, with and without a field right-hand side,
+    against the oracle iterate for iterate."""
+    import torch
+    phase = np.random.default_rng(sum(n)).integers(0, 2, size=n).astype(
+        np.uint8)
+    mats = [material.create(1.0, 0.3, 3), material.create(10.0, 0.25, 3)]
+    mat0 = material.create(5.5, 0.3, 3)
+    solver, table = build(phase, mats, mat0, 'filtered')
+    for p in solver.green.PLAN_PASSES:
+        solver.green.set_plan(p, 1)
+    assert set(solver.green.plans().values()) == {1}
+    solver.fused = fused
+    rng = np.random.default_rng(11)
+    E = rng.standard_normal(6)
+    rhs = rng.standard_normal(n + (6,)) if with_rhs else None
+    r = solver.solve(E, tol=0., max_iter=4,
+                     rhs=torch.from_numpy(rhs).cuda() if with_rhs else None)
+    eps_ref, res_ref = jo.basic_scheme(table[phase, 0], table[phase, 1], E,
+                                       'filtered', 5.5, 0.3, tol=0.,
+                                       max_iter=4, rhs=rhs)
+    np.testing.assert_allclose(r.residuals, res_ref, rtol=1e-10)
+    eps = r.eps.cpu().numpy()
+    assert np.abs(eps - eps_ref).max() / np.abs(eps_ref).max() < 1e-12
