@@ -193,10 +193,20 @@ __device__ __forceinline__ void gamma_point_pair(const GreenDev &G, int axis,
 #define JX_GAMMA_PASS_MIN_WAVES 3
 #endif
 
+// The same for the 6-value 96-point plan (three spilled registers): 0.0374
+// ms at 96^3 against 0.0488 ms at the compiler's 2 waves, which makes it
+// faster than the 12-value plan's 0.0393 ms (the 10-value plans spill
+// 20-36 registers at 3 waves and measured no faster).
+#ifndef JX_GAMMA_PASS_V6_MIN_WAVES
+#define JX_GAMMA_PASS_V6_MIN_WAVES 3
+#endif
+
 template <class P, int C, int NSEQ, int MODE>
 __global__ void __launch_bounds__(NSEQ * reg_tpp<P>(),
                                   MODE == 0 && reg_v<P>() == 16
                                       ? JX_GAMMA_PASS_MIN_WAVES
+                                  : MODE == 0 && reg_v<P>() == 6
+                                      ? JX_GAMMA_PASS_V6_MIN_WAVES
                                       : 1)
     k_axis_reg(const double *in, double *out, AxisArgs a, const double2 *W,
                GreenDev G, const SolveState *st, BatchRef br) {
