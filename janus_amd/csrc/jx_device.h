@@ -355,19 +355,22 @@ __device__ __forceinline__ void load_half_spectra6(const double *spec,
 // (lane = t * sequences + p) or thread-in-sequence fastest (lane = p * TPP +
 // t: the stage exchanges through LDS touch consecutive 16-byte slots, free
 // of bank conflicts).  JX_ROWS_TFAST 0: always the first, 1: always the
-// second, 2 (default) -- thread-fast for the mixed-radix plans
-// (80, 96, 160, where it measured faster: the 160-point fused pass 0.224
-// against 0.284 ms at 160^3 with 20 values, 0.199 against 0.242 ms with 10;
-// the 10-value 80-point one 0.027 against 0.051 ms at 80^3; the 6-value
-// 96-point one within 2%), sequence-fast for the power-of-two plans (the
-// 8-value 256- and 512-point passes measured slower thread-fast).
+// second, 2 (default) -- thread-fast for the mixed-radix plans of 80 and
+// 160 points and the 12-value one of 96 (where it measured faster: the
+// 160-point fused pass 0.224 against 0.284 ms at 160^3 with 20 values,
+// 0.199 against 0.242 ms with 10; the 10-value 80-point one 0.027 against
+// 0.051 ms at 80^3), sequence-fast for the power-of-two plans (the 8-value
+// 256- and 512-point passes measured slower thread-fast) and the 6-value
+// 96-point plan (96^3 fused pass 0.0380 against 0.0387 ms, c2r 0.0211
+// against 0.0226 ms).
 #ifndef JX_ROWS_TFAST
 #define JX_ROWS_TFAST 2
 #endif
 template <class P>
 constexpr bool rows_tfast() {
   return JX_ROWS_TFAST == 1 ||
-         (JX_ROWS_TFAST == 2 && (P::L & (P::L - 1)) != 0);
+         (JX_ROWS_TFAST == 2 && (P::L & (P::L - 1)) != 0 &&
+          !(P::L == 96 && P::V == 6));
 }
 
 // Twiddle source of the last-axis kernels: quarter table in LDS (1) or the
